@@ -1,0 +1,170 @@
+/* ppf_amd.h -- C ABI of the MI355X PPF registration library (libppf_amd.so).
+ *
+ * Drop-in boundary for the point-pair-feature registration path of
+ * nicolasavru/ppf-registration-spatial-hashing.  Plain C types only: host
+ * pointers and sizes in, opaque handles and status codes out.  Every function
+ * returns PPF_OK (0) on success, a negative/positive status otherwise, with a
+ * thread-local message in ppf_last_error().  Host inputs are copied during the
+ * call; nothing is borrowed after return.  Nothing calls exit().
+ *
+ * Clouds are AoS float[n*3] xyz points and float[n*3] normals (the fields of
+ * pcl::PointNormal).  Poses are row-major float[16].
+ *
+ * A context owns one HIP stream on one device; a context and the objects made
+ * from it must not be used from two threads at once.
+ */
+#ifndef PPF_AMD_H
+#define PPF_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define PPF_OK 0
+#define PPF_ERR_ARG 1        /* invalid argument (sizes, NULL pointers, limits) */
+#define PPF_ERR_HIP 2        /* a HIP runtime call failed */
+#define PPF_ERR_NOMEM 3      /* device allocation or HBM budget exhausted */
+#define PPF_ERR_NO_VOTES 4   /* no scene pair matched the model: no pose */
+#define PPF_ERR_STATE 5      /* wrong object state (e.g. result not computed) */
+
+/* The reference encodes votes as [s_r:32 | m_r:26 | alpha:6] (include/model.h:61-63)
+ * and indexes model pairs m_r*n_m + m_i; this build keeps pair indices in 32 bits. */
+#define PPF_MAX_MODEL_POINTS 65535
+
+typedef struct ppf_ctx_s *ppf_ctx;
+typedef struct ppf_model_s *ppf_model;
+typedef struct ppf_scene_s *ppf_scene;
+typedef struct ppf_result_s *ppf_result;
+
+/* Last error message of the calling thread ("" if none).
+ * Replaces HANDLE_ERROR's fprintf+exit (include/impl/util.hpp:18-26). */
+const char *ppf_last_error(void);
+
+/* ------------------------------------------------------------------ context
+ * Device selection as ppf_registration does it: min(numDevices-1, dev)
+ * (src/cuda/ppf.cu:37-49).  hbm_budget_bytes = 0 means no budget. */
+int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out);
+int ppf_ctx_destroy(ppf_ctx ctx);
+/* Bytes of device memory currently held by objects of this context. */
+uint64_t ppf_ctx_device_bytes(ppf_ctx ctx);
+
+/* -------------------------------------------------------------------- model
+ * Model::Model(cloud, d_dist, ...) (include/model.h:17-19, src/cuda/model.cu:43-82):
+ * PPF of every ordered model pair, quantized and FNV-hashed, then the sort-based
+ * parallel hash array (include/impl/parallel_hash_array.hpp:55-77).
+ * The vote-side parameters of the reference constructor are per lookup here
+ * (ppf_lookup_params). */
+int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int64_t n,
+                    float d_dist, ppf_model *out);
+int ppf_model_destroy(ppf_model model);
+
+typedef struct {
+  int64_t n_points;
+  float d_dist;
+  int64_t n_pairs;       /* n*n, diagonal included as in the reference */
+  int64_t n_unique_keys; /* U, key 0 included when present */
+  int32_t tile_points;   /* model points per vote accumulator tile */
+  int32_t n_tiles;
+  uint64_t device_bytes;
+} ppf_model_info_t;
+int ppf_model_info(ppf_model model, ppf_model_info_t *info);
+
+/* Copy the parallel hash array to the host: ParallelHashArray::GetHashkeys,
+ * GetCounts, GetFirstHashkeyIndices, GetHashkeyToDataMap
+ * (include/impl/parallel_hash_array.hpp:94-112).  Arrays sized U, U, U, n*n;
+ * any pointer may be NULL. */
+int ppf_model_hash_array(ppf_model model, uint32_t *hashkeys, uint32_t *counts, uint32_t *first,
+                         uint32_t *key2ppf);
+
+/* Model::SetModelPointVoteWeights (src/cuda/model.cu:91-93): w[m_r] scales the
+ * vote count of every kept vote of model point m_r before GPU clustering
+ * (vote_weight_kernel, src/cuda/kernel.cu:766-782).  Default all 1. */
+int ppf_model_set_weights(ppf_model model, const float *weights, int64_t n);
+
+/* -------------------------------------------------------------------- scene
+ * Scene::Scene(cloud, d_dist, ref_point_downsample_factor) (include/scene.h:15-16,
+ * src/cuda/scene.cu:24-55).  Scene features depend on d_dist, which belongs to
+ * the model, so they are computed inside ppf_lookup; the scene object holds the
+ * points and the local frames of its reference points (idx % rpdf == 0). */
+int ppf_scene_upload(ppf_ctx ctx, const float *points, const float *normals, int64_t n,
+                     uint32_t ref_point_downsample_factor, ppf_scene *out);
+int ppf_scene_destroy(ppf_scene scene);
+
+/* All n*n scene keys for the given d_dist, row-major [s_r][s_i]; 0 for rows
+ * that are not reference points and for the diagonal: Scene::getHashKeys after
+ * ppf_kernel + ppf_hash_kernel (src/cuda/scene.cu:64-99, kernel.cu:404-477).
+ * For inspection of small clouds only (n <= 8192). */
+int ppf_scene_keys(ppf_scene scene, float d_dist, uint32_t *keys_out);
+
+/* ------------------------------------------------------------------- lookup
+ * Model::ppf_lookup(Scene*) (include/model.h:35, src/cuda/model.cu:269-306):
+ * voting, vote threshold, poses, weighted counts, and clustering. */
+typedef struct {
+  float vote_count_threshold;  /* keep votes with count > thr * max (model.cu:164-170) */
+  int32_t cpu_clustering;      /* 1: greedy host clustering (transformation_clustering.cpp) */
+  int32_t use_l1_norm;         /* 1: skip the translation test (kernel.cu:740-746) */
+  int32_t use_averaged_clusters;
+  int32_t exact_votes;         /* 1: evaluate every alpha bin with the full float path */
+} ppf_lookup_params;
+
+int ppf_lookup(ppf_model model, ppf_scene scene, const ppf_lookup_params *params,
+               ppf_result *out);
+int ppf_result_destroy(ppf_result result);
+
+typedef struct {
+  int32_t status;          /* PPF_OK or PPF_ERR_NO_VOTES */
+  int64_t lookups;         /* scene pairs looked up: R*(n_s-1) */
+  int64_t max_count;       /* largest (s_r, m_r, alpha) count */
+  int64_t n_candidates;    /* (code,count) pairs emitted before the final threshold */
+  int64_t n_kept;          /* P: votes kept after the threshold */
+  int64_t max_idx;         /* GPU clustering argmax (model->max_idx), -1 for CPU */
+  int64_t n_clusters;      /* CPU clustering: number of clusters */
+  double vote_ms;          /* device time of the vote kernel (HIP events) */
+  double total_ms;         /* device time of the whole lookup */
+} ppf_result_info_t;
+int ppf_result_info(ppf_result result, ppf_result_info_t *info);
+
+/* Kept votes in reference order (count descending, code ascending):
+ * Model::votes / Model::voteCounts after ComputeUniqueVotes (model.cu:148-170). */
+int ppf_result_votes(ppf_result result, uint64_t *codes, uint32_t *counts);
+/* One row-major 4x4 per kept vote: Model::getTransformations (model.cu:191-200,308-310). */
+int ppf_result_transforms(ppf_result result, float *transforms);
+/* GPU clustering outputs per kept vote: vote_counts_out, transformation_trans,
+ * transformation_rots (model.cu:202-244).  Any pointer may be NULL. */
+int ppf_result_clusters(ppf_result result, float *vote_counts_out, float *translations,
+                        float *quaternions);
+/* Final pose of the (scene, model) pair as ppf_registration extracts it
+ * (src/cuda/ppf.cu:74-93). */
+int ppf_result_pose(ppf_result result, float *pose16);
+
+/* ----------------------------------------------------------- registration
+ * ppf_registration (include/ppf.h:9-15, src/cuda/ppf.cu:29-106): for every
+ * scene and every model, the pose of the model in the scene.
+ * scene_points[i]/scene_normals[i] have scene_sizes[i] points; same for models.
+ * poses_out: [n_scenes][n_models][16]; status_out (optional): [n_scenes][n_models].
+ * model_weights: optional per-model arrays (NULL entries allowed). */
+int ppf_registration(ppf_ctx ctx, int32_t n_scenes, const float *const *scene_points,
+                     const float *const *scene_normals, const int64_t *scene_sizes,
+                     int32_t n_models, const float *const *model_points,
+                     const float *const *model_normals, const int64_t *model_sizes,
+                     const float *model_d_dists, uint32_t ref_point_downsample_factor,
+                     float vote_count_threshold, int32_t cpu_clustering, int32_t use_l1_norm,
+                     int32_t use_averaged_clusters, const float *const *model_weights,
+                     float *poses_out, int32_t *status_out);
+
+/* ------------------------------------------------------------- diagnostics
+ * Evaluate one numerics primitive on the device for n inputs (used to check
+ * that device and host rounding agree).  op: 0 acos, 1 atan2(a,b), 2 sin,
+ * 3 cos, 4 fmod(a,b), 5 quant_downf(a,b), 6 a/b, 7 sqrt(a), 8 alpha_bin as
+ * float(a0,a1,b0,b1) from 4-vectors packed in a (n*2) and b (n*2). */
+int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, float *out,
+                       int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PPF_AMD_H */

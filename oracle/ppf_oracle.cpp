@@ -1,0 +1,941 @@
+// ppf_oracle.cpp -- CPU oracle for the PPF registration path.
+//
+// TEST INFRASTRUCTURE ONLY: the checker used by tests/, the smoke check and
+// bench.py's cpu_baseline.  The product library never links or calls it.
+//
+// A host restatement of the reference algorithm, following its source op for
+// op (citations are to files of nicolasavru/ppf-registration-spatial-hashing):
+//   src/cuda/kernel.cu      device math, features, keys, votes, poses, clustering
+//   include/kernel.h        constants
+//   include/impl/parallel_hash_array.hpp, include/impl/util.hpp   hash array
+//   src/cuda/model.cu, src/cuda/scene.cu, src/cuda/ppf.cu         orchestration
+//   src/transformation_clustering.cpp                             CPU clustering
+//
+// Independence from the product: FNV-1a is written byte-wise through a signed
+// char pointer as in the reference, quant_downf uses glibc's fmodf, the 4x4
+// matrix code is the reference's literal sequence, the hash array uses
+// std::stable_sort / std::lower_bound.  Only acosf/atan2f/sinf/cosf come from
+// the build's ppf_math.h: the reference's GPU libm bits are not reproducible,
+// so both sides must share one implementation (pinned against double-precision
+// libm in tests/test_numerics.py).
+//
+// Deliberate deviations from the reference (each also in the product):
+//   * NaN in a feature is hashed as 0x7fc00000 (reference: platform payload)
+//   * lower_bound past the end is a miss (reference: out-of-bounds read)
+//   * 64-bit indexing (reference: int overflow above ~46k points)
+//   * kernels run for count == 1 (reference: `if(count <= 1) return`)
+//   * use_averaged_clusters reads the pre-clustering translations
+//   * CPU clustering: stable sorts, rotation() taken as the linear part,
+//     rotation angle 2*atan2(|q.vec|, |q.w|)
+#include "ppf_oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../ppf-registration-spatial-hashing_amd/csrc/ppf_math.h"
+
+namespace {
+
+struct float3 {
+  float x, y, z;
+};
+struct float4 {
+  float x, y, z, w;
+};
+
+// --- transcendentals: the build's own, shared with the device code
+inline float acosf_(float x) { return ppfm::acosf_own(x); }
+inline float atan2f_(float y, float x) { return ppfm::atan2f_own(y, x); }
+inline float sinf_(float x) { return ppfm::sinf_own(x); }
+inline float cosf_(float x) { return ppfm::cosf_own(x); }
+
+// --- include/kernel.h:15-17 (CUDART_PI_F = 3.141592654f)
+const float CUDART_PI_F = 3.141592654f;
+const float D_ANGLE0 = (2.0f * float(CUDART_PI_F)) / float(30);
+const float ROT_THRESH = 2 * D_ANGLE0;
+const int N_BINS = 31;  // alpha_idx = lrintf(quant/D_ANGLE0) reaches 30
+
+// --- src/cuda/kernel.cu:23-30
+uint32_t hash(const void *f, int n, uint32_t h = 2166136261u) {
+  const signed char *s = (const signed char *)f;
+  while (n--) {
+    h ^= (uint32_t)(int)*s++;
+    h *= 16777619u;
+  }
+  return h;
+}
+
+// --- src/cuda/vector_ops.cu:80-103
+float3 operator*(float a, float3 v) { return {a * v.x, a * v.y, a * v.z}; }
+float3 operator+(float3 u, float3 v) { return {u.x + v.x, u.y + v.y, u.z + v.z}; }
+float3 operator-(float3 u, float3 v) { return {u.x - v.x, u.y - v.y, u.z - v.z}; }
+
+// --- src/cuda/kernel.cu:51-65, 83-88
+float dot(float3 v1, float3 v2) { return v1.x * v2.x + v1.y * v2.y + v1.z * v2.z; }
+float dot(float4 v1, float4 v2) { return v1.x * v2.x + v1.y * v2.y + v1.z * v2.z + v1.w * v2.w; }
+float norm(float3 v) { return sqrtf(dot(v, v)); }
+float norm(float4 v) { return sqrtf(dot(v, v)); }
+float3 cross(float3 u, float3 v) {
+  return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.z};
+}
+
+// --- src/cuda/kernel.cu:90-107 (glibc fmodf is exact)
+float quant_downf(float x, float y) { return x - fmodf(x, y); }
+float4 disc_feature(float4 f, float d_dist, float d_angle) {
+  f.x = quant_downf(f.x, d_dist);
+  f.y = quant_downf(f.y, d_angle);
+  f.z = quant_downf(f.z, d_angle);
+  f.w = quant_downf(f.w, d_angle);
+  return f;
+}
+float3 discretize(float3 f, float d_dist) {
+  f.x = quant_downf(f.x, d_dist);
+  f.y = quant_downf(f.y, d_dist);
+  f.z = quant_downf(f.z, d_dist);
+  return f;
+}
+
+// --- src/cuda/kernel.cu:109-122
+float4 compute_ppf(float3 p1, float3 n1, float3 p2, float3 n2) {
+  float3 d;
+  d.x = p2.x - p1.x;
+  d.y = p2.y - p1.y;
+  d.z = p2.z - p1.z;
+  float4 f;
+  f.x = norm(d);
+  f.y = acosf_(dot(n1, d) / (norm(n1) * norm(d)));
+  f.z = acosf_(dot(n2, d) / (norm(n2) * norm(d)));
+  f.w = acosf_(dot(n1, n2) / (norm(n1) * norm(n2)));
+  return f;
+}
+
+// --- src/cuda/kernel.cu:460-477 with NaN canonicalised before hashing
+uint32_t feature_key(float4 f) {
+  if (std::isnan(f.x)) return 0;
+  float4 c = f;
+  uint32_t canon = 0x7fc00000u;
+  if (std::isnan(c.y)) std::memcpy(&c.y, &canon, 4);
+  if (std::isnan(c.z)) std::memcpy(&c.z, &canon, 4);
+  if (std::isnan(c.w)) std::memcpy(&c.w, &canon, 4);
+  return hash(&c, sizeof(float4));
+}
+
+// --- src/cuda/kernel.cu:32-49, 124-299
+void zeroMat4(float T[4][4]) {
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) T[i][j] = 0;
+}
+float3 trans(float T[4][4]) { return {T[0][3], T[1][3], T[2][3]}; }
+float4 hrotmat2quat(float T[4][4]) {
+  float t, r;
+  float4 q;
+  t = T[0][0] + T[1][1] + T[2][2];
+  r = sqrtf(1 + t);
+  q.x = 0.5 * r;
+  q.y = copysignf(0.5 * sqrtf(1 + T[0][0] - T[1][1] - T[2][2]), T[2][1] - T[1][2]);
+  q.z = copysignf(0.5 * sqrtf(1 - T[0][0] + T[1][1] - T[2][2]), T[0][2] - T[2][0]);
+  q.w = copysignf(0.5 * sqrtf(1 - T[0][0] - T[1][1] + T[2][2]), T[1][0] - T[0][1]);
+  float n = sqrtf(norm(q));
+  q.x /= n;
+  q.y /= n;
+  q.z /= n;
+  q.w /= n;
+  return q;
+}
+void trans(float3 v, float T[4][4]) {
+  zeroMat4(T);
+  T[0][0] = 1;
+  T[1][1] = 1;
+  T[2][2] = 1;
+  T[3][3] = 1;
+  T[0][3] = v.x;
+  T[1][3] = v.y;
+  T[2][3] = v.z;
+}
+void rotx(float theta, float T[4][4]) {
+  zeroMat4(T);
+  T[0][0] = 1;
+  T[1][1] = cosf_(theta);
+  T[2][1] = sinf_(theta);
+  T[1][2] = -1 * T[2][1];
+  T[2][2] = T[1][1];
+  T[3][3] = 1;
+}
+void roty(float theta, float T[4][4]) {
+  zeroMat4(T);
+  T[0][0] = cosf_(theta);
+  T[0][2] = sinf_(theta);
+  T[1][1] = 1;
+  T[2][0] = -1 * T[0][2];
+  T[2][2] = T[0][0];
+  T[3][3] = 1;
+}
+void rotz(float theta, float T[4][4]) {
+  zeroMat4(T);
+  T[0][0] = cosf_(theta);
+  T[1][0] = sinf_(theta);
+  T[0][1] = -1 * T[1][0];
+  T[1][1] = T[0][0];
+  T[2][2] = 1;
+  T[3][3] = 1;
+}
+void mat4f_mul(const float A[4][4], const float B[4][4], float C[4][4]) {
+  zeroMat4(C);
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++) C[i][j] += A[i][k] * B[k][j];
+}
+float3 mat3f_vmul(const float A[3][3], const float3 b) {
+  float3 c;
+  c.x = dot(float3{A[0][0], A[0][1], A[0][2]}, b);
+  c.y = dot(float3{A[1][0], A[1][1], A[1][2]}, b);
+  c.z = dot(float3{A[2][0], A[2][1], A[2][2]}, b);
+  return c;
+}
+float4 mat4f_vmul(const float A[4][4], const float4 b) {
+  float4 c;
+  c.x = dot(float4{A[0][0], A[0][1], A[0][2], A[0][3]}, b);
+  c.y = dot(float4{A[1][0], A[1][1], A[1][2], A[1][3]}, b);
+  c.z = dot(float4{A[2][0], A[2][1], A[2][2], A[2][3]}, b);
+  c.w = dot(float4{A[3][0], A[3][1], A[3][2], A[3][3]}, b);
+  return c;
+}
+float4 homogenize(float3 v) { return {v.x, v.y, v.z, 1}; }
+float3 dehomogenize(float4 v) { return {v.x, v.y, v.z}; }
+void invht(float T[4][4], float T_inv[4][4]) {
+  T_inv[0][0] = T[0][0];
+  T_inv[0][1] = T[1][0];
+  T_inv[0][2] = T[2][0];
+  T_inv[1][0] = T[0][1];
+  T_inv[1][1] = T[1][1];
+  T_inv[1][2] = T[2][1];
+  T_inv[2][0] = T[0][2];
+  T_inv[2][1] = T[1][2];
+  T_inv[2][2] = T[2][2];
+  float neg_Rtranspose[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) neg_Rtranspose[i][j] = -T_inv[i][j];
+  float3 T_tmp;
+  T_tmp.x = T[0][3];
+  T_tmp.y = T[1][3];
+  T_tmp.z = T[2][3];
+  float3 tmp = mat3f_vmul(neg_Rtranspose, T_tmp);
+  T_inv[0][3] = tmp.x;
+  T_inv[1][3] = tmp.y;
+  T_inv[2][3] = tmp.z;
+  T_inv[3][0] = 0;
+  T_inv[3][1] = 0;
+  T_inv[3][2] = 0;
+  T_inv[3][3] = 1;
+}
+
+// Alpha tail of trans_model_scene (kernel.cu:338-342).
+uint32_t alpha_from_uv(float3 u, float3 v) {
+  u.x = 0;
+  v.x = 0;
+  float alpha = atan2f_(cross(u, v).x, dot(u, v));
+  alpha = quant_downf(alpha + CUDART_PI_F, D_ANGLE0);
+  return (uint32_t)(lrintf(alpha / D_ANGLE0));
+}
+
+// --- src/cuda/kernel.cu:302-349, literal per-vote form (the rotation built
+// after the alpha index, kernel.cu:343-347, does not feed the vote).
+uint32_t trans_model_scene(float3 m_r, float3 n_r_m, float3 m_i, float3 s_r, float3 n_r_s,
+                           float3 s_i) {
+  float transm[4][4], rot_y[4][4], rot_z[4][4], T_tmp[4][4], T_m_g[4][4], T_s_g[4][4];
+  float4 n_tmp;
+  m_r = -1 * m_r;
+  trans(m_r, transm);
+  roty(atan2f_(n_r_m.z, n_r_m.x), rot_y);
+  n_tmp = homogenize(n_r_m);
+  n_tmp = mat4f_vmul(rot_y, n_tmp);
+  rotz(-1 * atan2f_(n_tmp.y, n_tmp.x), rot_z);
+  mat4f_mul(rot_z, rot_y, T_tmp);
+  mat4f_mul(T_tmp, transm, T_m_g);
+
+  s_r = -1 * s_r;
+  trans(s_r, transm);
+  roty(atan2f_(n_r_s.z, n_r_s.x), rot_y);
+  n_tmp = homogenize(n_r_s);
+  n_tmp = mat4f_vmul(rot_y, n_tmp);
+  rotz(-1 * atan2f_(n_tmp.y, n_tmp.x), rot_z);
+  mat4f_mul(rot_z, rot_y, T_tmp);
+  mat4f_mul(T_tmp, transm, T_s_g);
+
+  n_tmp = homogenize(m_i);
+  n_tmp = mat4f_vmul(T_m_g, n_tmp);
+  float3 u = dehomogenize(n_tmp);
+  n_tmp = homogenize(s_i);
+  n_tmp = mat4f_vmul(T_s_g, n_tmp);
+  float3 v = dehomogenize(n_tmp);
+  return alpha_from_uv(u, v);
+}
+
+// --- src/cuda/kernel.cu:352-369
+void compute_rot_angles(float3 n_r_m, float3 n_r_s, float *m_roty, float *m_rotz, float *s_roty,
+                        float *s_rotz) {
+  float rot_y[4][4];
+  float4 n_tmp;
+  *m_roty = atan2f_(n_r_m.z, n_r_m.x);
+  roty(*m_roty, rot_y);
+  n_tmp = homogenize(n_r_m);
+  n_tmp = mat4f_vmul(rot_y, n_tmp);
+  *m_rotz = -1 * atan2f_(n_tmp.y, n_tmp.x);
+  *s_roty = atan2f_(n_r_s.z, n_r_s.x);
+  roty(*s_roty, rot_y);
+  n_tmp = homogenize(n_r_s);
+  n_tmp = mat4f_vmul(rot_y, n_tmp);
+  *s_rotz = -1 * atan2f_(n_tmp.y, n_tmp.x);
+}
+
+// frame T_{x->g} with the angles of compute_rot_angles (kernel.cu:382-395)
+void frame_from_angles(float3 p, float roty_a, float rotz_a, float T[4][4]) {
+  float transm[4][4], rot_y[4][4], rot_z[4][4], T_tmp[4][4];
+  p = -1 * p;
+  trans(p, transm);
+  roty(roty_a, rot_y);
+  rotz(rotz_a, rot_z);
+  mat4f_mul(rot_z, rot_y, T_tmp);
+  mat4f_mul(T_tmp, transm, T);
+}
+
+// --- src/cuda/kernel.cu:372-401
+void compute_transforms(unsigned int angle_idx, float3 m_r, float m_roty, float m_rotz, float3 s_r,
+                        float s_roty, float s_rotz, float *T) {
+  float rot_x[4][4], T_tmp[4][4], T_tmp2[4][4], T_m_g[4][4], T_s_g[4][4];
+  float(*T_arr)[4] = (float(*)[4])T;
+  frame_from_angles(m_r, m_roty, m_rotz, T_m_g);
+  frame_from_angles(s_r, s_roty, s_rotz, T_s_g);
+  rotx(angle_idx * D_ANGLE0 - CUDART_PI_F, rot_x);
+  invht(T_s_g, T_tmp);
+  mat4f_mul(T_tmp, rot_x, T_tmp2);
+  mat4f_mul(T_tmp2, T_m_g, T_arr);
+}
+
+// Frame used by the hoisted vote form: first half of trans_model_scene.
+void frame_to_global(float3 p, float3 n, float T[4][4]) {
+  float transm[4][4], rot_y[4][4], rot_z[4][4], T_tmp[4][4];
+  float4 n_tmp;
+  p = -1 * p;
+  trans(p, transm);
+  roty(atan2f_(n.z, n.x), rot_y);
+  n_tmp = homogenize(n);
+  n_tmp = mat4f_vmul(rot_y, n_tmp);
+  rotz(-1 * atan2f_(n_tmp.y, n_tmp.x), rot_z);
+  mat4f_mul(rot_z, rot_y, T_tmp);
+  mat4f_mul(T_tmp, transm, T);
+}
+
+float3 pt(const float *a, int64_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+
+// --- parallel_hash_array.hpp:55-77 + util.hpp:30-52
+struct HashArray {
+  std::vector<uint32_t> hashkeys;
+  std::vector<uint64_t> counts, first, key2data;
+};
+HashArray build_hash_array(const uint32_t *data, int64_t n) {
+  HashArray h;
+  h.key2data.resize(n);
+  std::iota(h.key2data.begin(), h.key2data.end(), (uint64_t)0);
+  std::stable_sort(h.key2data.begin(), h.key2data.end(),
+                   [&](uint64_t a, uint64_t b) { return data[a] < data[b]; });
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t k = data[h.key2data[i]];
+    if (i == 0 || k != h.hashkeys.back()) {
+      h.hashkeys.push_back(k);
+      h.counts.push_back(1);
+    } else {
+      h.counts.back()++;
+    }
+  }
+  h.first.resize(h.hashkeys.size());
+  uint64_t acc = 0;
+  for (size_t u = 0; u < h.hashkeys.size(); ++u) {
+    h.first[u] = acc;
+    acc += h.counts[u];
+  }
+  return h;
+}
+// lower_bound + equality (kernel.cu:491-492); past-the-end is a miss.
+int64_t find_key(const HashArray &h, uint32_t key) {
+  auto it = std::lower_bound(h.hashkeys.begin(), h.hashkeys.end(), key);
+  if (it == h.hashkeys.end() || *it != key) return -1;
+  return it - h.hashkeys.begin();
+}
+
+// Keys of one cloud, scene.cu:64-99 + model.cu:69-75 (rpdf = 1 for models).
+void cloud_keys(const float *pts, const float *nrm, int64_t n, int rpdf, float d_dist,
+                uint32_t *keys) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t idx = 0; idx < n; ++idx) {
+    for (int64_t j = 0; j < n; ++j) {
+      uint32_t k = 0;
+      if (idx % rpdf == 0 && j != idx) {
+        float4 f = compute_ppf(pt(pts, idx), pt(nrm, idx), pt(pts, j), pt(nrm, j));
+        k = feature_key(disc_feature(f, d_dist, D_ANGLE0));
+      }
+      keys[idx * n + j] = k;
+    }
+  }
+}
+
+struct ModelDesc {
+  int64_t n;
+  HashArray ha;
+};
+ModelDesc build_model(const float *pts, const float *nrm, int64_t n, float d_dist) {
+  std::vector<uint32_t> keys((size_t)(n * n));
+  cloud_keys(pts, nrm, n, 1, d_dist, keys.data());
+  return {n, build_hash_array(keys.data(), n * n)};
+}
+
+struct VoteEntry {
+  uint64_t code;
+  uint32_t count;
+};
+
+// Streaming form of ComputeUniqueVotes (model.cu:95-151): for each scene
+// reference point a dense [m_r][alpha] accumulator replaces the materialised
+// vote list + sort + histogram; bins come out in ascending code order.
+void vote_histogram(const ModelDesc &md, const float *m_pts, const float *m_nrm, const float *s_pts,
+                    const float *s_nrm, int64_t n_s, float d_dist, int mode,
+                    const std::vector<int64_t> &refs, std::vector<VoteEntry> &out,
+                    int64_t *lookups, int64_t *hits, int64_t *votes) {
+  const int64_t n_m = md.n;
+  std::vector<float> Tm;
+  if (mode == 1) {
+    Tm.resize((size_t)n_m * 16);
+    for (int64_t i = 0; i < n_m; ++i)
+      frame_to_global(pt(m_pts, i), pt(m_nrm, i), (float(*)[4]) & Tm[16 * i]);
+  }
+  std::vector<std::vector<VoteEntry>> per_ref(refs.size());
+  int64_t L = 0, H = 0, V = 0;
+#pragma omp parallel reduction(+ : L, H, V)
+  {
+    std::vector<uint32_t> acc((size_t)n_m * N_BINS);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t r = 0; r < (int64_t)refs.size(); ++r) {
+      int64_t s_r = refs[r];
+      std::fill(acc.begin(), acc.end(), 0u);
+      float3 sr = pt(s_pts, s_r), nsr = pt(s_nrm, s_r);
+      float Ts[4][4];
+      if (mode == 1) frame_to_global(sr, nsr, Ts);
+      for (int64_t s_i = 0; s_i < n_s; ++s_i) {
+        if (s_i == s_r) continue;
+        ++L;
+        float3 si = pt(s_pts, s_i);
+        float4 f = compute_ppf(sr, nsr, si, pt(s_nrm, s_i));
+        uint32_t key = feature_key(disc_feature(f, d_dist, D_ANGLE0));
+        if (key == 0) continue;
+        int64_t u = find_key(md.ha, key);
+        if (u < 0) continue;
+        ++H;
+        uint64_t cnt = md.ha.counts[u], first = md.ha.first[u];
+        float3 v{};
+        if (mode == 1) v = dehomogenize(mat4f_vmul(Ts, homogenize(si)));
+        for (uint64_t i = 0; i < cnt; ++i) {
+          uint64_t pair = md.ha.key2data[first + i];
+          uint64_t m_r = pair / (uint64_t)n_m;
+          uint64_t m_i = pair - m_r * (uint64_t)n_m;
+          uint32_t a;
+          if (mode == 0) {
+            a = trans_model_scene(pt(m_pts, m_r), pt(m_nrm, m_r), pt(m_pts, m_i), sr, nsr, si);
+          } else {
+            float3 uu =
+                dehomogenize(mat4f_vmul((float(*)[4]) & Tm[16 * m_r], homogenize(pt(m_pts, m_i))));
+            a = alpha_from_uv(uu, v);
+          }
+          acc[m_r * N_BINS + a]++;
+          ++V;
+        }
+      }
+      auto &o = per_ref[r];
+      for (int64_t m = 0; m < n_m; ++m)
+        for (int a = 0; a < N_BINS; ++a) {
+          uint32_t c = acc[m * N_BINS + a];
+          if (c) o.push_back({((uint64_t)s_r << 32) | (uint64_t)(((uint32_t)m << 6) | (uint32_t)a), c});
+        }
+    }
+  }
+  // refs ascending -> codes ascending
+  std::vector<int64_t> order(refs.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return refs[a] < refs[b]; });
+  for (int64_t r : order) out.insert(out.end(), per_ref[r].begin(), per_ref[r].end());
+  *lookups = L;
+  *hits = H;
+  *votes = V;
+}
+
+// model.cu:155-170: stable sort by count (compared as float, descending), keep
+// counts > vote_count_threshold * max.
+void threshold_votes(std::vector<VoteEntry> &e, float vote_count_threshold, uint32_t *maxc) {
+  std::stable_sort(e.begin(), e.end(),
+                   [](const VoteEntry &a, const VoteEntry &b) { return (float)a.count > (float)b.count; });
+  if (e.empty()) {
+    *maxc = 0;
+    return;
+  }
+  *maxc = e[0].count;
+  float min_votecount = vote_count_threshold * e[0].count;
+  size_t keep = 0;
+  for (auto &x : e)
+    if (x.count > min_votecount) ++keep;
+  e.resize(keep);
+}
+
+std::vector<int64_t> default_refs(int64_t n_s, int rpdf) {
+  std::vector<int64_t> r;
+  for (int64_t i = 0; i < n_s; i += rpdf) r.push_back(i);
+  return r;
+}
+
+// kernel.cu:605-645 (per kept vote)
+void pose_of_code(uint64_t vote, const float *m_pts, const float *m_nrm, const float *s_pts,
+                  const float *s_nrm, float *T) {
+  unsigned int low6 = ((unsigned int)-1) >> 26;
+  unsigned int scene_point_idx = (unsigned int)(vote >> 32);
+  unsigned int modelanglecode = (unsigned int)(vote & 0xffffffffu);
+  unsigned int model_point_idx = modelanglecode >> 6;
+  unsigned int angle_idx = modelanglecode & low6;
+  for (int i = 0; i < 16; ++i) T[i] = 0;
+  if (scene_point_idx == 0 && model_point_idx == 0 && angle_idx == 0) return;
+  float m_roty, m_rotz, s_roty, s_rotz;
+  compute_rot_angles(pt(m_nrm, model_point_idx), pt(s_nrm, scene_point_idx), &m_roty, &m_rotz,
+                     &s_roty, &s_rotz);
+  compute_transforms(angle_idx, pt(m_pts, model_point_idx), m_roty, m_rotz,
+                     pt(s_pts, scene_point_idx), s_roty, s_rotz, T);
+}
+
+// model.cu:202-244, kernel.cu:647-763
+void cluster_spatial(const float *T, const float *w, int64_t P, float d_dist, bool use_l1,
+                     bool use_avg, float *vc_out, float *trans_out, float *quat_out,
+                     uint32_t *hash_out, int64_t *max_idx) {
+  std::vector<float3> tr(P);
+  std::vector<float4> q(P);
+  std::vector<uint32_t> th(P), adj((size_t)P * 27);
+  for (int64_t i = 0; i < P; ++i) {
+    float M[4][4];
+    std::memcpy(M, T + 16 * i, sizeof(M));
+    tr[i] = trans(M);
+    q[i] = hrotmat2quat(M);
+    float3 disc = discretize(tr[i], d_dist);
+    int32_t c[3] = {ppfm::f2i_trunc(disc.x / d_dist), ppfm::f2i_trunc(disc.y / d_dist),
+                    ppfm::f2i_trunc(disc.z / d_dist)};
+    th[i] = hash(c, 12);
+    int cc = 0;
+    for (int a = -1; a < 2; a++)
+      for (int b = -1; b < 2; b++)
+        for (int k = -1; k < 2; k++, cc++) {
+          if (a == 0 && b == 0 && k == 0) {
+            adj[27 * i + cc] = 0;
+            continue;
+          }
+          int32_t nb[3] = {c[0] + a, c[1] + b, c[2] + k};
+          adj[27 * i + cc] = hash(nb, 12);
+        }
+  }
+  HashArray ha = build_hash_array(th.data(), P);
+  float rot_thresh_sq = ROT_THRESH * ROT_THRESH;
+  for (int64_t idx = 0; idx < P; ++idx) {
+    float3 thisTrans = tr[idx];
+    float4 thisQuat = q[idx];
+    float vote_count_out = 1;
+    float3 thisTrans_out = thisTrans;
+    for (int blk = 0; blk < 27; ++blk) {
+      uint32_t h = adj[27 * idx + blk];
+      if (h == 0) continue;
+      int64_t u = find_key(ha, h);
+      if (u < 0) continue;
+      for (uint64_t j = 0; j < ha.counts[u]; ++j) {
+        uint64_t ti = ha.key2data[ha.first[u] + j];
+        float thisVoteCount = w[ti];
+        float quatDiff = fabsf(8 * (1 - dot(thisQuat, q[ti])));
+        if (quatDiff < rot_thresh_sq) {
+          if (!use_l1) {
+            float normDiffTrans = norm(thisTrans - tr[ti]);
+            if (!(normDiffTrans < d_dist)) continue;
+          }
+          if (use_avg) {
+            thisTrans_out = vote_count_out * thisTrans_out;
+            thisTrans_out = thisTrans_out + w[ti] * tr[ti];
+            thisTrans_out = (1 / (vote_count_out + thisVoteCount)) * thisTrans_out;
+          }
+          vote_count_out += thisVoteCount;
+        }
+      }
+    }
+    vc_out[idx] = vote_count_out;
+    trans_out[3 * idx] = thisTrans_out.x;
+    trans_out[3 * idx + 1] = thisTrans_out.y;
+    trans_out[3 * idx + 2] = thisTrans_out.z;
+    if (quat_out) {
+      quat_out[4 * idx] = thisQuat.x;
+      quat_out[4 * idx + 1] = thisQuat.y;
+      quat_out[4 * idx + 2] = thisQuat.z;
+      quat_out[4 * idx + 3] = thisQuat.w;
+    }
+    if (hash_out) hash_out[idx] = th[idx];
+  }
+  // thrust::max_element: first maximum
+  int64_t best = 0;
+  for (int64_t i = 1; i < P; ++i)
+    if (vc_out[best] < vc_out[i]) best = i;
+  *max_idx = P > 0 ? best : -1;
+}
+
+// ---- CPU clustering (transformation_clustering.cpp:62-137) without Eigen.
+struct Quat {
+  float x, y, z, w;
+};
+// Rotation matrix -> quaternion, Shepperd's method (trace / largest diagonal).
+Quat quat_from_rot(const float R[3][3]) {
+  Quat q;
+  float t = R[0][0] + R[1][1] + R[2][2];
+  if (t > 0) {
+    t = sqrtf(t + 1.0f);
+    q.w = 0.5f * t;
+    t = 0.5f / t;
+    q.x = (R[2][1] - R[1][2]) * t;
+    q.y = (R[0][2] - R[2][0]) * t;
+    q.z = (R[1][0] - R[0][1]) * t;
+  } else {
+    int i = 0;
+    if (R[1][1] > R[0][0]) i = 1;
+    if (R[2][2] > R[i][i]) i = 2;
+    int j = (i + 1) % 3, k = (j + 1) % 3;
+    float c[3];
+    t = sqrtf(R[i][i] - R[j][j] - R[k][k] + 1.0f);
+    c[i] = 0.5f * t;
+    t = 0.5f / t;
+    q.w = (R[k][j] - R[j][k]) * t;
+    c[j] = (R[j][i] + R[i][j]) * t;
+    c[k] = (R[k][i] + R[i][k]) * t;
+    q.x = c[0];
+    q.y = c[1];
+    q.z = c[2];
+  }
+  return q;
+}
+float rot_angle_between(const float A[3][3], const float B[3][3]) {
+  float R[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = A[0][i] * B[0][j] + A[1][i] * B[1][j] + A[2][i] * B[2][j];
+  Quat q = quat_from_rot(R);
+  float n = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+  return 2.0f * ppfm::atan2f_own(n, fabsf(q.w));
+}
+struct PoseV {
+  float R[3][3];
+  float t[3];
+  uint32_t votes;
+};
+bool within(const PoseV &a, const PoseV &b, float tt, float rt) {
+  float dx = a.t[0] - b.t[0], dy = a.t[1] - b.t[1], dz = a.t[2] - b.t[2];
+  float pd = sqrtf(dx * dx + dy * dy + dz * dz);
+  float ang = fabsf(rot_angle_between(a.R, b.R));
+  return pd < tt && ang < rt;
+}
+void cluster_greedy(const float *T, const uint32_t *counts, int64_t P, float tt, float rt,
+                    std::vector<PoseV> &result, int64_t *n_clusters) {
+  std::vector<PoseV> poses(P);
+  for (int64_t i = 0; i < P; ++i) {
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) poses[i].R[r][c] = T[16 * i + 4 * r + c];
+      poses[i].t[r] = T[16 * i + 4 * r + 3];
+    }
+    poses[i].votes = counts[i];
+  }
+  std::stable_sort(poses.begin(), poses.end(),
+                   [](const PoseV &a, const PoseV &b) { return a.votes > b.votes; });
+  std::vector<std::vector<int64_t>> clusters;
+  std::vector<std::pair<size_t, uint32_t>> cv;
+  for (int64_t i = 0; i < P; ++i) {
+    bool found = false;
+    for (size_t c = 0; c < clusters.size(); ++c) {
+      if (within(poses[i], poses[clusters[c].front()], tt, rt)) {
+        found = true;
+        clusters[c].push_back(i);
+        cv[c].second += poses[i].votes;
+        break;
+      }
+    }
+    if (!found) {
+      clusters.push_back({i});
+      cv.push_back({clusters.size() - 1, poses[i].votes});
+    }
+  }
+  std::stable_sort(cv.begin(), cv.end(), [](const std::pair<size_t, uint32_t> &a,
+                                            const std::pair<size_t, uint32_t> &b) {
+    return a.second > b.second;
+  });
+  *n_clusters = (int64_t)clusters.size();
+  result.clear();
+  size_t mc = clusters.size() < 3 ? clusters.size() : 3;
+  for (size_t ci = 0; ci < mc; ++ci) {
+    const auto &cl = clusters[cv[ci].first];
+    float ta[3] = {0, 0, 0}, qa[4] = {0, 0, 0, 0};
+    for (int64_t pi : cl) {
+      for (int r = 0; r < 3; ++r) ta[r] += poses[pi].t[r];
+      Quat q = quat_from_rot(poses[pi].R);
+      qa[0] += q.x;
+      qa[1] += q.y;
+      qa[2] += q.z;
+      qa[3] += q.w;
+    }
+    float sz = (float)cl.size();
+    for (int r = 0; r < 3; ++r) ta[r] /= sz;
+    for (int r = 0; r < 4; ++r) qa[r] /= sz;
+    float qn = sqrtf(qa[0] * qa[0] + qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]);
+    float x = qa[0] / qn, y = qa[1] / qn, z = qa[2] / qn, w = qa[3] / qn;
+    float tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    float twx = tx * w, twy = ty * w, twz = tz * w;
+    float txx = tx * x, txy = ty * x, txz = tz * x;
+    float tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    PoseV p;
+    p.R[0][0] = 1 - (tyy + tzz);
+    p.R[0][1] = txy - twz;
+    p.R[0][2] = txz + twy;
+    p.R[1][0] = txy + twz;
+    p.R[1][1] = 1 - (txx + tzz);
+    p.R[1][2] = tyz - twx;
+    p.R[2][0] = txz - twy;
+    p.R[2][1] = tyz + twx;
+    p.R[2][2] = 1 - (txx + tyy);
+    for (int r = 0; r < 3; ++r) p.t[r] = ta[r];
+    p.votes = cv[ci].second;
+    result.push_back(p);
+  }
+}
+
+void pose_to_mat(const PoseV &p, float *M) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) M[4 * r + c] = p.R[r][c];
+    M[4 * r + 3] = p.t[r];
+  }
+  M[12] = M[13] = M[14] = 0;
+  M[15] = 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t oracle_fnv1a(const uint8_t *bytes, int n) { return hash(bytes, n); }
+
+float oracle_quant_downf(float x, float y) { return quant_downf(x, y); }
+
+void oracle_compute_ppf(const float *p1, const float *n1, const float *p2, const float *n2,
+                        float *f_raw4, float *f_quant4, float d_dist) {
+  float4 f = compute_ppf(pt(p1, 0), pt(n1, 0), pt(p2, 0), pt(n2, 0));
+  float4 q = disc_feature(f, d_dist, D_ANGLE0);
+  std::memcpy(f_raw4, &f, 16);
+  std::memcpy(f_quant4, &q, 16);
+}
+
+uint32_t oracle_feature_key(const float *f4) {
+  float4 f;
+  std::memcpy(&f, f4, 16);
+  return feature_key(f);
+}
+
+int oracle_cloud_keys(const float *pts, const float *nrm, int64_t n, int rpdf, float d_dist,
+                      uint32_t *keys_out) {
+  if (n < 0 || rpdf < 1) return 1;
+  cloud_keys(pts, nrm, n, rpdf, d_dist, keys_out);
+  return 0;
+}
+
+int oracle_hash_array(const uint32_t *keys, int64_t n, uint32_t *hashkeys_out,
+                      uint64_t *counts_out, uint64_t *first_out, uint64_t *key2data_out,
+                      int64_t *n_unique) {
+  HashArray h = build_hash_array(keys, n);
+  std::copy(h.hashkeys.begin(), h.hashkeys.end(), hashkeys_out);
+  std::copy(h.counts.begin(), h.counts.end(), counts_out);
+  std::copy(h.first.begin(), h.first.end(), first_out);
+  std::copy(h.key2data.begin(), h.key2data.end(), key2data_out);
+  *n_unique = (int64_t)h.hashkeys.size();
+  return 0;
+}
+
+uint32_t oracle_alpha_literal(const float *m_r, const float *n_r_m, const float *m_i,
+                              const float *s_r, const float *n_r_s, const float *s_i) {
+  return trans_model_scene(pt(m_r, 0), pt(n_r_m, 0), pt(m_i, 0), pt(s_r, 0), pt(n_r_s, 0),
+                           pt(s_i, 0));
+}
+
+int oracle_vote(const float *m_pts, const float *m_nrm, int64_t n_m, const float *s_pts,
+                const float *s_nrm, int64_t n_s, float d_dist, int rpdf,
+                float vote_count_threshold, int mode, const int64_t *ref_list, int64_t n_ref,
+                uint64_t *codes_out, uint32_t *counts_out, int64_t cap, int64_t *n_out,
+                int64_t *stats) {
+  if (n_m < 1 || n_s < 1 || rpdf < 1 || n_m >= 65536) return 1;
+  ModelDesc md = build_model(m_pts, m_nrm, n_m, d_dist);
+  std::vector<int64_t> refs =
+      ref_list ? std::vector<int64_t>(ref_list, ref_list + n_ref) : default_refs(n_s, rpdf);
+  std::vector<VoteEntry> e;
+  int64_t L, H, V;
+  vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, mode, refs, e, &L, &H, &V);
+  int64_t U = (int64_t)e.size();
+  uint32_t maxc = 0;
+  threshold_votes(e, vote_count_threshold, &maxc);
+  stats[0] = L;
+  stats[1] = H;
+  stats[2] = V;
+  stats[3] = U;
+  stats[4] = maxc;
+  stats[5] = (int64_t)e.size();
+  *n_out = (int64_t)e.size();
+  if ((int64_t)e.size() > cap) return 2;
+  for (size_t i = 0; i < e.size(); ++i) {
+    codes_out[i] = e[i].code;
+    counts_out[i] = e[i].count;
+  }
+  return 0;
+}
+
+int oracle_poses(const uint64_t *codes, int64_t P, const float *m_pts, const float *m_nrm,
+                 const float *s_pts, const float *s_nrm, float *T_out) {
+  for (int64_t i = 0; i < P; ++i) pose_of_code(codes[i], m_pts, m_nrm, s_pts, s_nrm, T_out + 16 * i);
+  return 0;
+}
+
+int oracle_cluster_spatial(const float *T, const uint64_t *codes, const uint32_t *counts,
+                           const float *model_weights, int64_t P, float d_dist,
+                           int use_l1_norm, int use_averaged_clusters, float *vc_out,
+                           float *trans_out, float *quat_out, uint32_t *hash_out,
+                           int64_t *max_idx) {
+  // kernel.cu:766-782: weighted count = w[m_r] * count
+  std::vector<float> w(P);
+  for (int64_t i = 0; i < P; ++i) {
+    uint32_t m_r = ((uint32_t)(codes[i] & 0xffffffffu)) >> 6;
+    float wm = model_weights ? model_weights[m_r] : 1.0f;
+    w[i] = wm * counts[i];
+  }
+  cluster_spatial(T, w.data(), P, d_dist, use_l1_norm != 0, use_averaged_clusters != 0, vc_out,
+                  trans_out, quat_out, hash_out, max_idx);
+  return 0;
+}
+
+int oracle_cluster_greedy(const float *T, const uint32_t *counts, int64_t P, float trans_thresh,
+                          float rot_thresh, float *poses_out, uint32_t *votes_out,
+                          int64_t *n_clusters_out, int64_t *n_result) {
+  std::vector<PoseV> res;
+  cluster_greedy(T, counts, P, trans_thresh, rot_thresh, res, n_clusters_out);
+  for (size_t i = 0; i < res.size(); ++i) {
+    pose_to_mat(res[i], poses_out + 16 * i);
+    votes_out[i] = res[i].votes;
+  }
+  *n_result = (int64_t)res.size();
+  return 0;
+}
+
+int oracle_register(const float *m_pts, const float *m_nrm, int64_t n_m, const float *s_pts,
+                    const float *s_nrm, int64_t n_s, float d_dist, int rpdf,
+                    float vote_count_threshold, int cpu_clustering, int use_l1_norm,
+                    int use_averaged_clusters, float *pose_out, int64_t *stats) {
+  for (int i = 0; i < 16; ++i) pose_out[i] = 0;
+  if (n_m < 1 || n_s < 1 || rpdf < 1 || n_m >= 65536) return 1;
+  ModelDesc md = build_model(m_pts, m_nrm, n_m, d_dist);
+  std::vector<int64_t> refs = default_refs(n_s, rpdf);
+  std::vector<VoteEntry> e;
+  int64_t L, H, V;
+  vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, 1, refs, e, &L, &H, &V);
+  int64_t U = (int64_t)e.size();
+  uint32_t maxc = 0;
+  threshold_votes(e, vote_count_threshold, &maxc);
+  int64_t P = (int64_t)e.size();
+  stats[0] = L;
+  stats[1] = H;
+  stats[2] = V;
+  stats[3] = U;
+  stats[4] = maxc;
+  stats[5] = P;
+  stats[6] = -1;
+  if (P == 0) return 3;
+  std::vector<float> T((size_t)P * 16);
+  std::vector<uint64_t> codes(P);
+  std::vector<uint32_t> counts(P);
+  for (int64_t i = 0; i < P; ++i) {
+    codes[i] = e[i].code;
+    counts[i] = e[i].count;
+    pose_of_code(codes[i], m_pts, m_nrm, s_pts, s_nrm, &T[16 * i]);
+  }
+  if (cpu_clustering) {
+    // model.cu:246-266 and ppf.cu:75-77: result[0] of clusterPoses
+    std::vector<PoseV> res;
+    int64_t nc;
+    cluster_greedy(T.data(), counts.data(), P, d_dist, D_ANGLE0, res, &nc);
+    pose_to_mat(res[0], pose_out);
+    stats[6] = nc;
+  } else {
+    // model.cu:291-296 and ppf.cu:78-93
+    std::vector<float> vc(P), tr((size_t)P * 3);
+    std::vector<float> w(P);
+    for (int64_t i = 0; i < P; ++i) w[i] = 1.0f * counts[i];
+    int64_t mi;
+    cluster_spatial(T.data(), w.data(), P, d_dist, use_l1_norm != 0, use_averaged_clusters != 0,
+                    vc.data(), tr.data(), nullptr, nullptr, &mi);
+    std::memcpy(pose_out, &T[16 * mi], 64);
+    pose_out[3] = tr[3 * mi];
+    pose_out[7] = tr[3 * mi + 1];
+    pose_out[11] = tr[3 * mi + 2];
+    stats[6] = mi;
+  }
+  return 0;
+}
+
+int oracle_vote_throughput(const float *m_pts, const float *m_nrm, int64_t n_m,
+                           const float *s_pts, const float *s_nrm, int64_t n_s, float d_dist,
+                           const int64_t *ref_list, int64_t n_ref, int64_t *stats,
+                           double *seconds) {
+  if (n_m < 1 || n_s < 1 || n_m >= 65536) return 1;
+  ModelDesc md = build_model(m_pts, m_nrm, n_m, d_dist);
+  std::vector<int64_t> refs(ref_list, ref_list + n_ref);
+  std::vector<VoteEntry> e;
+  int64_t L, H, V;
+  auto t0 = std::chrono::steady_clock::now();
+  vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, 1, refs, e, &L, &H, &V);
+  auto t1 = std::chrono::steady_clock::now();
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  stats[0] = L;
+  stats[1] = H;
+  stats[2] = V;
+  stats[3] = (int64_t)e.size();
+  return 0;
+}
+
+int oracle_num_threads(void) { return omp_get_max_threads(); }
+
+// Host evaluation of the same primitives as ppf_debug_numerics.  fmod,
+// quant_downf and the alpha bin use the oracle's own restatement (glibc fmodf,
+// lrintf); the transcendentals are the shared ones.
+int oracle_numerics(int op, const float *a, const float *b, float *out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    float r = 0.0f;
+    switch (op) {
+      case 0: r = acosf_(a[i]); break;
+      case 1: r = atan2f_(a[i], b[i]); break;
+      case 2: r = sinf_(a[i]); break;
+      case 3: r = cosf_(a[i]); break;
+      case 4: r = fmodf(a[i], b[i]); break;
+      case 5: r = quant_downf(a[i], b[i]); break;
+      case 6: r = a[i] / b[i]; break;
+      case 7: r = sqrtf(a[i]); break;
+      case 8: r = (float)alpha_from_uv(float3{0.0f, a[2 * i], a[2 * i + 1]},
+                                       float3{0.0f, b[2 * i], b[2 * i + 1]});
+        break;
+      default: return 1;
+    }
+    out[i] = r;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,718 @@
+// api.hip -- the C ABI of libppf_amd.so (include/ppf_amd.h): objects, stage
+// orchestration of Model::ppf_lookup and ppf_registration, and the host greedy
+// pose clustering of the reference's --cpu_clustering path.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <utility>
+
+#include "ppf_geom.h"
+#include "ppf_objects.h"
+
+using namespace ppfi;
+
+// ------------------------------------------------------------------ errors
+namespace ppfi {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int DevBuf::ensure(size_t n) {
+  if (n <= bytes && ptr) return 0;
+  release();
+  size_t want = std::max<size_t>(n, 256);
+  hipError_t e = hipMalloc(&ptr, want);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+    return PPF_ERR_NOMEM;
+  }
+  bytes = want;
+  return 0;
+}
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+}  // namespace ppfi
+
+int TBuf::alloc(ppf_ctx c, size_t n) {
+  release();
+  ctx = c;
+  size_t want = std::max<size_t>(n, 256);
+  if (c->budget && c->bytes.load() + want > c->budget) {
+    set_error("device memory budget exceeded (" + std::to_string(c->bytes.load() + want) + " > " +
+              std::to_string(c->budget) + " bytes)");
+    return PPF_ERR_NOMEM;
+  }
+  hipError_t e = hipMalloc(&ptr, want);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+    return PPF_ERR_NOMEM;
+  }
+  bytes = want;
+  c->bytes += want;
+  return 0;
+}
+void TBuf::release() {
+  if (ptr) {
+    (void)hipFree(ptr);
+    if (ctx) ctx->bytes -= bytes;
+  }
+  ptr = nullptr;
+  bytes = 0;
+}
+
+#define ARG_CHECK(cond, msg)  \
+  do {                        \
+    if (!(cond)) {            \
+      set_error(msg);         \
+      return PPF_ERR_ARG;     \
+    }                         \
+  } while (0)
+
+#define HIPC(expr)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+      return PPF_ERR_HIP;                                                                  \
+    }                                                                                      \
+  } while (0)
+#define TRY(expr)                         \
+  do {                                    \
+    int r_ = (expr);                      \
+    if (r_ != 0) return r_;               \
+  } while (0)
+
+namespace {
+
+int set_device(ppf_ctx c) {
+  HIPC(hipSetDevice(c->device));
+  return 0;
+}
+
+// Copy an AoS host cloud to the device and build SoA + frames.
+int upload_cloud(ppf_ctx ctx, const float *pts, const float *nrm, int64_t n, CloudDev &cloud,
+                 TBuf *frames, uint32_t rpdf, int64_t n_frames) {
+  cloud.n = n;
+  TRY(cloud.soa.alloc(ctx, (size_t)n * 7 * 4));
+  if (frames) TRY(frames->alloc(ctx, (size_t)std::max<int64_t>(n_frames, 1) * 64));
+  TRY(ctx->tmp_a.ensure((size_t)n * 24));
+  float *d = ctx->tmp_a.as<float>();
+  HIPC(hipMemcpyAsync(d, pts, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(d + 3 * n, nrm, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+  TRY(ppfk::prep_cloud(ctx, d, d + 3 * n, n, cloud.soa.as<float>(),
+                       frames ? frames->as<float>() : nullptr, rpdf));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+uint32_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return (uint32_t)p;
+}
+
+// ---------------------------------------------------------- CPU clustering
+// Greedy clustering of transformation_clustering.cpp:62-123 with the pose test
+// of :127-137.  Deviations (documented): stable sorts instead of std::sort,
+// rotation() taken as the linear part, rotation angle 2*atan2(|q.vec|, |q.w|).
+struct HostPose {
+  float R[3][3];
+  float t[3];
+  uint32_t votes;
+};
+
+void quat_of(const float R[3][3], float &x, float &y, float &z, float &w) {
+  // Shepperd's method: trace branch, else the largest diagonal element.
+  float t = R[0][0] + R[1][1] + R[2][2];
+  if (t > 0) {
+    t = sqrtf(t + 1.0f);
+    w = 0.5f * t;
+    t = 0.5f / t;
+    x = (R[2][1] - R[1][2]) * t;
+    y = (R[0][2] - R[2][0]) * t;
+    z = (R[1][0] - R[0][1]) * t;
+    return;
+  }
+  int i = 0;
+  if (R[1][1] > R[0][0]) i = 1;
+  if (R[2][2] > R[i][i]) i = 2;
+  int j = (i + 1) % 3, k = (j + 1) % 3;
+  float c[3];
+  t = sqrtf(R[i][i] - R[j][j] - R[k][k] + 1.0f);
+  c[i] = 0.5f * t;
+  t = 0.5f / t;
+  w = (R[k][j] - R[j][k]) * t;
+  c[j] = (R[j][i] + R[i][j]) * t;
+  c[k] = (R[k][i] + R[i][k]) * t;
+  x = c[0];
+  y = c[1];
+  z = c[2];
+}
+
+float angle_between(const float A[3][3], const float B[3][3]) {
+  float R[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = A[0][i] * B[0][j] + A[1][i] * B[1][j] + A[2][i] * B[2][j];
+  float x, y, z, w;
+  quat_of(R, x, y, z, w);
+  const float n = sqrtf(x * x + y * y + z * z);
+  return 2.0f * ppfm::atan2f_own(n, fabsf(w));
+}
+
+bool within_bounds(const HostPose &a, const HostPose &b, float tt, float rt) {
+  const float dx = a.t[0] - b.t[0], dy = a.t[1] - b.t[1], dz = a.t[2] - b.t[2];
+  const float pd = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float ang = fabsf(angle_between(a.R, b.R));
+  return pd < tt && ang < rt;
+}
+
+void greedy_cluster(const std::vector<float> &T, const std::vector<uint32_t> &counts, float tt,
+                    float rt, float pose_out[16], int64_t *n_clusters) {
+  const size_t P = counts.size();
+  std::vector<HostPose> poses(P);
+  for (size_t i = 0; i < P; ++i) {
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) poses[i].R[r][c] = T[16 * i + 4 * r + c];
+      poses[i].t[r] = T[16 * i + 4 * r + 3];
+    }
+    poses[i].votes = counts[i];
+  }
+  std::stable_sort(poses.begin(), poses.end(),
+                   [](const HostPose &a, const HostPose &b) { return a.votes > b.votes; });
+  std::vector<size_t> first;            // first pose of each cluster
+  std::vector<std::vector<size_t>> members;
+  std::vector<std::pair<size_t, uint32_t>> votes;
+  for (size_t i = 0; i < P; ++i) {
+    bool found = false;
+    for (size_t c = 0; c < first.size(); ++c) {
+      if (within_bounds(poses[i], poses[first[c]], tt, rt)) {
+        members[c].push_back(i);
+        votes[c].second += poses[i].votes;
+        found = true;
+        break;
+      }
+    }
+    if (!found) {
+      first.push_back(i);
+      members.push_back({i});
+      votes.push_back({first.size() - 1, poses[i].votes});
+    }
+  }
+  std::stable_sort(votes.begin(), votes.end(),
+                   [](const std::pair<size_t, uint32_t> &a, const std::pair<size_t, uint32_t> &b) {
+                     return a.second > b.second;
+                   });
+  *n_clusters = (int64_t)first.size();
+  // result[0]: average of the winning cluster (translation mean, quaternion
+  // coefficient mean, normalised)
+  const auto &cl = members[votes[0].first];
+  float ta[3] = {0, 0, 0}, qa[4] = {0, 0, 0, 0};
+  for (size_t pi : cl) {
+    for (int r = 0; r < 3; ++r) ta[r] += poses[pi].t[r];
+    float x, y, z, w;
+    quat_of(poses[pi].R, x, y, z, w);
+    qa[0] += x;
+    qa[1] += y;
+    qa[2] += z;
+    qa[3] += w;
+  }
+  const float sz = (float)cl.size();
+  for (int r = 0; r < 3; ++r) ta[r] /= sz;
+  for (int r = 0; r < 4; ++r) qa[r] /= sz;
+  const float qn = sqrtf(qa[0] * qa[0] + qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]);
+  const float x = qa[0] / qn, y = qa[1] / qn, z = qa[2] / qn, w = qa[3] / qn;
+  const float tx = 2 * x, ty = 2 * y, tz = 2 * z;
+  const float twx = tx * w, twy = ty * w, twz = tz * w;
+  const float txx = tx * x, txy = ty * x, txz = tz * x;
+  const float tyy = ty * y, tyz = tz * y, tzz = tz * z;
+  float R[3][3] = {{1 - (tyy + tzz), txy - twz, txz + twy},
+                   {txy + twz, 1 - (txx + tzz), tyz - twx},
+                   {txz - twy, tyz + twx, 1 - (txx + tyy)}};
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) pose_out[4 * r + c] = R[r][c];
+    pose_out[4 * r + 3] = ta[r];
+  }
+  pose_out[12] = pose_out[13] = pose_out[14] = 0.0f;
+  pose_out[15] = 1.0f;
+}
+
+constexpr uint32_t CAND_CAP0 = 1u << 23;
+constexpr uint32_t CAND_CAP_MAX = 1u << 24;
+
+}  // namespace
+
+extern "C" {
+
+const char *ppf_last_error(void) { return g_err.c_str(); }
+
+int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
+  ARG_CHECK(out, "ppf_ctx_create: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  HIPC(hipGetDeviceCount(&n));
+  ARG_CHECK(n > 0, "ppf_ctx_create: no HIP device");
+  auto *c = new ppf_ctx_s();
+  c->device = std::min(n - 1, std::max(0, device));
+  c->budget = hbm_budget_bytes;
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+  int lds = 0;
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+  c->max_lds = (size_t)lds;
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev2);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev3);
+  if (e != hipSuccess) {
+    set_error(std::string("ppf_ctx_create: ") + hipGetErrorString(e));
+    delete c;
+    return PPF_ERR_HIP;
+  }
+  *out = c;
+  return PPF_OK;
+}
+
+int ppf_ctx_destroy(ppf_ctx ctx) {
+  if (!ctx) return PPF_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->sort.counts.release();
+  ctx->sort.scan.release();
+  ctx->sort.tmp_keys.release();
+  ctx->sort.tmp_vals.release();
+  for (hipEvent_t ev : {ctx->ev0, ctx->ev1, ctx->ev2, ctx->ev3})
+    if (ev) (void)hipEventDestroy(ev);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PPF_OK;
+}
+
+uint64_t ppf_ctx_device_bytes(ppf_ctx ctx) { return ctx ? ctx->bytes.load() : 0; }
+
+// ------------------------------------------------------------------ model
+int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int64_t n,
+                    float d_dist, ppf_model *out) {
+  ARG_CHECK(ctx && out, "ppf_model_build: NULL handle");
+  *out = nullptr;
+  ARG_CHECK(points && normals, "ppf_model_build: NULL cloud");
+  ARG_CHECK(n >= 2 && n <= PPF_MAX_MODEL_POINTS, "ppf_model_build: need 2 <= n <= 65535 points");
+  ARG_CHECK(d_dist > 0.0f && std::isfinite(d_dist), "ppf_model_build: d_dist must be > 0");
+  TRY(set_device(ctx));
+  std::unique_ptr<ppf_model_s> m(new ppf_model_s());
+  m->ctx = ctx;
+  m->d_dist = d_dist;
+  hipStream_t st = ctx->stream;
+  TRY(upload_cloud(ctx, points, normals, n, m->cloud, &m->frames, 1, n));
+  // accumulator tile: TM model points x 31 bins x 4 B must fit the LDS of one block
+  const int64_t lds_points = (int64_t)((ctx->max_lds - 1024) / (N_BINS * 4)) / 32 * 32;
+  m->TM = (int32_t)std::min<int64_t>(n, std::min<int64_t>(DEFAULT_TILE_POINTS, lds_points));
+  ARG_CHECK(m->TM >= 32, "ppf_model_build: device LDS too small for the vote accumulator");
+  m->T = (int32_t)((n + m->TM - 1) / m->TM);
+  const int64_t N = n * n;
+  // words = key << 32 | pair index, in pair order; a stable sort on the key
+  // bits gives (key, pair) order = the reference's stable sort_by_key
+  ppfi::DevBuf words;
+  TRY(words.ensure((size_t)N * 8));
+  TRY(ppfk::cloud_pair_words(ctx, m->cloud, d_dist, 1, words.as<uint64_t>()));
+  TRY(ppfi::radix_sort_u64(words.as<uint64_t>(), nullptr, N, 32, 64, ctx->sort, st));
+  int64_t U = 0;
+  TRY(ppfk::unique_count(ctx, words.as<uint64_t>(), N, &U));
+  m->U = U;
+  TRY(m->pair.alloc(ctx, (size_t)N * 4));
+  TRY(m->hkeys.alloc(ctx, (size_t)U * 4));
+  TRY(m->hfirst.alloc(ctx, (size_t)U * 4));
+  TRY(m->hcount.alloc(ctx, (size_t)U * 4));
+  TRY(ppfk::unique_emit(ctx, words.as<uint64_t>(), N, U, m->hkeys.as<uint32_t>(),
+                        m->hfirst.as<uint32_t>(), m->hcount.as<uint32_t>(), m->pair.as<uint32_t>()));
+  HIPC(hipStreamSynchronize(st));
+  words.release();
+  // key -> unique index table
+  m->hcap = std::max<uint32_t>(1024, next_pow2((uint64_t)U * 2));
+  uint32_t lg = 0;
+  while ((1u << lg) < m->hcap) ++lg;
+  m->hshift = 32 - lg;
+  TRY(m->htab.alloc(ctx, (size_t)m->hcap * 8));
+  TRY(ppfk::build_htable(ctx, m->hkeys.as<uint32_t>(), U, m->htab.as<uint32_t>(), m->hcap, m->hshift));
+  // per-tile sub-ranges and packed vote records
+  TRY(m->toff.alloc(ctx, (size_t)U * (m->T + 1) * 4));
+  TRY(ppfk::build_tile_offsets(ctx, m->hfirst.as<uint32_t>(), m->hcount.as<uint32_t>(),
+                               m->pair.as<uint32_t>(), U, (int32_t)n, m->TM, m->T, m->toff.as<uint32_t>()));
+  TRY(m->rec.alloc(ctx, (size_t)N * 4));
+  TRY(ppfk::build_records(ctx, m->pair.as<uint32_t>(), N, m->frames.as<float>(), m->cloud, m->TM,
+                          m->rec.as<uint32_t>()));
+  TRY(m->weights.alloc(ctx, (size_t)n * 4));
+  std::vector<float> ones((size_t)n, 1.0f);
+  HIPC(hipMemcpyAsync(m->weights.ptr, ones.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipStreamSynchronize(st));
+  *out = m.release();
+  return PPF_OK;
+}
+
+int ppf_model_destroy(ppf_model model) {
+  if (!model) return PPF_OK;
+  (void)hipSetDevice(model->ctx->device);
+  (void)hipStreamSynchronize(model->ctx->stream);
+  delete model;
+  return PPF_OK;
+}
+
+int ppf_model_info(ppf_model m, ppf_model_info_t *info) {
+  ARG_CHECK(m && info, "ppf_model_info: NULL");
+  info->n_points = m->cloud.n;
+  info->d_dist = m->d_dist;
+  info->n_pairs = m->cloud.n * m->cloud.n;
+  info->n_unique_keys = m->U;
+  info->tile_points = m->TM;
+  info->n_tiles = m->T;
+  info->device_bytes = m->cloud.soa.bytes + m->frames.bytes + m->hkeys.bytes + m->hcount.bytes +
+                       m->hfirst.bytes + m->pair.bytes + m->rec.bytes + m->toff.bytes +
+                       m->htab.bytes + m->weights.bytes;
+  return PPF_OK;
+}
+
+int ppf_model_hash_array(ppf_model m, uint32_t *hashkeys, uint32_t *counts, uint32_t *first,
+                         uint32_t *key2ppf) {
+  ARG_CHECK(m, "ppf_model_hash_array: NULL model");
+  TRY(set_device(m->ctx));
+  HIPC(hipStreamSynchronize(m->ctx->stream));
+  if (hashkeys) HIPC(hipMemcpy(hashkeys, m->hkeys.ptr, (size_t)m->U * 4, hipMemcpyDeviceToHost));
+  if (counts) HIPC(hipMemcpy(counts, m->hcount.ptr, (size_t)m->U * 4, hipMemcpyDeviceToHost));
+  if (first) HIPC(hipMemcpy(first, m->hfirst.ptr, (size_t)m->U * 4, hipMemcpyDeviceToHost));
+  if (key2ppf)
+    HIPC(hipMemcpy(key2ppf, m->pair.ptr, (size_t)m->cloud.n * m->cloud.n * 4, hipMemcpyDeviceToHost));
+  return PPF_OK;
+}
+
+int ppf_model_set_weights(ppf_model m, const float *weights, int64_t n) {
+  ARG_CHECK(m && weights, "ppf_model_set_weights: NULL");
+  ARG_CHECK(n == m->cloud.n, "ppf_model_set_weights: need one weight per model point");
+  TRY(set_device(m->ctx));
+  HIPC(hipMemcpy(m->weights.ptr, weights, (size_t)n * 4, hipMemcpyHostToDevice));
+  return PPF_OK;
+}
+
+// ------------------------------------------------------------------ scene
+int ppf_scene_upload(ppf_ctx ctx, const float *points, const float *normals, int64_t n,
+                     uint32_t rpdf, ppf_scene *out) {
+  ARG_CHECK(ctx && out, "ppf_scene_upload: NULL handle");
+  *out = nullptr;
+  ARG_CHECK(points && normals, "ppf_scene_upload: NULL cloud");
+  ARG_CHECK(n >= 2 && n < ((int64_t)1 << 31), "ppf_scene_upload: need 2 <= n < 2^31 points");
+  ARG_CHECK(rpdf >= 1, "ppf_scene_upload: ref_point_downsample_factor must be >= 1");
+  TRY(set_device(ctx));
+  auto *s = new ppf_scene_s();
+  s->ctx = ctx;
+  s->rpdf = rpdf;
+  s->R = (n + rpdf - 1) / rpdf;
+  int rc = upload_cloud(ctx, points, normals, n, s->cloud, &s->frames, rpdf, s->R);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return PPF_OK;
+}
+
+int ppf_scene_destroy(ppf_scene scene) {
+  if (!scene) return PPF_OK;
+  (void)hipSetDevice(scene->ctx->device);
+  (void)hipStreamSynchronize(scene->ctx->stream);
+  delete scene;
+  return PPF_OK;
+}
+
+int ppf_scene_keys(ppf_scene s, float d_dist, uint32_t *keys_out) {
+  ARG_CHECK(s && keys_out, "ppf_scene_keys: NULL");
+  ARG_CHECK(s->cloud.n <= 8192, "ppf_scene_keys: inspection limited to 8192 points");
+  ARG_CHECK(d_dist > 0.0f, "ppf_scene_keys: d_dist must be > 0");
+  ppf_ctx ctx = s->ctx;
+  TRY(set_device(ctx));
+  const int64_t N = s->cloud.n * s->cloud.n;
+  ppfi::DevBuf k;
+  TRY(k.ensure((size_t)N * 4));
+  TRY(ppfk::cloud_pair_keys(ctx, s->cloud, d_dist, s->rpdf, k.as<uint32_t>()));
+  HIPC(hipMemcpyAsync(keys_out, k.ptr, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return PPF_OK;
+}
+
+// ----------------------------------------------------------------- lookup
+int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result *out) {
+  ARG_CHECK(m && s && p && out, "ppf_lookup: NULL argument");
+  *out = nullptr;
+  ARG_CHECK(m->ctx == s->ctx, "ppf_lookup: model and scene belong to different contexts");
+  ppf_ctx ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  TRY(set_device(ctx));
+  std::unique_ptr<ppf_result_s> r(new ppf_result_s());
+  r->ctx = ctx;
+  r->n_m = m->cloud.n;
+  r->cpu = p->cpu_clustering != 0;
+  HIPC(hipEventRecord(ctx->ev2, st));
+
+  // ---- votes.  Candidates (code, count) are pruned against a running lower
+  // bound of the global maximum; on overflow the vote pass is repeated with
+  // the final maximum preset, which emits exactly the kept votes.
+  ppfi::DevBuf cand;
+  uint32_t cap = CAND_CAP0;
+  uint32_t meta_h[2] = {0, 0};
+  float vote_ms = 0.0f;
+  for (int attempt = 0;; ++attempt) {
+    TRY(cand.ensure((size_t)cap * 12 + 64));
+    unsigned long long *cc = cand.as<unsigned long long>();
+    uint32_t *cn = reinterpret_cast<uint32_t *>(cc + cap);
+    uint32_t *meta = cn + cap;  // [0] candidates emitted, [1] running maximum
+    const uint32_t preset[2] = {0u, meta_h[1]};
+    HIPC(hipMemcpyAsync(meta, preset, 8, hipMemcpyHostToDevice, st));
+    ppfk::VoteJob J{};
+    J.sx = s->cloud.x();
+    J.sy = s->cloud.y();
+    J.sz = s->cloud.z();
+    J.snx = s->cloud.nx();
+    J.sny = s->cloud.ny();
+    J.snz = s->cloud.nz();
+    J.snn = s->cloud.nn();
+    J.sframe = s->frames.as<float>();
+    J.n_s = (int32_t)s->cloud.n;
+    J.rpdf = (int32_t)s->rpdf;
+    J.R = (int32_t)s->R;
+    J.htab = m->htab.as<uint32_t>();
+    J.hshift = m->hshift;
+    J.hmask = m->hcap - 1;
+    J.toff = m->toff.as<uint32_t>();
+    J.rec = m->rec.as<uint32_t>();
+    J.pair = m->pair.as<uint32_t>();
+    J.mframe = m->frames.as<float>();
+    J.mx = m->cloud.x();
+    J.my = m->cloud.y();
+    J.mz = m->cloud.z();
+    J.n_m = (int32_t)m->cloud.n;
+    J.TM = m->TM;
+    J.T = m->T;
+    J.d_dist = m->d_dist;
+    J.thr = p->vote_count_threshold;
+    J.max_count = meta + 1;
+    J.cand_code = cc;
+    J.cand_count = cn;
+    J.cand_n = meta;
+    J.cand_cap = cap;
+    J.item_begin = 0;
+    float ms = 0.0f;
+    TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)s->R, m->TM, s->cloud.n, p->exact_votes != 0, &ms));
+    vote_ms += ms;
+    HIPC(hipMemcpy(meta_h, meta, 8, hipMemcpyDeviceToHost));
+    if (meta_h[0] <= cap) break;
+    if (attempt >= 2 || meta_h[0] > CAND_CAP_MAX) {
+      set_error("ppf_lookup: kept-vote buffer overflow (" + std::to_string(meta_h[0]) + " candidates)");
+      return PPF_ERR_NOMEM;
+    }
+    if (attempt >= 1) cap = meta_h[0];
+  }
+  const uint32_t n_cand = meta_h[0];
+  r->info.lookups = (int64_t)s->R * (s->cloud.n - 1);
+  r->info.max_count = meta_h[1];
+  r->info.n_candidates = n_cand;
+  r->info.vote_ms = vote_ms;
+  r->info.max_idx = -1;
+
+  // ---- threshold and order (count desc, code asc)
+  const unsigned long long *ccode = cand.as<unsigned long long>();
+  const uint32_t *ccount = reinterpret_cast<const uint32_t *>(ccode + cap);
+  ppfi::DevBuf sel;
+  const size_t sel_n = std::max<uint32_t>(n_cand, 1);
+  TRY(sel.ensure(sel_n * 12));
+  unsigned long long *scode = sel.as<unsigned long long>();
+  uint32_t *scount = reinterpret_cast<uint32_t *>(scode + sel_n);
+  int64_t P = 0;
+  TRY(ppfk::select_and_sort(ctx, ccode, ccount, n_cand, meta_h[1], p->vote_count_threshold,
+                            s->cloud.n, m->cloud.n, scode, scount, &P));
+  r->P = P;
+  r->info.n_kept = P;
+  if (P > 0) {
+    TRY(r->codes.alloc(ctx, (size_t)P * 8));
+    TRY(r->counts.alloc(ctx, (size_t)P * 4));
+    TRY(r->T.alloc(ctx, (size_t)P * 64));
+    HIPC(hipMemcpyAsync(r->codes.ptr, scode, (size_t)P * 8, hipMemcpyDeviceToDevice, st));
+    HIPC(hipMemcpyAsync(r->counts.ptr, scount, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
+    // ---- poses
+    TRY(ppfk::compute_poses(ctx, r->codes.as<unsigned long long>(), P, m->frames.as<float>(),
+                            s->frames.as<float>(), s->rpdf, r->T.as<float>()));
+    // ---- clustering
+    if (p->cpu_clustering) {
+      std::vector<float> T((size_t)P * 16);
+      std::vector<uint32_t> counts((size_t)P);
+      HIPC(hipMemcpyAsync(T.data(), r->T.ptr, (size_t)P * 64, hipMemcpyDeviceToHost, st));
+      HIPC(hipMemcpyAsync(counts.data(), r->counts.ptr, (size_t)P * 4, hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      greedy_cluster(T, counts, m->d_dist, ppfm::D_ANGLE0, r->pose, &r->info.n_clusters);
+    } else {
+      TRY(r->vc.alloc(ctx, (size_t)P * 4));
+      TRY(r->trans.alloc(ctx, (size_t)P * 12));
+      TRY(r->quats.alloc(ctx, (size_t)P * 16));
+      int64_t mi = -1;
+      TRY(ppfk::cluster_spatial(ctx, r->T.as<float>(), r->codes.as<unsigned long long>(),
+                                r->counts.as<uint32_t>(), m->weights.as<float>(), P, m->d_dist,
+                                p->use_l1_norm != 0, p->use_averaged_clusters != 0, r->vc.as<float>(),
+                                r->trans.as<float>(), r->quats.as<float>(), &mi));
+      r->info.max_idx = mi;
+      float t3[3];
+      HIPC(hipMemcpyAsync(r->pose, r->T.as<float>() + mi * 16, 64, hipMemcpyDeviceToHost, st));
+      HIPC(hipMemcpyAsync(t3, r->trans.as<float>() + mi * 3, 12, hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      // src/cuda/ppf.cu:303-313: translation column from transformation_trans
+      r->pose[3] = t3[0];
+      r->pose[7] = t3[1];
+      r->pose[11] = t3[2];
+    }
+    r->info.status = PPF_OK;
+  } else {
+    r->info.status = PPF_ERR_NO_VOTES;
+  }
+  HIPC(hipEventRecord(ctx->ev3, st));
+  HIPC(hipEventSynchronize(ctx->ev3));
+  float tm = 0.0f;
+  HIPC(hipEventElapsedTime(&tm, ctx->ev2, ctx->ev3));
+  r->info.total_ms = tm;
+  *out = r.release();
+  return PPF_OK;
+}
+
+int ppf_result_destroy(ppf_result r) {
+  if (!r) return PPF_OK;
+  (void)hipSetDevice(r->ctx->device);
+  (void)hipStreamSynchronize(r->ctx->stream);
+  delete r;
+  return PPF_OK;
+}
+
+int ppf_result_info(ppf_result r, ppf_result_info_t *info) {
+  ARG_CHECK(r && info, "ppf_result_info: NULL");
+  *info = r->info;
+  return PPF_OK;
+}
+
+int ppf_result_votes(ppf_result r, uint64_t *codes, uint32_t *counts) {
+  ARG_CHECK(r, "ppf_result_votes: NULL");
+  if (r->P == 0) return PPF_OK;
+  TRY(set_device(r->ctx));
+  if (codes) HIPC(hipMemcpy(codes, r->codes.ptr, (size_t)r->P * 8, hipMemcpyDeviceToHost));
+  if (counts) HIPC(hipMemcpy(counts, r->counts.ptr, (size_t)r->P * 4, hipMemcpyDeviceToHost));
+  return PPF_OK;
+}
+
+int ppf_result_transforms(ppf_result r, float *T) {
+  ARG_CHECK(r && T, "ppf_result_transforms: NULL");
+  if (r->P == 0) return PPF_OK;
+  TRY(set_device(r->ctx));
+  HIPC(hipMemcpy(T, r->T.ptr, (size_t)r->P * 64, hipMemcpyDeviceToHost));
+  return PPF_OK;
+}
+
+int ppf_result_clusters(ppf_result r, float *vc, float *trans, float *quats) {
+  ARG_CHECK(r, "ppf_result_clusters: NULL");
+  if (r->P == 0) return PPF_OK;
+  if (r->cpu) {
+    set_error("ppf_result_clusters: lookup used CPU clustering");
+    return PPF_ERR_STATE;
+  }
+  TRY(set_device(r->ctx));
+  if (vc) HIPC(hipMemcpy(vc, r->vc.ptr, (size_t)r->P * 4, hipMemcpyDeviceToHost));
+  if (trans) HIPC(hipMemcpy(trans, r->trans.ptr, (size_t)r->P * 12, hipMemcpyDeviceToHost));
+  if (quats) HIPC(hipMemcpy(quats, r->quats.ptr, (size_t)r->P * 16, hipMemcpyDeviceToHost));
+  return PPF_OK;
+}
+
+int ppf_result_pose(ppf_result r, float *pose16) {
+  ARG_CHECK(r && pose16, "ppf_result_pose: NULL");
+  std::memcpy(pose16, r->pose, 64);
+  return r->info.status == PPF_OK ? PPF_OK : r->info.status;
+}
+
+// ------------------------------------------------------------ registration
+int ppf_registration(ppf_ctx ctx, int32_t n_scenes, const float *const *scene_points,
+                     const float *const *scene_normals, const int64_t *scene_sizes,
+                     int32_t n_models, const float *const *model_points,
+                     const float *const *model_normals, const int64_t *model_sizes,
+                     const float *model_d_dists, uint32_t rpdf, float vote_count_threshold,
+                     int32_t cpu_clustering, int32_t use_l1_norm, int32_t use_averaged_clusters,
+                     const float *const *model_weights, float *poses_out, int32_t *status_out) {
+  ARG_CHECK(ctx && poses_out, "ppf_registration: NULL argument");
+  ARG_CHECK(n_scenes >= 0 && n_models >= 0, "ppf_registration: negative counts");
+  ARG_CHECK(n_models == 0 || (model_points && model_normals && model_sizes && model_d_dists),
+            "ppf_registration: NULL model arrays");
+  ARG_CHECK(n_scenes == 0 || (scene_points && scene_normals && scene_sizes),
+            "ppf_registration: NULL scene arrays");
+  std::vector<ppf_model> models((size_t)n_models, nullptr);
+  auto cleanup = [&]() {
+    for (auto mm : models) ppf_model_destroy(mm);
+  };
+  // Each model is built once (the reference rebuilds it per scene, ppf.cu:69;
+  // the hash array does not depend on the scene).
+  for (int32_t j = 0; j < n_models; ++j) {
+    int rc = ppf_model_build(ctx, model_points[j], model_normals[j], model_sizes[j], model_d_dists[j],
+                             &models[j]);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+    if (model_weights && model_weights[j]) {
+      rc = ppf_model_set_weights(models[j], model_weights[j], model_sizes[j]);
+      if (rc) {
+        cleanup();
+        return rc;
+      }
+    }
+  }
+  ppf_lookup_params prm{vote_count_threshold, cpu_clustering, use_l1_norm, use_averaged_clusters, 0};
+  for (int32_t i = 0; i < n_scenes; ++i) {
+    ppf_scene sc = nullptr;
+    int rc = ppf_scene_upload(ctx, scene_points[i], scene_normals[i], scene_sizes[i], rpdf, &sc);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+    for (int32_t j = 0; j < n_models; ++j) {
+      ppf_result res = nullptr;
+      rc = ppf_lookup(models[j], sc, &prm, &res);
+      float *dst = poses_out + ((size_t)i * n_models + j) * 16;
+      if (rc) {
+        ppf_scene_destroy(sc);
+        cleanup();
+        return rc;
+      }
+      int st = ppf_result_pose(res, dst);
+      if (status_out) status_out[(size_t)i * n_models + j] = st;
+      ppf_result_destroy(res);
+    }
+    ppf_scene_destroy(sc);
+  }
+  cleanup();
+  return PPF_OK;
+}
+
+int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, float *out,
+                       int64_t n) {
+  ARG_CHECK(ctx && a && out, "ppf_debug_numerics: NULL");
+  ARG_CHECK(op >= 0 && op <= 8, "ppf_debug_numerics: unknown op");
+  TRY(set_device(ctx));
+  const int64_t w = op == 8 ? 2 : 1;
+  ppfi::DevBuf da, db, dout;
+  TRY(da.ensure((size_t)n * w * 4));
+  TRY(db.ensure((size_t)n * w * 4));
+  TRY(dout.ensure((size_t)n * 4));
+  HIPC(hipMemcpy(da.ptr, a, (size_t)n * w * 4, hipMemcpyHostToDevice));
+  if (b) HIPC(hipMemcpy(db.ptr, b, (size_t)n * w * 4, hipMemcpyHostToDevice));
+  TRY(ppfk::debug_numerics(ctx, op, da.as<float>(), db.as<float>(), dout.as<float>(), n));
+  HIPC(hipMemcpyAsync(out, dout.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return PPF_OK;
+}
+
+}  // extern "C"

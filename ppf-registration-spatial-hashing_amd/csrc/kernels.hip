@@ -1,0 +1,897 @@
+// kernels.hip -- gfx950 kernels of the PPF registration path and their launchers.
+//
+//   prep_cloud          AoS -> SoA, |n|, local frames (Scene/Model ctors,
+//                       src/cuda/scene.cu:24-70, src/cuda/model.cu:43-66)
+//   cloud_pair_words    quantized PPF + FNV key of every ordered pair, packed with
+//                       the pair index (ppf_kernel + ppf_hash_kernel,
+//                       src/cuda/kernel.cu:404-477)
+//   unique_words        unique keys / first index / counts of the sorted words
+//                       (histogram + exclusive_scan, include/impl/util.hpp:30-52,
+//                       include/impl/parallel_hash_array.hpp:67-76)
+//   build_htable        key -> unique-key index table (replaces lower_bound,
+//                       parallel_hash_array.hpp:80-92; same hit/miss set)
+//   build_tile_offsets  per unique key, start of each model-point tile
+//   build_records       packed per-pair vote records (model point, phi_u)
+//   run_votes           persistent voting kernel (ppf_vote_count_kernel +
+//                       ppf_vote_kernel + the vote sort/histogram of
+//                       src/cuda/model.cu:95-151, kernel.cu:480-554)
+//   select_and_sort     threshold + order of kept votes (model.cu:155-170)
+//   compute_poses       trans_calc_kernel2 (kernel.cu:605-645)
+//   cluster_spatial     mat2transquat, trans2idx, rot_clustering, max_element
+//                       (kernel.cu:647-763, model.cu:202-244,293-295)
+#include <algorithm>
+#include <cmath>
+
+#include "ppf_geom.h"
+#include "ppf_objects.h"
+
+using namespace ppfi;
+
+namespace ppfk {
+
+namespace {
+
+constexpr int B256 = 256;
+
+__device__ __forceinline__ uint32_t htab_find(const uint32_t *__restrict__ htab, uint32_t hshift,
+                                              uint32_t hmask, uint32_t key) {
+  uint32_t slot = (key * 2654435761u) >> hshift;
+  for (;;) {
+    const uint2 e = *reinterpret_cast<const uint2 *>(htab + 2 * slot);
+    if (e.x == key) return e.y;
+    if (e.x == 0u) return 0xffffffffu;
+    slot = (slot + 1u) & hmask;
+  }
+}
+
+// ------------------------------------------------------------ point prep
+__global__ void k_prep_cloud(const float *__restrict__ pts, const float *__restrict__ nrm, int64_t n,
+                             float *__restrict__ soa, float *__restrict__ frames, uint32_t rpdf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+  const float nx = nrm[3 * i], ny = nrm[3 * i + 1], nz = nrm[3 * i + 2];
+  soa[i] = px;
+  soa[n + i] = py;
+  soa[2 * n + i] = pz;
+  soa[3 * n + i] = nx;
+  soa[4 * n + i] = ny;
+  soa[5 * n + i] = nz;
+  soa[6 * n + i] = ppfg::vnorm(nx, ny, nz);
+  if (frames && (i % rpdf) == 0) {
+    float T[16];
+    ppfg::frame_to_global(px, py, pz, nx, ny, nz, T);
+    float *o = frames + (i / rpdf) * 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) o[k] = T[k];
+  }
+}
+
+// --------------------------------------------------------- pair keys
+// blockIdx.y = reference point r, x-dimension over partners i.  Rows with
+// r % rpdf != 0 and the diagonal get key 0 (reference: .x = NaN,
+// kernel.cu:432-441).
+template <bool WORDS>
+__global__ __launch_bounds__(B256) void k_pair_keys(const float *__restrict__ soa, int64_t n,
+                                                    float d_dist, uint32_t rpdf,
+                                                    void *__restrict__ out) {
+  const int64_t r = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * B256 + threadIdx.x;
+  if (i >= n) return;
+  const float *X = soa, *Y = soa + n, *Z = soa + 2 * n, *NX = soa + 3 * n, *NY = soa + 4 * n,
+              *NZ = soa + 5 * n, *NN = soa + 6 * n;
+  uint32_t key = 0;
+  if ((r % rpdf) == 0 && i != r) {
+    float f[4];
+    ppfg::ppf_feature_q(X[r], Y[r], Z[r], NX[r], NY[r], NZ[r], NN[r], X[i], Y[i], Z[i], NX[i], NY[i],
+                        NZ[i], NN[i], d_dist, f);
+    key = ppfm::ppf_key(f[0], f[1], f[2], f[3]);
+  }
+  const int64_t g = r * n + i;
+  if (WORDS)
+    static_cast<uint64_t *>(out)[g] = ((uint64_t)key << 32) | (uint64_t)(uint32_t)g;
+  else
+    static_cast<uint32_t *>(out)[g] = key;
+}
+
+// ------------------------------------------------------------ unique
+constexpr int UQ_MAX_BLOCKS = 1024;
+
+__device__ __forceinline__ bool is_head(const uint64_t *__restrict__ w, int64_t j) {
+  return j == 0 || (uint32_t)(w[j] >> 32) != (uint32_t)(w[j - 1] >> 32);
+}
+
+__global__ __launch_bounds__(B256) void k_unique_count(const uint64_t *__restrict__ w, int64_t n,
+                                                       int64_t per, uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s[B256 / WAVE];
+  const int64_t beg = (int64_t)blockIdx.x * per, end = min(beg + per, n);
+  uint32_t c = 0;
+  for (int64_t j = beg + threadIdx.x; j < end; j += B256) c += is_head(w, j) ? 1u : 0u;
+  c = wave_sum_u32(c);
+  if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int k = 0; k < B256 / WAVE; ++k) t += s[k];
+    counts[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(B256) void k_unique_emit(const uint64_t *__restrict__ w, int64_t n,
+                                                      int64_t per,
+                                                      const uint32_t *__restrict__ offsets,
+                                                      uint32_t *__restrict__ hkeys,
+                                                      uint32_t *__restrict__ hfirst,
+                                                      uint32_t *__restrict__ low) {
+  __shared__ uint32_t s_w[B256 / WAVE];
+  __shared__ uint32_t s_run;
+  const int tid = threadIdx.x, wv = tid / WAVE, l = tid % WAVE;
+  const int64_t beg = (int64_t)blockIdx.x * per, end = min(beg + per, n);
+  if (tid == 0) s_run = offsets[blockIdx.x];
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  for (int64_t c0 = beg; c0 < end; c0 += B256) {
+    const int64_t j = c0 + tid;
+    const bool valid = j < end;
+    uint64_t wj = valid ? w[j] : 0ull;
+    const bool h = valid && is_head(w, j);
+    if (valid && low) low[j] = (uint32_t)wj;
+    const uint64_t m = __ballot(h);
+    if (l == 0) s_w[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = s_run;
+    for (int k = 0; k < wv; ++k) base += s_w[k];
+    if (h) {
+      const uint32_t u = base + (uint32_t)__popcll(m & lt);
+      hkeys[u] = (uint32_t)(wj >> 32);
+      hfirst[u] = (uint32_t)j;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t t = 0;
+      for (int k = 0; k < B256 / WAVE; ++k) t += s_w[k];
+      s_run += t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_unique_counts(const uint32_t *__restrict__ hfirst, int64_t U, int64_t n,
+                                uint32_t *__restrict__ hcount) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  const uint64_t nxt = (u + 1 < U) ? hfirst[u + 1] : (uint64_t)n;
+  hcount[u] = (uint32_t)(nxt - hfirst[u]);
+}
+
+// ---------------------------------------------------------- hash table
+__global__ void k_htable_insert(const uint32_t *__restrict__ hkeys, int64_t U, uint32_t *htab,
+                                uint32_t hshift, uint32_t hmask) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  const uint32_t key = hkeys[u];
+  if (key == 0u) return;  // key 0 never matches (kernel.cu:491)
+  uint32_t slot = (key * 2654435761u) >> hshift;
+  for (;;) {
+    const uint32_t prev = atomicCAS(htab + 2 * slot, 0u, key);
+    if (prev == 0u) {
+      htab[2 * slot + 1] = (uint32_t)u;
+      return;
+    }
+    slot = (slot + 1u) & hmask;
+  }
+}
+
+// -------------------------------------------------------- tile offsets
+__global__ void k_tile_offsets(const uint32_t *__restrict__ hfirst, const uint32_t *__restrict__ hcount,
+                               const uint32_t *__restrict__ pair, int64_t U, int32_t n_m, int32_t TM,
+                               int32_t T, uint32_t *__restrict__ toff) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  const uint32_t f = hfirst[u], c = hcount[u];
+  uint32_t *o = toff + u * (T + 1);
+  o[0] = f;
+  for (int t = 1; t < T; ++t) {
+    const uint64_t bound = (uint64_t)t * TM * (uint64_t)n_m;
+    uint32_t lo = 0, hi = c;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if ((uint64_t)pair[f + mid] < bound)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    o[t] = f + lo;
+  }
+  o[T] = f + c;
+}
+
+// -------------------------------------------------------------- records
+__global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const float *__restrict__ mframe,
+                          const float *__restrict__ soa, int32_t n_m, int32_t TM,
+                          uint32_t *__restrict__ rec) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint32_t p = pair[j];
+  const uint32_t m_r = p / (uint32_t)n_m, m_i = p - m_r * (uint32_t)n_m;
+  const float *F = mframe + (int64_t)m_r * 16;
+  float uy, uz;
+  ppfg::frame_apply_yz(F + 4, F + 8, soa[m_i], soa[n_m + m_i], soa[2 * n_m + m_i], uy, uz);
+  const double r2 = (double)uy * uy + (double)uz * uz;
+  uint32_t slow = 0u, q = 0u;
+  if (!(r2 > 1e-30) || !(r2 < 1e30)) {
+    slow = REC_SLOW_BIT;
+  } else {
+    const double phi = atan2((double)uz, (double)uy);  // (-pi, pi]
+    const double turns = phi * (1048576.0 / 6.283185307179586);
+    q = (uint32_t)(int32_t)llrint(turns) & 0xfffffu;
+  }
+  rec[j] = (q << 12) | slow | (m_r % (uint32_t)TM);
+}
+
+// --------------------------------------------------------------- voting
+// Fast alpha bin: with theta = phi_v - phi_u the exact angle between the y/z
+// parts of u and v, the float pipeline of alpha_bin() yields
+// floor((theta + pi_f)/D) except within ~1e-6 rad of a bin edge (rounding of
+// cross/dot, atan2f error <= 3.2 ulp, rounding of alpha + pi).  phi_u carries
+// <= 3e-6 rad quantization, phi_v the float atan2 error.  Whenever theta is
+// farther than MARGIN (2e-5 rad) from every nominal edge k*2pi/30 (nominal and
+// true edges differ by < 3e-7 rad) the bin is decided; otherwise, and for
+// degenerate u or v, the exact float pipeline runs.
+constexpr uint32_t FAST_MARGIN = 410166u;  // 2e-5 rad * 30/(2 pi) * 2^32
+
+template <bool EXACT>
+__global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
+                                                     uint64_t n_items,
+                                                     unsigned long long *__restrict__ queue,
+                                                     uint4 *__restrict__ hit_scratch,
+                                                     uint32_t hit_cap) {
+  extern __shared__ uint32_t acc[];
+  __shared__ unsigned long long s_item;
+  __shared__ int s_job;
+  __shared__ uint32_t s_nh, s_next;
+  __shared__ uint32_t s_wmax[VOTE_BLOCK / WAVE];
+  __shared__ float s_thr;
+  const int tid = threadIdx.x, wv = tid / WAVE, lane = tid % WAVE;
+  const uint64_t lt = lanemask_lt();
+  uint4 *hits = hit_scratch + (size_t)blockIdx.x * hit_cap;
+
+  for (;;) {
+    if (tid == 0) {
+      const unsigned long long it = atomicAdd(queue, 1ull);
+      s_item = it;
+      if (it < n_items) {
+        int lo = 0, hi = n_jobs - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (jobs[mid].item_begin <= it)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        s_job = lo;
+      }
+      s_nh = 0;
+    }
+    __syncthreads();
+    const unsigned long long item = s_item;
+    if (item >= n_items) break;
+    const VoteJob &J = jobs[s_job];
+    const int32_t n_s = J.n_s;
+    const uint32_t r = (uint32_t)(item - J.item_begin);
+    const int32_t s_r = (int32_t)r * J.rpdf;
+    const float *Fs = J.sframe + (size_t)r * 16;
+    const float srx = J.sx[s_r], sry = J.sy[s_r], srz = J.sz[s_r];
+    const float snx = J.snx[s_r], sny = J.sny[s_r], snz = J.snz[s_r], snn = J.snn[s_r];
+    float row1[4], row2[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      row1[k] = Fs[4 + k];
+      row2[k] = Fs[8 + k];
+    }
+    const float d_dist = J.d_dist;
+    const uint32_t *__restrict__ htab = J.htab;
+    const uint32_t hshift = J.hshift, hmask = J.hmask;
+
+    // ---- phase 1: key, lookup and v for every partner
+    for (int32_t c0 = 0; c0 < n_s; c0 += VOTE_BLOCK) {
+      const int32_t s_i = c0 + tid;
+      bool hit = false;
+      uint4 hr;
+      if (s_i < n_s && s_i != s_r) {
+        const float px = J.sx[s_i], py = J.sy[s_i], pz = J.sz[s_i];
+        float f[4];
+        ppfg::ppf_feature_q(srx, sry, srz, snx, sny, snz, snn, px, py, pz, J.snx[s_i], J.sny[s_i],
+                            J.snz[s_i], J.snn[s_i], d_dist, f);
+        const uint32_t key = ppfm::ppf_key(f[0], f[1], f[2], f[3]);
+        if (key != 0u) {
+          const uint32_t u = htab_find(htab, hshift, hmask, key);
+          if (u != 0xffffffffu) {
+            hit = true;
+            float vy, vz;
+            ppfg::frame_apply_yz(row1, row2, px, py, pz, vy, vz);
+            uint32_t flag = 0u, v32 = 0u;
+            const float r2 = vy * vy + vz * vz;
+            if (EXACT || !(r2 > 1e-30f) || !(r2 < 1e30f)) {
+              flag = 0x80000000u;
+            } else {
+              const float phi = ppfm::atan2f_own(vz, vy);
+              const double turns = (double)phi * (4294967296.0 / 6.283185307179586);
+              v32 = (uint32_t)(long long)llrint(turns) + 0x80000000u;
+            }
+            hr = make_uint4(u | flag, v32, ppfm::fbits(vy), ppfm::fbits(vz));
+          }
+        }
+      }
+      const uint64_t m = __ballot(hit);
+      uint32_t base = 0;
+      if (lane == 0 && m) base = atomicAdd(&s_nh, (uint32_t)__popcll(m));
+      base = __shfl(base, 0, WAVE);
+      if (hit) hits[base + (uint32_t)__popcll(m & lt)] = hr;
+    }
+    __syncthreads();
+    const uint32_t nh = s_nh;
+
+    // ---- phase 2: per model-point tile, accumulate then emit
+    const int32_t T = J.T, TM = J.TM, n_m = J.n_m;
+    const uint32_t *__restrict__ toff = J.toff;
+    const uint32_t *__restrict__ rec = J.rec;
+    for (int32_t t = 0; t < T; ++t) {
+      const int32_t tile_base = t * TM;
+      const int32_t tm = min(TM, n_m - tile_base);
+      const int32_t nb = tm * N_BINS;
+      for (int32_t i = tid; i < nb; i += VOTE_BLOCK) acc[i] = 0u;
+      if (tid == 0) s_next = 0;
+      __syncthreads();
+      for (;;) {
+        uint32_t h = 0;
+        if (lane == 0) h = atomicAdd(&s_next, 1u);
+        h = __shfl(h, 0, WAVE);
+        if (h >= nh) break;
+        const uint4 hr = hits[h];
+        const uint32_t u = hr.x & 0x7fffffffu;
+        const bool slow_all = EXACT || (hr.x >> 31) != 0u;
+        const uint32_t b = toff[(size_t)u * (T + 1) + t], e = toff[(size_t)u * (T + 1) + t + 1];
+        const uint32_t v32 = hr.y;
+        const float vy = ppfm::bitsf(hr.z), vz = ppfm::bitsf(hr.w);
+        for (uint32_t j = b + lane; j < e; j += WAVE) {
+          const uint32_t rc = rec[j];
+          uint32_t bin = 0, mloc = rc & REC_M_MASK;
+          bool ok = false;
+          if (!slow_all && !(rc & REC_SLOW_BIT)) {
+            const uint32_t a32 = v32 - (rc & REC_PHI_MASK);
+            const uint64_t p = (uint64_t)a32 * 30ull;
+            const uint32_t frac = (uint32_t)p;
+            bin = (uint32_t)(p >> 32);
+            ok = (frac - FAST_MARGIN) <= (0xffffffffu - 2u * FAST_MARGIN);
+          }
+          if (!ok) {
+            const uint32_t pr = J.pair[j];
+            const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
+            const float *Fm = J.mframe + (size_t)m_r * 16;
+            float uy, uz;
+            ppfg::frame_apply_yz(Fm + 4, Fm + 8, J.mx[m_i], J.my[m_i], J.mz[m_i], uy, uz);
+            bin = ppfg::alpha_bin(uy, uz, vy, vz);
+            mloc = m_r - (uint32_t)tile_base;
+          }
+          atomicAdd(&acc[mloc * N_BINS + bin], 1u);
+        }
+      }
+      __syncthreads();
+      // local maximum -> running job maximum -> pruning threshold
+      uint32_t lm = 0;
+      for (int32_t i = tid; i < nb; i += VOTE_BLOCK) lm = max(lm, acc[i]);
+      lm = wave_max_u32(lm);
+      if (lane == 0) s_wmax[wv] = lm;
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t m = 0;
+        for (int k = 0; k < VOTE_BLOCK / WAVE; ++k) m = max(m, s_wmax[k]);
+        uint32_t g = atomicMax(J.max_count, m);
+        g = max(g, m);
+        s_thr = J.thr * (float)g;
+      }
+      __syncthreads();
+      const float thr_lb = s_thr;
+      const int32_t nb_r = (nb + VOTE_BLOCK - 1) / VOTE_BLOCK * VOTE_BLOCK;
+      for (int32_t i = tid; i < nb_r; i += VOTE_BLOCK) {
+        const uint32_t c = i < nb ? acc[i] : 0u;
+        const bool emit = c > 0u && (float)c > thr_lb;
+        const uint64_t m = __ballot(emit);
+        uint32_t base = 0;
+        if (lane == 0 && m) base = atomicAdd(J.cand_n, (uint32_t)__popcll(m));
+        base = __shfl(base, 0, WAVE);
+        if (emit) {
+          const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+          if (pos < J.cand_cap) {
+            const uint32_t mr = (uint32_t)tile_base + (uint32_t)(i / N_BINS);
+            const uint32_t a = (uint32_t)(i % N_BINS);
+            J.cand_code[pos] = ((unsigned long long)(uint32_t)s_r << 32) | (unsigned long long)((mr << 6) | a);
+            J.cand_count[pos] = c;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------- vote selection
+__global__ void k_select_flags(const uint32_t *__restrict__ cnt, uint32_t n, float thr_f,
+                               uint32_t *__restrict__ flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = ((float)cnt[i] > thr_f) ? 1u : 0u;
+}
+
+// compact survivors into composite sort keys (count desc, code asc) when the
+// fields fit in 64 bits, else into (code, count) pairs
+__global__ void k_select_scatter(const unsigned long long *__restrict__ code,
+                                 const uint32_t *__restrict__ cnt, uint32_t n,
+                                 const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos,
+                                 uint32_t max_count, int composite, int sh_m, int sh_c,
+                                 uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flags[i]) return;
+  const unsigned long long c = code[i];
+  const uint32_t p = pos[i];
+  if (composite) {
+    const uint64_t s_r = c >> 32, mc = c & 0xffffffffull;
+    keys[p] = ((uint64_t)(max_count - cnt[i]) << sh_c) | (s_r << sh_m) | mc;
+  } else {
+    keys[p] = c;
+    vals[p] = cnt[i];
+  }
+}
+
+__global__ void k_select_decode(const uint64_t *__restrict__ keys, int64_t P, uint32_t max_count,
+                                int sh_m, int sh_c, unsigned long long *__restrict__ out_code,
+                                uint32_t *__restrict__ out_cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const uint64_t k = keys[i];
+  const uint64_t mc = k & ((1ull << sh_m) - 1ull);
+  const uint64_t s_r = (k & ((1ull << sh_c) - 1ull)) >> sh_m;
+  out_code[i] = (s_r << 32) | mc;
+  out_cnt[i] = max_count - (uint32_t)(k >> sh_c);
+}
+
+__global__ void k_iota_inv(const uint32_t *__restrict__ cnt, int64_t P, uint32_t max_count,
+                           uint64_t *__restrict__ keys2, uint32_t *__restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  keys2[i] = (uint64_t)(max_count - cnt[i]);
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_pairs(const uint64_t *__restrict__ codes, const uint32_t *__restrict__ cnt,
+                               const uint32_t *__restrict__ idx, int64_t P,
+                               unsigned long long *__restrict__ out_code,
+                               uint32_t *__restrict__ out_cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  out_code[i] = codes[idx[i]];
+  out_cnt[i] = cnt[idx[i]];
+}
+
+// ----------------------------------------------------------------- poses
+__global__ void k_poses(const unsigned long long *__restrict__ codes, int64_t P,
+                        const float *__restrict__ mframe, const float *__restrict__ sframe,
+                        uint32_t rpdf, float *__restrict__ T) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const unsigned long long c = codes[i];
+  const uint32_t s_r = (uint32_t)(c >> 32);
+  const uint32_t mc = (uint32_t)c;
+  const uint32_t m_r = mc >> 6, a = mc & 63u;
+  float out[16];
+  if (s_r == 0 && m_r == 0 && a == 0) {
+    // code 0 is skipped by the reference and keeps a zero matrix (kernel.cu:628-630)
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[k] = 0.0f;
+  } else {
+    float Tm[16], Ts[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      Tm[k] = mframe[(size_t)m_r * 16 + k];
+      Ts[k] = sframe[(size_t)(s_r / rpdf) * 16 + k];
+    }
+    ppfg::pose_from_frames(Tm, Ts, a, out);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[i * 16 + k] = out[k];
+}
+
+// ------------------------------------------------------------ clustering
+__global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long long *__restrict__ codes,
+                               const uint32_t *__restrict__ cnt, const float *__restrict__ weights,
+                               int64_t P, float d_dist, float *__restrict__ trans,
+                               float *__restrict__ quats, float *__restrict__ w,
+                               uint64_t *__restrict__ words, uint32_t *__restrict__ adj) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  float M[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) M[k] = T[i * 16 + k];
+  trans[3 * i] = M[3];
+  trans[3 * i + 1] = M[7];
+  trans[3 * i + 2] = M[11];
+  float q[4];
+  ppfg::rotmat_to_quat(M, q);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) quats[4 * i + k] = q[k];
+  const uint32_t m_r = ((uint32_t)codes[i]) >> 6;
+  w[i] = (weights ? weights[m_r] : 1.0f) * (float)cnt[i];
+  int32_t c[3];
+  ppfg::cell_of(M[3], M[7], M[11], d_dist, c);
+  words[i] = ((uint64_t)ppfm::cell_hash(c[0], c[1], c[2]) << 32) | (uint64_t)i;
+  int cc = 0;
+  for (int a = -1; a < 2; ++a)
+    for (int b = -1; b < 2; ++b)
+      for (int k = -1; k < 2; ++k, ++cc)
+        adj[27 * i + cc] = (a == 0 && b == 0 && k == 0) ? 0u
+                                                         : ppfm::cell_hash(c[0] + a, c[1] + b, c[2] + k);
+}
+
+__global__ void k_cluster_votes(const float *__restrict__ trans, const float *__restrict__ quats,
+                                const float *__restrict__ w, const uint32_t *__restrict__ adj,
+                                const uint32_t *__restrict__ ukeys, const uint32_t *__restrict__ ufirst,
+                                const uint32_t *__restrict__ ucount, int64_t U,
+                                const uint32_t *__restrict__ members, int64_t P, float trans_thresh,
+                                int use_l1, int use_avg, float *__restrict__ vc_out,
+                                float *__restrict__ trans_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float rot_thresh_sq = ppfm::ROT_THRESH * ppfm::ROT_THRESH;
+  const float tx = trans[3 * i], ty = trans[3 * i + 1], tz = trans[3 * i + 2];
+  const float qx = quats[4 * i], qy = quats[4 * i + 1], qz = quats[4 * i + 2], qw = quats[4 * i + 3];
+  float vc = 1.0f;
+  float ox = tx, oy = ty, oz = tz;
+  for (int blk = 0; blk < 27; ++blk) {
+    const uint32_t h = adj[27 * i + blk];
+    if (h == 0u) continue;
+    int64_t lo = 0, hi = U;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (ukeys[mid] < h)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo >= U || ukeys[lo] != h) continue;
+    const uint32_t f = ufirst[lo], n = ucount[lo];
+    for (uint32_t jj = 0; jj < n; ++jj) {
+      const uint32_t j = members[f + jj];
+      const float wj = w[j];
+      const float d4 = qx * quats[4 * j] + qy * quats[4 * j + 1] + qz * quats[4 * j + 2] +
+                       qw * quats[4 * j + 3];
+      const float qd = fabsf(8.0f * (1.0f - d4));
+      if (qd < rot_thresh_sq) {
+        const float jx = trans[3 * j], jy = trans[3 * j + 1], jz = trans[3 * j + 2];
+        if (!use_l1) {
+          const float dx = tx - jx, dy = ty - jy, dz = tz - jz;
+          const float nd = sqrtf(dx * dx + dy * dy + dz * dz);
+          if (!(nd < trans_thresh)) continue;
+        }
+        if (use_avg) {
+          ox = vc * ox;
+          oy = vc * oy;
+          oz = vc * oz;
+          ox = ox + wj * jx;
+          oy = oy + wj * jy;
+          oz = oz + wj * jz;
+          const float s = 1.0f / (vc + wj);
+          ox = s * ox;
+          oy = s * oy;
+          oz = s * oz;
+        }
+        vc += wj;
+      }
+    }
+  }
+  vc_out[i] = vc;
+  trans_out[3 * i] = ox;
+  trans_out[3 * i + 1] = oy;
+  trans_out[3 * i + 2] = oz;
+}
+
+// first index of the maximum (thrust::max_element), one block
+__global__ __launch_bounds__(1024) void k_argmax(const float *__restrict__ v, int64_t P,
+                                                 long long *__restrict__ out) {
+  __shared__ float sv[1024];
+  __shared__ long long si[1024];
+  float bv = 0.0f;
+  long long bi = -1;
+  for (int64_t i = threadIdx.x; i < P; i += 1024) {
+    const float x = v[i];
+    if (bi < 0 || bv < x) {
+      bv = x;
+      bi = i;
+    }
+  }
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const float ov = sv[threadIdx.x + s];
+      const long long oi = si[threadIdx.x + s];
+      const long long mi = si[threadIdx.x];
+      if (oi >= 0 && (mi < 0 || sv[threadIdx.x] < ov || (!(ov < sv[threadIdx.x]) && oi < mi))) {
+        sv[threadIdx.x] = ov;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = si[0];
+}
+
+// ---------------------------------------------------------- diagnostics
+__global__ void k_debug_numerics(int op, const float *__restrict__ a, const float *__restrict__ b,
+                                 float *__restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float r = 0.0f;
+  switch (op) {
+    case 0: r = ppfm::acosf_own(a[i]); break;
+    case 1: r = ppfm::atan2f_own(a[i], b[i]); break;
+    case 2: r = ppfm::sinf_own(a[i]); break;
+    case 3: r = ppfm::cosf_own(a[i]); break;
+    case 4: r = ppfm::fmodf_exact(a[i], b[i]); break;
+    case 5: r = ppfm::quant_downf(a[i], b[i]); break;
+    case 6: r = a[i] / b[i]; break;
+    case 7: r = sqrtf(a[i]); break;
+    case 8: r = (float)ppfg::alpha_bin(a[2 * i], a[2 * i + 1], b[2 * i], b[2 * i + 1]); break;
+    default: break;
+  }
+  out[i] = r;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int bitlen(uint64_t x) {
+  int b = 0;
+  while (x) {
+    ++b;
+    x >>= 1;
+  }
+  return b;
+}
+
+}  // namespace
+
+// =============================================================== launchers
+int prep_cloud(ppf_ctx ctx, const float *d_aos_pts, const float *d_aos_nrm, int64_t n, float *soa,
+               float *frames, uint32_t rpdf) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_prep_cloud, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, d_aos_pts, d_aos_nrm,
+                     n, soa, frames, rpdf);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int cloud_pair_words(ppf_ctx ctx, const CloudDev &c, float d_dist, uint32_t rpdf, uint64_t *words) {
+  if (c.n <= 0) return 0;
+  dim3 g(nblk(c.n, B256), (unsigned)c.n);
+  hipLaunchKernelGGL(k_pair_keys<true>, g, dim3(B256), 0, ctx->stream, c.soa.as<float>(), c.n, d_dist,
+                     rpdf, (void *)words);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int cloud_pair_keys(ppf_ctx ctx, const CloudDev &c, float d_dist, uint32_t rpdf, uint32_t *keys) {
+  if (c.n <= 0) return 0;
+  dim3 g(nblk(c.n, B256), (unsigned)c.n);
+  hipLaunchKernelGGL(k_pair_keys<false>, g, dim3(B256), 0, ctx->stream, c.soa.as<float>(), c.n,
+                     d_dist, rpdf, (void *)keys);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out) {
+  *U_out = 0;
+  if (n <= 0) return 0;
+  const int64_t chunks = (n + B256 - 1) / B256;
+  const int G = (int)std::min<int64_t>(chunks, UQ_MAX_BLOCKS);
+  const int64_t per = (chunks + G - 1) / G * B256;
+  PPF_TRY(ctx->tmp_c.ensure((size_t)G * 8 + 16));
+  uint32_t *counts = ctx->tmp_c.as<uint32_t>();
+  uint32_t *offs = counts + G;
+  uint32_t *total = offs + G;
+  hipLaunchKernelGGL(k_unique_count, dim3(G), dim3(B256), 0, ctx->stream, words, n, per, counts);
+  PPF_TRY(exclusive_scan_u32(counts, offs, G, ctx->tmp_d, ctx->stream, total));
+  PPF_HIP(hipGetLastError());
+  uint32_t U = 0;
+  PPF_HIP(hipMemcpyAsync(&U, total, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PPF_HIP(hipStreamSynchronize(ctx->stream));
+  *U_out = U;
+  return 0;
+}
+
+int unique_emit(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t U, uint32_t *hkeys,
+                uint32_t *hfirst, uint32_t *hcount, uint32_t *low) {
+  if (n <= 0) return 0;
+  const int64_t chunks = (n + B256 - 1) / B256;
+  const int G = (int)std::min<int64_t>(chunks, UQ_MAX_BLOCKS);
+  const int64_t per = (chunks + G - 1) / G * B256;
+  const uint32_t *offs = ctx->tmp_c.as<uint32_t>() + G;  // left by unique_count
+  hipLaunchKernelGGL(k_unique_emit, dim3(G), dim3(B256), 0, ctx->stream, words, n, per, offs, hkeys,
+                     hfirst, low);
+  if (U > 0)
+    hipLaunchKernelGGL(k_unique_counts, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, hfirst, U, n,
+                       hcount);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int build_htable(ppf_ctx ctx, const uint32_t *hkeys, int64_t U, uint32_t *htab, uint32_t hcap,
+                 uint32_t hshift) {
+  PPF_HIP(hipMemsetAsync(htab, 0, (size_t)hcap * 8, ctx->stream));
+  if (U <= 0) return 0;
+  hipLaunchKernelGGL(k_htable_insert, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, hkeys, U, htab,
+                     hshift, hcap - 1);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcount,
+                       const uint32_t *pair, int64_t U, int32_t n_m, int32_t TM, int32_t T,
+                       uint32_t *toff) {
+  if (U <= 0) return 0;
+  hipLaunchKernelGGL(k_tile_offsets, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, hfirst, hcount,
+                     pair, U, n_m, TM, T, toff);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
+                  const CloudDev &m, int32_t TM, uint32_t *rec) {
+  if (N <= 0) return 0;
+  hipLaunchKernelGGL(k_records, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, pair, N, mframe,
+                     m.soa.as<float>(), (int32_t)m.n, TM, rec);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
+              int64_t max_ns, bool exact, float *vote_ms) {
+  *vote_ms = 0.0f;
+  if (n_items == 0 || n_jobs <= 0) return 0;
+  const size_t lds = (size_t)max_tm * N_BINS * 4;
+  if (lds > ctx->max_lds) {
+    set_error("run_votes: accumulator tile exceeds the LDS limit");
+    return PPF_ERR_ARG;
+  }
+  const int G = ctx->n_cu;
+  PPF_TRY(ctx->tmp_a.ensure(sizeof(VoteJob) * n_jobs));
+  PPF_TRY(ctx->queue.ensure(16));
+  PPF_TRY(ctx->hits.ensure((size_t)G * (size_t)max_ns * 16));
+  PPF_HIP(hipMemcpyAsync(ctx->tmp_a.ptr, h_jobs, sizeof(VoteJob) * n_jobs, hipMemcpyHostToDevice,
+                         ctx->stream));
+  PPF_HIP(hipMemsetAsync(ctx->queue.ptr, 0, 16, ctx->stream));
+  auto kern = exact ? k_vote<true> : k_vote<false>;
+  PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream,
+                     ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items,
+                     ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), (uint32_t)max_ns);
+  PPF_HIP(hipGetLastError());
+  PPF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  PPF_HIP(hipEventSynchronize(ctx->ev1));
+  PPF_HIP(hipEventElapsedTime(vote_ms, ctx->ev0, ctx->ev1));
+  return 0;
+}
+
+int select_and_sort(ppf_ctx ctx, const unsigned long long *cand_code, const uint32_t *cand_count,
+                    uint32_t n_cand, uint32_t max_count, float thr, int64_t n_s, int64_t n_m,
+                    unsigned long long *out_codes, uint32_t *out_counts, int64_t *P_out) {
+  *P_out = 0;
+  if (n_cand == 0) return 0;
+  const float thr_f = thr * (float)max_count;  // model.cu:164
+  PPF_TRY(ctx->tmp_b.ensure((size_t)n_cand * 8 + 16));
+  uint32_t *flags = ctx->tmp_b.as<uint32_t>();
+  uint32_t *pos = flags + n_cand;
+  uint32_t *total = pos + n_cand;
+  hipLaunchKernelGGL(k_select_flags, dim3(nblk(n_cand, 256)), dim3(256), 0, ctx->stream, cand_count,
+                     n_cand, thr_f, flags);
+  PPF_TRY(exclusive_scan_u32(flags, pos, n_cand, ctx->tmp_d, ctx->stream, total));
+  uint32_t P = 0;
+  PPF_HIP(hipMemcpyAsync(&P, total, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PPF_HIP(hipStreamSynchronize(ctx->stream));
+  *P_out = P;
+  if (P == 0) return 0;
+  const int bs = std::max(1, bitlen((uint64_t)(n_s > 0 ? n_s - 1 : 0)));
+  const int bm = bitlen((uint64_t)(n_m > 0 ? n_m - 1 : 0));
+  const int bc = bitlen(max_count);
+  const int sh_m = 6 + bm, sh_c = 6 + bm + bs;
+  const bool composite = sh_c + bc <= 64;
+  PPF_TRY(ctx->tmp_c.ensure((size_t)P * 16));
+  uint64_t *keys = ctx->tmp_c.as<uint64_t>();
+  uint32_t *vals = reinterpret_cast<uint32_t *>(keys + P);
+  hipLaunchKernelGGL(k_select_scatter, dim3(nblk(n_cand, 256)), dim3(256), 0, ctx->stream, cand_code,
+                     cand_count, n_cand, flags, pos, max_count, composite ? 1 : 0, sh_m, sh_c, keys,
+                     vals);
+  PPF_HIP(hipGetLastError());
+  if (composite) {
+    PPF_TRY(radix_sort_u64(keys, nullptr, P, 0, sh_c + bc, ctx->sort, ctx->stream));
+    hipLaunchKernelGGL(k_select_decode, dim3(nblk(P, 256)), dim3(256), 0, ctx->stream, keys,
+                       (int64_t)P, max_count, sh_m, sh_c, out_codes, out_counts);
+  } else {
+    PPF_TRY(radix_sort_u64(keys, vals, P, 0, 64, ctx->sort, ctx->stream));
+    PPF_TRY(ctx->tmp_d.ensure((size_t)P * 12));
+    uint64_t *k2 = ctx->tmp_d.as<uint64_t>();
+    uint32_t *ix = reinterpret_cast<uint32_t *>(k2 + P);
+    hipLaunchKernelGGL(k_iota_inv, dim3(nblk(P, 256)), dim3(256), 0, ctx->stream, vals, (int64_t)P,
+                       max_count, k2, ix);
+    PPF_TRY(radix_sort_u64(k2, ix, P, 0, std::max(8, bc), ctx->sort, ctx->stream));
+    hipLaunchKernelGGL(k_gather_pairs, dim3(nblk(P, 256)), dim3(256), 0, ctx->stream, keys, vals, ix,
+                       (int64_t)P, out_codes, out_counts);
+  }
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const float *mframe,
+                  const float *sframe, uint32_t rpdf, float *T) {
+  if (P <= 0) return 0;
+  hipLaunchKernelGGL(k_poses, dim3(nblk(P, 128)), dim3(128), 0, ctx->stream, codes, P, mframe, sframe,
+                     rpdf, T);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
+                    const uint32_t *counts, const float *weights, int64_t P, float d_dist,
+                    bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
+                    int64_t *max_idx) {
+  *max_idx = -1;
+  if (P <= 0) return 0;
+  // trans (3P) w (P) words (P u64) adj (27P) ukeys/ufirst/ucount (3P) members (P) argmax (8)
+  const size_t bytes = (size_t)P * (12 + 4 + 8 + 108 + 12 + 4) + 64;
+  PPF_TRY(ctx->tmp_b.ensure(bytes));
+  char *p = ctx->tmp_b.as<char>();
+  uint64_t *words = reinterpret_cast<uint64_t *>(p);
+  p += (size_t)P * 8;
+  float *trans = reinterpret_cast<float *>(p);
+  p += (size_t)P * 12;
+  float *w = reinterpret_cast<float *>(p);
+  p += (size_t)P * 4;
+  uint32_t *adj = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 108;
+  uint32_t *ukeys = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *ufirst = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *ucount = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *members = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  long long *am = reinterpret_cast<long long *>(((uintptr_t)p + 7) & ~(uintptr_t)7);
+  hipLaunchKernelGGL(k_cluster_prep, dim3(nblk(P, 128)), dim3(128), 0, ctx->stream, T, codes, counts,
+                     weights, P, d_dist, trans, quats, w, words, adj);
+  PPF_HIP(hipGetLastError());
+  PPF_TRY(radix_sort_u64(words, nullptr, P, 32, 64, ctx->sort, ctx->stream));
+  int64_t U = 0;
+  PPF_TRY(unique_count(ctx, words, P, &U));
+  PPF_TRY(unique_emit(ctx, words, P, U, ukeys, ufirst, ucount, members));
+  hipLaunchKernelGGL(k_cluster_votes, dim3(nblk(P, 128)), dim3(128), 0, ctx->stream, trans, quats, w,
+                     adj, ukeys, ufirst, ucount, U, members, P, d_dist, use_l1 ? 1 : 0,
+                     use_avg ? 1 : 0, vc, trans_out);
+  hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
+  PPF_HIP(hipGetLastError());
+  long long mi = -1;
+  PPF_HIP(hipMemcpyAsync(&mi, am, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PPF_HIP(hipStreamSynchronize(ctx->stream));
+  *max_idx = mi;
+  return 0;
+}
+
+int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *out, int64_t n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_debug_numerics, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, op, a, b, out, n);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace ppfk

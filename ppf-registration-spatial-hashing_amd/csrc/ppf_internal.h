@@ -1,0 +1,92 @@
+// ppf_internal.h -- declarations shared by the HIP translation units of
+// libppf_amd.so (not part of the public C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace ppfi {
+
+constexpr int WAVE = 64;
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+#define PPF_HIP(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      ::ppfi::set_error(std::string(#expr) + ": " + hipGetErrorString(e_) + " (" + \
+                        __FILE__ + ":" + std::to_string(__LINE__) + ")");          \
+      return 2; /* PPF_ERR_HIP */                                                  \
+    }                                                                              \
+  } while (0)
+#define PPF_TRY(expr)      \
+  do {                     \
+    int r_ = (expr);       \
+    if (r_ != 0) return r_; \
+  } while (0)
+
+// -------------------------------------------------------- device buffers
+// A growable device buffer owned by whoever holds it; never shrinks.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n);  // grow to at least n bytes (contents not kept)
+  void release();
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
+// ------------------------------------------------------------ radix sort
+// Stable LSD radix sort of 64-bit keys (optionally carrying 32-bit values) on
+// bits [bit_lo, bit_hi), 8-bit digits.  Result ends in keys/vals (the input
+// buffers); tmp_keys/tmp_vals are ping-pong space of the same length.
+struct SortScratch {
+  DevBuf counts, scan, tmp_keys, tmp_vals;
+};
+int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
+                   SortScratch &s, hipStream_t st);
+
+// ------------------------------------------------------- scans / reduces
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp, hipStream_t st,
+                       uint32_t *total_out_dev);
+
+}  // namespace ppfi
+
+// ----------------------------------------------------------- wave helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  int l = __lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// inclusive prefix sum across the wave
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+  int l = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t w = __shfl_up(v, o, 64);
+    if (l >= o) v += w;
+  }
+  return v;
+}

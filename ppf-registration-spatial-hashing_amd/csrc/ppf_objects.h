@@ -1,0 +1,155 @@
+// ppf_objects.h -- definitions of the opaque handle types and of the device
+// launchers shared by kernels.hip and api.hip.
+#pragma once
+
+#include <atomic>
+#include <vector>
+
+#include "../../include/ppf_amd.h"
+#include "ppf_internal.h"
+
+// Vote records: bits 0..10 model point within its tile, bit 11 "evaluate with
+// the full float path", bits 12..31 the angle phi_u = atan2(u.z, u.y) of the
+// model pair in units of 2*pi/2^20.
+constexpr uint32_t REC_M_MASK = 0x7ffu;
+constexpr uint32_t REC_SLOW_BIT = 0x800u;
+constexpr uint32_t REC_PHI_MASK = 0xfffff000u;
+constexpr int MAX_TILE_POINTS = 2048;
+constexpr int DEFAULT_TILE_POINTS = 1024;
+constexpr int N_BINS = 31;
+constexpr int VOTE_BLOCK = 1024;
+
+struct ppf_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  std::atomic<uint64_t> bytes{0};
+  int n_cu = 0;
+  size_t max_lds = 0;
+  ppfi::SortScratch sort;
+  ppfi::DevBuf tmp_a, tmp_b, tmp_c, tmp_d, hits, queue;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+};
+
+// Tracked device allocation owned by a model/scene/result.
+struct TBuf {
+  ppf_ctx ctx = nullptr;
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int alloc(ppf_ctx c, size_t n);
+  void release();
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+  ~TBuf() { release(); }
+  TBuf() = default;
+  TBuf(const TBuf &) = delete;
+  TBuf &operator=(const TBuf &) = delete;
+};
+
+// Structure-of-arrays cloud: x y z nx ny nz |n|, each n floats.
+struct CloudDev {
+  int64_t n = 0;
+  TBuf soa;
+  const float *x() const { return soa.as<float>(); }
+  const float *y() const { return soa.as<float>() + n; }
+  const float *z() const { return soa.as<float>() + 2 * n; }
+  const float *nx() const { return soa.as<float>() + 3 * n; }
+  const float *ny() const { return soa.as<float>() + 4 * n; }
+  const float *nz() const { return soa.as<float>() + 5 * n; }
+  const float *nn() const { return soa.as<float>() + 6 * n; }
+};
+
+struct ppf_model_s {
+  ppf_ctx ctx = nullptr;
+  CloudDev cloud;
+  float d_dist = 0;
+  int32_t TM = 0, T = 0;
+  int64_t U = 0;
+  uint32_t hcap = 0, hshift = 0;
+  TBuf frames;   // [n][16] local frames T_{m->g}
+  TBuf hkeys;    // [U] unique keys (ascending)
+  TBuf hcount;   // [U]
+  TBuf hfirst;   // [U]
+  TBuf pair;     // [n*n] pair index m_r*n + m_i, sorted by (key, pair)
+  TBuf rec;      // [n*n] packed vote records, same order
+  TBuf toff;     // [U][T+1] absolute start of each tile's sub-range
+  TBuf htab;     // [hcap][2] (key, u) open addressing, key 0 = empty
+  TBuf weights;  // [n]
+};
+
+struct ppf_scene_s {
+  ppf_ctx ctx = nullptr;
+  CloudDev cloud;
+  uint32_t rpdf = 1;
+  int64_t R = 0;
+  TBuf frames;  // [R][16] frames of reference points
+};
+
+struct ppf_result_s {
+  ppf_ctx ctx = nullptr;
+  ppf_result_info_t info{};
+  int64_t P = 0;
+  int64_t n_m = 0;
+  TBuf codes, counts, T, vc, trans, quats;
+  float pose[16] = {0};
+  bool cpu = false;
+};
+
+// ------------------------------------------------------------- launchers
+namespace ppfk {
+
+struct VoteJob {
+  const float *sx, *sy, *sz, *snx, *sny, *snz, *snn;
+  const float *sframe;
+  int32_t n_s, rpdf, R, pad0;
+  const uint32_t *htab;
+  uint32_t hshift, hmask;
+  const uint32_t *toff;
+  const uint32_t *rec;
+  const uint32_t *pair;
+  const float *mframe;
+  const float *mx, *my, *mz;
+  int32_t n_m, TM, T, pad1;
+  float d_dist, thr;
+  uint32_t *max_count;
+  unsigned long long *cand_code;
+  uint32_t *cand_count;
+  uint32_t *cand_n;
+  uint32_t cand_cap, pad2;
+  uint64_t item_begin;
+};
+
+int prep_cloud(ppf_ctx ctx, const float *d_aos_pts, const float *d_aos_nrm, int64_t n,
+               float *soa, float *frames, uint32_t rpdf);
+int cloud_pair_words(ppf_ctx ctx, const CloudDev &c, float d_dist, uint32_t rpdf, uint64_t *words);
+int cloud_pair_keys(ppf_ctx ctx, const CloudDev &c, float d_dist, uint32_t rpdf, uint32_t *keys);
+// sorted words -> number of unique keys U (host value); then unique keys
+// (hkeys), first indices (hfirst), counts (hcount) and the low halves of the
+// words (low, may be NULL).  unique_emit must follow unique_count on the same
+// context with the same words.
+int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out);
+int unique_emit(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t U, uint32_t *hkeys,
+                uint32_t *hfirst, uint32_t *hcount, uint32_t *low);
+int build_htable(ppf_ctx ctx, const uint32_t *hkeys, int64_t U, uint32_t *htab, uint32_t hcap,
+                 uint32_t hshift);
+int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcount,
+                       const uint32_t *pair, int64_t U, int32_t n_m, int32_t TM, int32_t T,
+                       uint32_t *toff);
+int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
+                  const CloudDev &m, int32_t TM, uint32_t *rec);
+int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
+              int64_t max_ns, bool exact, float *vote_ms);
+int select_and_sort(ppf_ctx ctx, const unsigned long long *cand_code, const uint32_t *cand_count,
+                    uint32_t n_cand, uint32_t max_count, float thr, int64_t n_s, int64_t n_m,
+                    unsigned long long *out_codes, uint32_t *out_counts, int64_t *P_out);
+int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const float *mframe,
+                  const float *sframe, uint32_t rpdf, float *T);
+int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
+                    const uint32_t *counts, const float *weights, int64_t P, float d_dist,
+                    bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
+                    int64_t *max_idx);
+int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *out, int64_t n);
+
+}  // namespace ppfk

@@ -1,0 +1,273 @@
+// radix_sort.hip -- stable LSD radix sort of 64-bit keys (+ optional 32-bit
+// values) and an exclusive scan, written for wave64 CDNA.
+//
+// Replaces the reference's thrust::sort_by_key / thrust::sort / exclusive_scan
+// call sites (include/impl/parallel_hash_array.hpp:63-65,73-76;
+// src/cuda/model.cu:111-113,148,155-158,222-223).
+//
+// Per 8-bit digit pass:
+//   count   : each block owns a contiguous run of 2048-element tiles and writes
+//             its 256 digit counts, digit-major, to counts[d*G + b];
+//   scan    : exclusive scan of counts gives each (digit, block) its base;
+//   scatter : each block re-reads its tiles in order; inside a tile every wave
+//             ranks its 64-element slots with 8 ballots (lanes holding the same
+//             digit find each other), which keeps the sort stable; per-wave
+//             digit totals in LDS turn the ranks into global positions.
+#include "ppf_internal.h"
+
+namespace ppfi {
+
+namespace {
+
+constexpr int RS_BLOCK = 256;
+constexpr int RS_WAVES = RS_BLOCK / WAVE;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;  // 2048
+constexpr int RS_MAX_BLOCKS = 1024;
+
+__global__ __launch_bounds__(RS_BLOCK) void k_radix_count(const uint64_t *__restrict__ keys,
+                                                          int64_t n, int shift,
+                                                          int64_t tiles_per_block, int64_t n_tiles,
+                                                          uint32_t *__restrict__ counts, int G) {
+  __shared__ uint32_t h[RS_WAVES][256];
+  const int tid = threadIdx.x, w = tid / WAVE;
+  for (int i = tid; i < RS_WAVES * 256; i += RS_BLOCK) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t t0 = (int64_t)blockIdx.x * tiles_per_block;
+  const int64_t t1 = min(t0 + tiles_per_block, n_tiles);
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t base = t * RS_TILE;
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      int64_t idx = base + (int64_t)i * RS_BLOCK + tid;
+      if (idx < n) {
+        uint32_t d = (uint32_t)(keys[idx] >> shift) & 255u;
+        atomicAdd(&h[w][d], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < RS_WAVES; ++k) s += h[k][tid];
+  counts[(int64_t)tid * G + blockIdx.x] = s;
+}
+
+template <bool VALS>
+__global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
+    const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+    uint32_t *__restrict__ vout, int64_t n, int shift, int64_t tiles_per_block, int64_t n_tiles,
+    const uint32_t *__restrict__ offsets, int G) {
+  __shared__ uint32_t s_off[256];
+  __shared__ uint32_t s_cnt[RS_WAVES][256];
+  const int tid = threadIdx.x, w = tid / WAVE, l = tid % WAVE;
+  s_off[tid] = offsets[(int64_t)tid * G + blockIdx.x];
+  const uint64_t lt = lanemask_lt();
+  const int64_t t0 = (int64_t)blockIdx.x * tiles_per_block;
+  const int64_t t1 = min(t0 + tiles_per_block, n_tiles);
+  for (int64_t t = t0; t < t1; ++t) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s_cnt[w][l + 64 * k] = 0;
+    __syncthreads();
+    // wave w owns elements [base_w, base_w + 64*ITEMS); slot i, lane l
+    const int64_t base_w = t * RS_TILE + (int64_t)w * WAVE * RS_ITEMS;
+    uint64_t k[RS_ITEMS];
+    uint32_t v[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const int64_t idx = base_w + (int64_t)i * WAVE + l;
+      const bool valid = idx < n;
+      k[i] = valid ? kin[idx] : 0ull;
+      if (VALS) v[i] = valid ? vin[idx] : 0u;
+      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      const uint32_t before = (uint32_t)__popcll(peers & lt);
+      uint32_t prev = 0;
+      if (valid) prev = s_cnt[w][d];
+      rank[i] = prev + before;
+      if (valid && before == 0) s_cnt[w][d] = prev + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+      const int d = tid;
+      uint32_t c[RS_WAVES];
+#pragma unroll
+      for (int q = 0; q < RS_WAVES; ++q) c[q] = s_cnt[q][d];
+      uint32_t run = s_off[d];
+#pragma unroll
+      for (int q = 0; q < RS_WAVES; ++q) {
+        s_cnt[q][d] = run;
+        run += c[q];
+      }
+      s_off[d] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const int64_t idx = base_w + (int64_t)i * WAVE + l;
+      if (idx < n) {
+        const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+        const uint32_t pos = s_cnt[w][d] + rank[i];
+        kout[pos] = k[i];
+        if (VALS) vout[pos] = v[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- exclusive scan, three kernels (chunk sums, scan of sums, down-sweep)
+constexpr int SC_BLOCK = 1024;
+constexpr int SC_ITEMS = 4;
+constexpr int SC_CHUNK = SC_BLOCK * SC_ITEMS;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wave,
+                                                    uint32_t *total) {
+  const int tid = threadIdx.x, w = tid / WAVE, l = tid % WAVE;
+  const int nw = blockDim.x / WAVE;
+  uint32_t inc = wave_incl_scan_u32(v);
+  if (l == WAVE - 1) s_wave[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t x = l < nw ? s_wave[l] : 0u;
+    uint32_t xi = wave_incl_scan_u32(x);
+    if (l < nw) s_wave[l] = xi - x;
+    if (l == nw - 1) s_wave[nw] = xi;
+  }
+  __syncthreads();
+  uint32_t r = inc - v + s_wave[w];
+  *total = s_wave[nw];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_reduce(const uint32_t *__restrict__ in,
+                                                          int64_t n, uint32_t *__restrict__ sums) {
+  __shared__ uint32_t s[SC_BLOCK / WAVE + 1];
+  const int64_t base = (int64_t)blockIdx.x * SC_CHUNK + (int64_t)threadIdx.x * SC_ITEMS;
+  uint32_t v = 0;
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i)
+    if (base + i < n) v += in[base + i];
+  v = wave_sum_u32(v);
+  if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < SC_BLOCK / WAVE; ++i) t += s[i];
+    sums[blockIdx.x] = t;
+  }
+}
+
+// one block; nb <= SC_CHUNK; writes exclusive scan in place and the total
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_top(uint32_t *__restrict__ sums, int64_t nb,
+                                                       uint32_t *__restrict__ total_out) {
+  __shared__ uint32_t s[SC_BLOCK / WAVE + 1];
+  const int64_t base = (int64_t)threadIdx.x * SC_ITEMS;
+  uint32_t x[SC_ITEMS];
+  uint32_t v = 0;
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    x[i] = base + i < nb ? sums[base + i] : 0u;
+    v += x[i];
+  }
+  uint32_t tot;
+  uint32_t r = block_excl_scan(v, s, &tot);
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    if (base + i < nb) sums[base + i] = r;
+    r += x[i];
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = tot;
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_down(const uint32_t *__restrict__ in, int64_t n,
+                                                        const uint32_t *__restrict__ sums,
+                                                        uint32_t *__restrict__ out) {
+  __shared__ uint32_t s[SC_BLOCK / WAVE + 1];
+  const int64_t base = (int64_t)blockIdx.x * SC_CHUNK + (int64_t)threadIdx.x * SC_ITEMS;
+  uint32_t x[SC_ITEMS];
+  uint32_t v = 0;
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    x[i] = base + i < n ? in[base + i] : 0u;
+    v += x[i];
+  }
+  uint32_t tot;
+  uint32_t r = block_excl_scan(v, s, &tot) + sums[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    if (base + i < n) out[base + i] = r;
+    r += x[i];
+  }
+}
+
+}  // namespace
+
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp, hipStream_t st,
+                       uint32_t *total_out_dev) {
+  if (n <= 0) {
+    if (total_out_dev) PPF_HIP(hipMemsetAsync(total_out_dev, 0, 4, st));
+    return 0;
+  }
+  const int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
+  if (nb > SC_CHUNK) {
+    set_error("exclusive_scan_u32: input too long");
+    return 1;
+  }
+  PPF_TRY(tmp.ensure((size_t)nb * 4));
+  uint32_t *sums = tmp.as<uint32_t>();
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BLOCK), 0, st, in, n, sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SC_BLOCK), 0, st, sums, nb, total_out_dev);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SC_BLOCK), 0, st, in, n, sums, out);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
+                   SortScratch &s, hipStream_t st) {
+  if (n <= 1 || bit_hi <= bit_lo) return 0;
+  if (n >= (int64_t)1 << 32) {
+    set_error("radix_sort_u64: more than 2^32 elements");
+    return 1;
+  }
+  const int64_t n_tiles = (n + RS_TILE - 1) / RS_TILE;
+  const int G = (int)std::min<int64_t>(n_tiles, RS_MAX_BLOCKS);
+  const int64_t tpb = (n_tiles + G - 1) / G;
+  PPF_TRY(s.counts.ensure((size_t)256 * G * 4 * 2));
+  PPF_TRY(s.tmp_keys.ensure((size_t)n * 8));
+  if (vals) PPF_TRY(s.tmp_vals.ensure((size_t)n * 4));
+  uint32_t *counts = s.counts.as<uint32_t>();
+  uint32_t *offs = counts + (size_t)256 * G;
+  uint64_t *ka = keys, *kb = s.tmp_keys.as<uint64_t>();
+  uint32_t *va = vals, *vb = vals ? s.tmp_vals.as<uint32_t>() : nullptr;
+  int passes = 0;
+  for (int shift = bit_lo; shift < bit_hi; shift += 8, ++passes) {
+    hipLaunchKernelGGL(k_radix_count, dim3(G), dim3(RS_BLOCK), 0, st, ka, n, shift, tpb, n_tiles,
+                       counts, G);
+    PPF_TRY(exclusive_scan_u32(counts, offs, (int64_t)256 * G, s.scan, st, nullptr));
+    if (vals)
+      hipLaunchKernelGGL(k_radix_scatter<true>, dim3(G), dim3(RS_BLOCK), 0, st, ka, va, kb, vb, n,
+                         shift, tpb, n_tiles, offs, G);
+    else
+      hipLaunchKernelGGL(k_radix_scatter<false>, dim3(G), dim3(RS_BLOCK), 0, st, ka, va, kb, vb, n,
+                         shift, tpb, n_tiles, offs, G);
+    PPF_HIP(hipGetLastError());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (passes & 1) {
+    PPF_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+    if (vals) PPF_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  return 0;
+}
+
+}  // namespace ppfi

@@ -1,0 +1,367 @@
+"""Python binding of libppf_amd.so, mirroring the reference's host API.
+
+The reference exposes ``Scene(cloud, d_dist, ref_point_downsample_factor)``
+(include/scene.h:15-16), ``Model(cloud, d_dist, vote_count_threshold,
+cpu_clustering, use_l1_norm, use_averaged_clusters)`` (include/model.h:17-19)
+with ``Model.ppf_lookup(scene)`` (include/model.h:35), and
+``ppf_registration(...)`` (include/ppf.h:9-15).  The same names and argument
+meanings are kept; clouds are (points[n,3], normals[n,3]) float32 arrays.
+
+Every call goes through the C ABI in include/ppf_amd.h into hand-written HIP
+kernels.  There is no CPU fallback: if libppf_amd.so is missing or no GPU is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libppf_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "ppf_amd.h")
+
+PPF_OK = 0
+PPF_ERR_ARG = 1
+PPF_ERR_HIP = 2
+PPF_ERR_NOMEM = 3
+PPF_ERR_NO_VOTES = 4
+PPF_ERR_STATE = 5
+MAX_MODEL_POINTS = 65535
+
+
+class PPFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ModelInfo(C.Structure):
+    _fields_ = [("n_points", C.c_int64), ("d_dist", C.c_float), ("n_pairs", C.c_int64),
+                ("n_unique_keys", C.c_int64), ("tile_points", C.c_int32), ("n_tiles", C.c_int32),
+                ("device_bytes", C.c_uint64)]
+
+
+class LookupParams(C.Structure):
+    _fields_ = [("vote_count_threshold", C.c_float), ("cpu_clustering", C.c_int32),
+                ("use_l1_norm", C.c_int32), ("use_averaged_clusters", C.c_int32),
+                ("exact_votes", C.c_int32)]
+
+
+class ResultInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("lookups", C.c_int64), ("max_count", C.c_int64),
+                ("n_candidates", C.c_int64), ("n_kept", C.c_int64), ("max_idx", C.c_int64),
+                ("n_clusters", C.c_int64), ("vote_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+_lib = None
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libppf_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PPFError(PPF_ERR_STATE, f"{LIB_PATH} is missing: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    L.ppf_last_error.restype = C.c_char_p
+    L.ppf_ctx_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(_vp)]
+    L.ppf_ctx_destroy.argtypes = [_vp]
+    L.ppf_ctx_device_bytes.argtypes = [_vp]
+    L.ppf_ctx_device_bytes.restype = C.c_uint64
+    L.ppf_model_build.argtypes = [_vp, _f32p, _f32p, C.c_int64, C.c_float, C.POINTER(_vp)]
+    L.ppf_model_destroy.argtypes = [_vp]
+    L.ppf_model_info.argtypes = [_vp, C.POINTER(ModelInfo)]
+    L.ppf_model_hash_array.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.ppf_model_set_weights.argtypes = [_vp, _f32p, C.c_int64]
+    L.ppf_scene_upload.argtypes = [_vp, _f32p, _f32p, C.c_int64, C.c_uint32, C.POINTER(_vp)]
+    L.ppf_scene_destroy.argtypes = [_vp]
+    L.ppf_scene_keys.argtypes = [_vp, C.c_float, _u32p]
+    L.ppf_lookup.argtypes = [_vp, _vp, C.POINTER(LookupParams), C.POINTER(_vp)]
+    L.ppf_result_destroy.argtypes = [_vp]
+    L.ppf_result_info.argtypes = [_vp, C.POINTER(ResultInfo)]
+    L.ppf_result_votes.argtypes = [_vp, _vp, _vp]
+    L.ppf_result_transforms.argtypes = [_vp, _f32p]
+    L.ppf_result_clusters.argtypes = [_vp, _vp, _vp, _vp]
+    L.ppf_result_pose.argtypes = [_vp, _f32p]
+    L.ppf_registration.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp, _vp,
+                                   C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.c_int32, _vp,
+                                   _f32p, _vp]
+    L.ppf_debug_numerics.argtypes = [_vp, C.c_int32, _f32p, _vp, _f32p, C.c_int64]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != PPF_OK:
+        raise PPFError(rc, lib().ppf_last_error().decode())
+
+
+def _cloud(points, normals) -> Tuple[np.ndarray, np.ndarray]:
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+    if p.shape != n.shape:
+        raise ValueError("points and normals must have the same shape")
+    return p, n
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One device + one HIP stream (the reference's device selection, ppf.cu:37-49)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        h = _vp()
+        _check(lib().ppf_ctx_create(device, hbm_budget_bytes, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ppf_ctx_destroy(self.handle)
+            self.handle = None
+
+    def device_bytes(self) -> int:
+        return int(lib().ppf_ctx_device_bytes(self.handle))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def debug_numerics(self, op: int, a: np.ndarray, b: Optional[np.ndarray] = None) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        n = len(a) // 2 if op == 8 else len(a)
+        out = np.zeros(n, np.float32)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float32)
+        _check(lib().ppf_debug_numerics(self.handle, op, a, _ptr(bb), out, n))
+        return out
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Scene:
+    """Scene(cloud, d_dist, ref_point_downsample_factor=1) -- include/scene.h:15-16.
+
+    d_dist is accepted for signature parity; scene features are quantized with
+    the model's d_dist at lookup time (the reference rebuilds the scene per
+    model for the same reason, src/cuda/ppf.cu:64-67)."""
+
+    def __init__(self, cloud, d_dist: float = 0.0, ref_point_downsample_factor: int = 1,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        p, n = _cloud(*cloud)
+        self.n = len(p)
+        self.d_dist = d_dist
+        self.rpdf = int(ref_point_downsample_factor)
+        h = _vp()
+        _check(lib().ppf_scene_upload(self.ctx.handle, p, n, self.n, self.rpdf, C.byref(h)))
+        self.handle = h
+
+    def numPoints(self) -> int:
+        return self.n
+
+    def getHashKeys(self, d_dist: Optional[float] = None) -> np.ndarray:
+        """All n*n keys [s_r][s_i] for d_dist (Scene::getHashKeys)."""
+        out = np.zeros(self.n * self.n, np.uint32)
+        _check(lib().ppf_scene_keys(self.handle, float(d_dist if d_dist else self.d_dist), out))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ppf_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LookupResult:
+    def __init__(self, handle, cpu: bool):
+        self.handle = handle
+        self.cpu = cpu
+        info = ResultInfo()
+        _check(lib().ppf_result_info(handle, C.byref(info)))
+        self.info = {k: getattr(info, k) for k, _ in ResultInfo._fields_}
+
+    @property
+    def n_kept(self) -> int:
+        return int(self.info["n_kept"])
+
+    def votes(self) -> Tuple[np.ndarray, np.ndarray]:
+        P = self.n_kept
+        codes = np.zeros(max(P, 1), np.uint64)
+        counts = np.zeros(max(P, 1), np.uint32)
+        _check(lib().ppf_result_votes(self.handle, codes.ctypes.data, counts.ctypes.data))
+        return codes[:P], counts[:P]
+
+    def transformations(self) -> np.ndarray:
+        P = self.n_kept
+        T = np.zeros(max(P, 1) * 16, np.float32)
+        _check(lib().ppf_result_transforms(self.handle, T))
+        return T[:16 * P].reshape(P, 4, 4)
+
+    def clusters(self):
+        P = self.n_kept
+        vc = np.zeros(max(P, 1), np.float32)
+        tr = np.zeros(max(P, 1) * 3, np.float32)
+        q = np.zeros(max(P, 1) * 4, np.float32)
+        _check(lib().ppf_result_clusters(self.handle, vc.ctypes.data, tr.ctypes.data, q.ctypes.data))
+        return vc[:P], tr[:3 * P].reshape(P, 3), q[:4 * P].reshape(P, 4)
+
+    def pose(self) -> np.ndarray:
+        out = np.zeros(16, np.float32)
+        rc = lib().ppf_result_pose(self.handle, out)
+        if rc not in (PPF_OK, PPF_ERR_NO_VOTES):
+            _check(rc)
+        return out.reshape(4, 4)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ppf_result_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Model:
+    """Model(cloud, d_dist, vote_count_threshold, cpu_clustering, use_l1_norm,
+    use_averaged_clusters) -- include/model.h:17-19, src/cuda/model.cu:43-82."""
+
+    def __init__(self, cloud, d_dist: float, vote_count_threshold: float = 0.4,
+                 cpu_clustering: bool = False, use_l1_norm: bool = False,
+                 use_averaged_clusters: bool = False, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        p, n = _cloud(*cloud)
+        self.n = len(p)
+        self.d_dist = float(d_dist)
+        self.vote_count_threshold = float(vote_count_threshold)
+        self.cpu_clustering = bool(cpu_clustering)
+        self.use_l1_norm = bool(use_l1_norm)
+        self.use_averaged_clusters = bool(use_averaged_clusters)
+        h = _vp()
+        _check(lib().ppf_model_build(self.ctx.handle, p, n, self.n, self.d_dist, C.byref(h)))
+        self.handle = h
+        self.result: Optional[LookupResult] = None
+
+    def info(self) -> dict:
+        i = ModelInfo()
+        _check(lib().ppf_model_info(self.handle, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in ModelInfo._fields_}
+
+    def hash_array(self):
+        """(hashkeys, counts, firstHashkeyIndex, hashkeyToDataMap) --
+        ParallelHashArray getters (include/impl/parallel_hash_array.hpp:94-112)."""
+        U = int(self.info()["n_unique_keys"])
+        hk = np.zeros(U, np.uint32)
+        cnt = np.zeros(U, np.uint32)
+        first = np.zeros(U, np.uint32)
+        k2d = np.zeros(self.n * self.n, np.uint32)
+        _check(lib().ppf_model_hash_array(self.handle, hk.ctypes.data, cnt.ctypes.data,
+                                          first.ctypes.data, k2d.ctypes.data))
+        return hk, cnt, first, k2d
+
+    def SetModelPointVoteWeights(self, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        _check(lib().ppf_model_set_weights(self.handle, w, len(w)))
+
+    def ppf_lookup(self, scene: Scene, exact_votes: bool = False) -> LookupResult:
+        prm = LookupParams(self.vote_count_threshold, int(self.cpu_clustering),
+                           int(self.use_l1_norm), int(self.use_averaged_clusters),
+                           int(exact_votes))
+        h = _vp()
+        _check(lib().ppf_lookup(self.handle, scene.handle, C.byref(prm), C.byref(h)))
+        self.result = LookupResult(h, self.cpu_clustering)
+        return self.result
+
+    # reference-style accessors of the last lookup
+    @property
+    def max_idx(self) -> int:
+        return int(self.result.info["max_idx"]) if self.result else -1
+
+    def getTransformations(self) -> np.ndarray:
+        return self.result.transformations()
+
+    def close(self):
+        if self.result is not None:
+            self.result.close()
+        if getattr(self, "handle", None):
+            lib().ppf_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ppf_registration(scene_clouds: Sequence, model_clouds: Sequence, model_d_dists: Sequence[float],
+                     ref_point_downsample_factor: int, vote_count_threshold: float,
+                     cpu_clustering: bool, use_l1_norm: bool = False,
+                     use_averaged_clusters: bool = False, devUse: int = 0,
+                     model_weights: Optional[Sequence] = None, ctx: Optional[Context] = None,
+                     return_status: bool = False):
+    """Pose of every model in every scene: list[scene][model] of 4x4 float32
+    (include/ppf.h:9-15, src/cuda/ppf.cu:29-106)."""
+    ctx = ctx or Context(devUse)
+    scenes = [_cloud(*c) for c in scene_clouds]
+    models = [_cloud(*c) for c in model_clouds]
+    ns, nm = len(scenes), len(models)
+    sp = (C.c_void_p * max(ns, 1))(*[s[0].ctypes.data for s in scenes])
+    sn = (C.c_void_p * max(ns, 1))(*[s[1].ctypes.data for s in scenes])
+    ss = np.array([len(s[0]) for s in scenes] or [0], np.int64)
+    mp = (C.c_void_p * max(nm, 1))(*[m[0].ctypes.data for m in models])
+    mn = (C.c_void_p * max(nm, 1))(*[m[1].ctypes.data for m in models])
+    ms = np.array([len(m[0]) for m in models] or [0], np.int64)
+    dd = np.ascontiguousarray(model_d_dists, dtype=np.float32)
+    wts = None
+    keep = []
+    if model_weights is not None:
+        arrs = [None if w is None else np.ascontiguousarray(w, dtype=np.float32) for w in model_weights]
+        keep = arrs
+        wts = (C.c_void_p * max(nm, 1))(*[0 if a is None else a.ctypes.data for a in arrs])
+    poses = np.zeros(max(ns * nm, 1) * 16, np.float32)
+    status = np.zeros(max(ns * nm, 1), np.int32)
+    _check(lib().ppf_registration(ctx.handle, ns, sp, sn, ss.ctypes.data, nm, mp, mn, ms.ctypes.data,
+                                  dd.ctypes.data, int(ref_point_downsample_factor),
+                                  float(vote_count_threshold), int(cpu_clustering), int(use_l1_norm),
+                                  int(use_averaged_clusters), wts, poses, status.ctypes.data))
+    del keep
+    P = poses[:ns * nm * 16].reshape(ns, nm, 4, 4)
+    out = [[P[i, j].copy() for j in range(nm)] for i in range(ns)]
+    if return_status:
+        return out, status[:ns * nm].reshape(ns, nm)
+    return out
+
+
+def exported_symbols_from_header(path: str = HEADER) -> List[str]:
+    """Names of the functions declared in include/ppf_amd.h."""
+    import re
+    text = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(ppf_[a-z_0-9]+)\s*\(", text,
+                                 flags=re.M)))

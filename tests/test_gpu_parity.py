@@ -1,0 +1,228 @@
+"""GPU parity: libppf_amd.so (gfx950 kernels via the C ABI) against the CPU
+oracle on the same seeded inputs.  Integer results (keys, hash array, vote
+codes/counts, argmax) must be bit-exact; poses are compared bit-exact where
+the device evaluates the same float expressions as the oracle, and within
+1e-5 (absolute, per matrix entry) for the host greedy clustering average.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import ppf_amd
+import ppf_oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def medium():
+    w = synth.plain_small(seed=3, n_model=500, n_clutter=700)
+    return w.models[0], w.scenes[0], w.d_dists[0]
+
+
+@pytest.mark.parametrize("op", range(9))
+def test_device_numerics_match_host(ctx, op):
+    rng = np.random.default_rng(op)
+    n = 1 << 18
+    if op == 0:
+        a = rng.uniform(-1.05, 1.05, n).astype(np.float32)
+        b = None
+    elif op in (2, 3):
+        a = rng.uniform(-3.3, 3.3, n).astype(np.float32)
+        b = None
+    elif op == 7:
+        a = np.abs(rng.normal(size=n) * np.exp2(rng.integers(-30, 30, n))).astype(np.float32)
+        b = None
+    elif op == 8:
+        a = rng.normal(size=2 * n).astype(np.float32)
+        b = rng.normal(size=2 * n).astype(np.float32)
+        a[:64] = np.array([1, 0, 1, -0.0, 0, 1, -0.0, 0] * 8, np.float32)
+        b[:64] = np.array([-1, 0, -1, -0.0, 0, -1, 0, -1] * 8, np.float32)
+    else:
+        a = (rng.normal(size=n) * np.exp2(rng.integers(-12, 12, n))).astype(np.float32)
+        b = (rng.normal(size=n) * np.exp2(rng.integers(-12, 4, n))).astype(np.float32)
+        if op in (4, 5):
+            a = np.abs(a) if op == 5 else a
+            b = np.abs(b) + np.float32(1e-3)
+    dev = ctx.debug_numerics(op, a, b)
+    host = O.numerics(op, a, b)
+    assert np.array_equal(_bits(dev), _bits(host)), f"op {op}: {np.sum(_bits(dev) != _bits(host))} differ"
+
+
+def test_model_hash_array_matches_oracle(ctx, medium):
+    m, _, d = medium
+    model = ppf_amd.Model(m, d, ctx=ctx)
+    hk, cnt, first, k2d = model.hash_array()
+    keys = O.cloud_keys(m[0], m[1], 1, d)
+    ok, oc, of, ok2d = O.hash_array(keys)
+    assert np.array_equal(hk, ok)
+    assert np.array_equal(cnt, oc.astype(np.uint32))
+    assert np.array_equal(first, of.astype(np.uint32))
+    assert np.array_equal(k2d, ok2d.astype(np.uint32))
+    info = model.info()
+    assert info["n_unique_keys"] == len(ok) and info["n_pairs"] == len(m[0]) ** 2
+
+
+def test_scene_keys_match_oracle(ctx, medium):
+    _, s, d = medium
+    scene = ppf_amd.Scene(s, d, 5, ctx=ctx)
+    assert np.array_equal(scene.getHashKeys(d), O.cloud_keys(s[0], s[1], 5, d))
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_votes_match_oracle(ctx, medium, exact):
+    m, s, d = medium
+    model = ppf_amd.Model(m, d, 0.3, ctx=ctx)
+    scene = ppf_amd.Scene(s, d, 5, ctx=ctx)
+    res = model.ppf_lookup(scene, exact_votes=exact)
+    codes, counts = res.votes()
+    oc, on, st = O.vote(m, s, d, 5, 0.3)
+    assert res.info["max_count"] == st["max_count"]
+    assert res.info["lookups"] == st["lookups"]
+    assert np.array_equal(codes, oc) and np.array_equal(counts, on)
+
+
+def test_tiny_golden_case(ctx):
+    g = np.load(os.path.join(GOLDEN, "tiny_case.npz"))
+    m = (g["model_points"], g["model_normals"])
+    s = (g["scene_points"], g["scene_normals"])
+    d = float(g["d_dist"])
+    model = ppf_amd.Model(m, d, float(g["thr"]), ctx=ctx)
+    res = model.ppf_lookup(ppf_amd.Scene(s, d, int(g["rpdf"]), ctx=ctx))
+    codes, counts = res.votes()
+    assert np.array_equal(codes, g["codes"]) and np.array_equal(counts, g["counts"])
+    assert np.array_equal(_bits(res.transformations()), _bits(g["transforms"]))
+    assert np.array_equal(_bits(res.pose()), _bits(g["pose"]))
+
+
+def test_poses_and_spatial_clustering_match_oracle(ctx, medium):
+    m, s, d = medium
+    model = ppf_amd.Model(m, d, 0.3, ctx=ctx)
+    res = model.ppf_lookup(ppf_amd.Scene(s, d, 5, ctx=ctx))
+    codes, counts = res.votes()
+    T = res.transformations()
+    assert np.array_equal(_bits(T), _bits(O.poses(codes, m, s)))
+    vc, tr, q = res.clusters()
+    ref = O.cluster_spatial(T, codes, counts, d)
+    assert np.array_equal(_bits(q), _bits(ref["quats"]))
+    assert np.array_equal(_bits(vc), _bits(ref["vote_counts"]))
+    assert np.array_equal(_bits(tr), _bits(ref["trans"]))
+    assert res.info["max_idx"] == ref["max_idx"]
+    opose, st = O.register(m, s, d, 5, 0.3, False)
+    assert np.array_equal(_bits(res.pose()), _bits(opose))
+
+
+def test_cpu_clustering_pose_matches_oracle(ctx, medium):
+    m, s, d = medium
+    model = ppf_amd.Model(m, d, 0.3, cpu_clustering=True, ctx=ctx)
+    res = model.ppf_lookup(ppf_amd.Scene(s, d, 5, ctx=ctx))
+    opose, st = O.register(m, s, d, 5, 0.3, True)
+    assert res.info["n_clusters"] == st["cluster"]
+    assert np.allclose(res.pose(), opose, atol=1e-5, rtol=0)
+
+
+def test_plain_small_full_against_golden(ctx):
+    with open(os.path.join(GOLDEN, "plain_small_oracle.json")) as f:
+        gold = json.load(f)
+    w = synth.plain_small()
+    m, s, d = w.models[0], w.scenes[0], w.d_dists[0]
+    model = ppf_amd.Model(m, d, 0.3, cpu_clustering=True, ctx=ctx)
+    scene = ppf_amd.Scene(s, d, 5, ctx=ctx)
+    res = model.ppf_lookup(scene)
+    codes, counts = res.votes()
+    h = hashlib.sha256(codes.tobytes() + counts.tobytes()).hexdigest()
+    assert res.info["max_count"] == gold["stats"]["max_count"]
+    assert res.info["n_kept"] == gold["stats"]["kept"]
+    assert h == gold["votes_sha256"]
+    assert hashlib.sha256(res.transformations().tobytes()).hexdigest() == gold["poses_sha256"]
+    assert res.info["n_clusters"] == gold["cpu_n_clusters"]
+    assert np.allclose(res.pose(), np.array(gold["pose_cpu_clustering"]), atol=1e-5, rtol=0)
+    g2 = ppf_amd.Model(m, d, 0.3, cpu_clustering=False, ctx=ctx).ppf_lookup(scene)
+    assert g2.info["max_idx"] == gold["gpu_max_idx"]
+    assert np.array_equal(g2.pose(), np.array(gold["pose_gpu_clustering"], np.float32))
+
+
+def test_fast_bins_equal_exact_bins_dataset_like(ctx):
+    # full-size scene of the dataset-like workload, one model: the fast alpha
+    # path must give the same kept votes as evaluating every vote exactly
+    w = synth.dataset_like()
+    m, s, d = w.models[0], w.scenes[0], w.d_dists[0]
+    model = ppf_amd.Model(m, d, 0.3, ctx=ctx)
+    scene = ppf_amd.Scene(s, d, 25, ctx=ctx)
+    fast = model.ppf_lookup(scene)
+    cf, nf = fast.votes()
+    exact = model.ppf_lookup(scene, exact_votes=True)
+    ce, ne = exact.votes()
+    assert fast.info["max_count"] == exact.info["max_count"]
+    assert np.array_equal(cf, ce) and np.array_equal(nf, ne)
+    # run twice: identical
+    again = model.ppf_lookup(scene)
+    c2, n2 = again.votes()
+    assert np.array_equal(cf, c2) and np.array_equal(nf, n2)
+    # every kept pose maps its model reference point onto its scene point
+    T = fast.transformations()
+    for code, M in zip(cf[:200], T[:200]):
+        sr, mr = int(code) >> 32, (int(code) & 0xFFFFFFFF) >> 6
+        assert np.allclose(M[:3, :3] @ m[0][mr] + M[:3, 3], s[0][sr], atol=1e-3)
+
+
+def test_registration_api_matches_lookups(ctx):
+    w1 = synth.plain_small(seed=5, n_model=300, n_clutter=300)
+    w2 = synth.plain_small(seed=6, n_model=250, n_clutter=400)
+    models = [w1.models[0], w2.models[0]]
+    scenes = [w1.scenes[0], w2.scenes[0]]
+    dd = [w1.d_dists[0], w2.d_dists[0]]
+    poses, status = ppf_amd.ppf_registration(scenes, models, dd, 3, 0.3, False, ctx=ctx,
+                                             return_status=True)
+    for i, s in enumerate(scenes):
+        for j, m in enumerate(models):
+            res = ppf_amd.Model(m, dd[j], 0.3, ctx=ctx).ppf_lookup(ppf_amd.Scene(s, dd[j], 3, ctx=ctx))
+            assert np.array_equal(poses[i][j], res.pose())
+            assert status[i, j] == res.info["status"]
+
+
+def test_model_weights_scale_cluster_votes(ctx, medium):
+    m, s, d = medium
+    rng = np.random.default_rng(4)
+    wts = rng.uniform(0.5, 2.0, len(m[0])).astype(np.float32)
+    model = ppf_amd.Model(m, d, 0.3, ctx=ctx)
+    model.SetModelPointVoteWeights(wts)
+    res = model.ppf_lookup(ppf_amd.Scene(s, d, 5, ctx=ctx))
+    codes, counts = res.votes()
+    vc, _, _ = res.clusters()
+    ref = O.cluster_spatial(res.transformations(), codes, counts, d, weights=wts)
+    assert np.array_equal(_bits(vc), _bits(ref["vote_counts"]))
+    assert res.info["max_idx"] == ref["max_idx"]
+
+
+def test_degenerate_inputs(ctx):
+    m = synth.ellipsoid_box_model(50, 1)
+    far = (m[0] * 0 + np.float32(1000.0), m[1])
+    model = ppf_amd.Model(m, 0.05, 0.3, ctx=ctx)
+    # two coincident points far away: every key misses -> no votes, defined status
+    s = (np.full((2, 3), 1000.0, np.float32), np.tile(np.array([0, 0, 1], np.float32), (2, 1)))
+    res = model.ppf_lookup(ppf_amd.Scene(s, 0.05, 1, ctx=ctx))
+    assert res.info["status"] == ppf_amd.PPF_ERR_NO_VOTES and res.n_kept == 0
+    with pytest.raises(ppf_amd.PPFError):
+        ppf_amd.Model((m[0][:1], m[1][:1]), 0.05, ctx=ctx)
+    with pytest.raises(ppf_amd.PPFError):
+        ppf_amd.Model(m, -1.0, ctx=ctx)
+    del far
+
+
+def test_budget_rejects_allocation():
+    c = ppf_amd.Context(0, hbm_budget_bytes=4096)
+    m = synth.ellipsoid_box_model(200, 2)
+    with pytest.raises(ppf_amd.PPFError) as e:
+        ppf_amd.Model(m, 0.05, ctx=c)
+    assert e.value.code == ppf_amd.PPF_ERR_NOMEM
+    c.close()
