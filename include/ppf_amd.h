@@ -118,6 +118,7 @@ int ppf_result_destroy(ppf_result result);
 typedef struct {
   int32_t status;          /* PPF_OK or PPF_ERR_NO_VOTES */
   int64_t lookups;         /* scene pairs looked up: R*(n_s-1) */
+  int64_t votes;           /* nonunique votes: matched (scene pair, model pair) combinations */
   int64_t max_count;       /* largest (s_r, m_r, alpha) count */
   int64_t n_candidates;    /* (code,count) pairs emitted before the final threshold */
   int64_t n_kept;          /* P: votes kept after the threshold */
@@ -156,6 +157,10 @@ int ppf_registration(ppf_ctx ctx, int32_t n_scenes, const float *const *scene_po
                      int32_t use_averaged_clusters, const float *const *model_weights,
                      float *poses_out, int32_t *status_out);
 
+/* sizeof(ppf_model_info_t), sizeof(ppf_lookup_params), sizeof(ppf_result_info_t):
+ * lets a binding check that it was written against the same header. */
+void ppf_abi_sizes(uint32_t sizes_out[3]);
+
 /* ------------------------------------------------------------- diagnostics
  * Evaluate one numerics primitive on the device for n inputs (used to check
  * that device and host rounding agree).  op: 0 acos, 1 atan2(a,b), 2 sin,
@@ -163,6 +168,14 @@ int ppf_registration(ppf_ctx ctx, int32_t n_scenes, const float *const *scene_po
  * float(a0,a1,b0,b1) from 4-vectors packed in a (n*2) and b (n*2). */
 int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, float *out,
                        int64_t n);
+
+/* Run one of the library's device primitives on host data (for tests):
+ * op 0: exclusive scan of n uint32 (keys_io as uint32[n], result in place; the
+ *       total is returned in vals_io[0]);
+ * op 1: stable radix sort of n uint64 keys_io carrying uint32 vals_io (may be
+ *       NULL) on bits [bit_lo, bit_hi). */
+int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
+                        int32_t bit_lo, int32_t bit_hi);
 
 #ifdef __cplusplus
 }

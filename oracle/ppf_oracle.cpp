@@ -913,6 +913,32 @@ int oracle_vote_throughput(const float *m_pts, const float *m_nrm, int64_t n_m,
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }
 
+void *oracle_model_create(const float *m_pts, const float *m_nrm, int64_t n_m, float d_dist) {
+  if (n_m < 1 || n_m >= 65536) return nullptr;
+  return new ModelDesc(build_model(m_pts, m_nrm, n_m, d_dist));
+}
+
+void oracle_model_destroy(void *h) { delete static_cast<ModelDesc *>(h); }
+
+int oracle_vote_throughput_h(void *h, const float *m_pts, const float *m_nrm, const float *s_pts,
+                             const float *s_nrm, int64_t n_s, float d_dist, const int64_t *ref_list,
+                             int64_t n_ref, int64_t *stats, double *seconds) {
+  if (!h) return 1;
+  const ModelDesc &md = *static_cast<ModelDesc *>(h);
+  std::vector<int64_t> refs(ref_list, ref_list + n_ref);
+  std::vector<VoteEntry> e;
+  int64_t L, H, V;
+  auto t0 = std::chrono::steady_clock::now();
+  vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, 1, refs, e, &L, &H, &V);
+  auto t1 = std::chrono::steady_clock::now();
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  stats[0] = L;
+  stats[1] = H;
+  stats[2] = V;
+  stats[3] = (int64_t)e.size();
+  return 0;
+}
+
 // Host evaluation of the same primitives as ppf_debug_numerics.  fmod,
 // quant_downf and the alpha bin use the oracle's own restatement (glibc fmodf,
 // lrintf); the transcendentals are the shared ones.

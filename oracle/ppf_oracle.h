@@ -89,6 +89,13 @@ int oracle_vote_throughput(const float *m_pts, const float *m_nrm, int64_t n_m,
 
 int oracle_num_threads(void);
 
+/* ---- model handle for repeated timing calls (the hash array is built once) */
+void *oracle_model_create(const float *m_pts, const float *m_nrm, int64_t n_m, float d_dist);
+void oracle_model_destroy(void *h);
+int oracle_vote_throughput_h(void *h, const float *m_pts, const float *m_nrm, const float *s_pts,
+                             const float *s_nrm, int64_t n_s, float d_dist, const int64_t *ref_list,
+                             int64_t n_ref, int64_t *stats, double *seconds);
+
 /* ---- host evaluation of the primitives of ppf_debug_numerics (same op codes) */
 int oracle_numerics(int op, const float *a, const float *b, float *out, int64_t n);
 

@@ -64,6 +64,12 @@ def lib():
                                              C.POINTER(C.c_double)]
         L.oracle_num_threads.restype = C.c_int
         L.oracle_numerics.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_int64]
+        L.oracle_model_create.argtypes = [_f32p, _f32p, C.c_int64, C.c_float]
+        L.oracle_model_create.restype = C.c_void_p
+        L.oracle_model_destroy.argtypes = [C.c_void_p]
+        L.oracle_vote_throughput_h.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, _f32p, C.c_int64,
+                                               C.c_float, _i64p, C.c_int64, _i64p,
+                                               C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -216,6 +222,34 @@ def numerics(op: int, a, b=None) -> np.ndarray:
     out = np.zeros(n, np.float32)
     assert lib().oracle_numerics(op, a, bb, out, n) == 0
     return out
+
+
+class OracleModel:
+    """Oracle model hash array built once, for repeated vote timing."""
+
+    def __init__(self, model, d_dist):
+        self.model = (_c(model[0], np.float32), _c(model[1], np.float32))
+        self.d_dist = float(d_dist)
+        self.h = lib().oracle_model_create(self.model[0], self.model[1], len(self.model[0]),
+                                           self.d_dist)
+        assert self.h, "oracle_model_create failed"
+
+    def vote_throughput(self, scene, refs):
+        refs = _c(refs, np.int64)
+        stats = np.zeros(8, np.int64)
+        sec = C.c_double()
+        rc = lib().oracle_vote_throughput_h(self.h, self.model[0], self.model[1],
+                                            _c(scene[0], np.float32), _c(scene[1], np.float32),
+                                            len(scene[0]), self.d_dist, refs, len(refs), stats,
+                                            C.byref(sec))
+        assert rc == 0
+        return dict(lookups=int(stats[0]), hits=int(stats[1]), votes=int(stats[2]),
+                    seconds=float(sec.value))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_model_destroy(self.h)
+            self.h = None
 
 
 def num_threads() -> int:

@@ -241,14 +241,20 @@ void greedy_cluster(const std::vector<float> &T, const std::vector<uint32_t> &co
   pose_out[15] = 1.0f;
 }
 
-constexpr uint32_t CAND_CAP0 = 1u << 23;
-constexpr uint32_t CAND_CAP_MAX = 1u << 24;
+constexpr uint32_t CAND_CAP_MIN = 1u << 23;
+constexpr uint32_t CAND_CAP_MAX = 1u << 28;
 
 }  // namespace
 
 extern "C" {
 
 const char *ppf_last_error(void) { return g_err.c_str(); }
+
+void ppf_abi_sizes(uint32_t sizes_out[3]) {
+  sizes_out[0] = (uint32_t)sizeof(ppf_model_info_t);
+  sizes_out[1] = (uint32_t)sizeof(ppf_lookup_params);
+  sizes_out[2] = (uint32_t)sizeof(ppf_result_info_t);
+}
 
 int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
   ARG_CHECK(out, "ppf_ctx_create: out is NULL");
@@ -461,16 +467,19 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   // bound of the global maximum; on overflow the vote pass is repeated with
   // the final maximum preset, which emits exactly the kept votes.
   ppfi::DevBuf cand;
-  uint32_t cap = CAND_CAP0;
+  // candidates per reference point are bounded by the pruning; start at
+  // 2048 per reference point (at least 8M), grow on overflow
+  uint32_t cap = (uint32_t)std::min<int64_t>(CAND_CAP_MAX, std::max<int64_t>(CAND_CAP_MIN, s->R * 2048));
   uint32_t meta_h[2] = {0, 0};
+  int64_t votes = 0;
   float vote_ms = 0.0f;
   for (int attempt = 0;; ++attempt) {
     TRY(cand.ensure((size_t)cap * 12 + 64));
     unsigned long long *cc = cand.as<unsigned long long>();
     uint32_t *cn = reinterpret_cast<uint32_t *>(cc + cap);
-    uint32_t *meta = cn + cap;  // [0] candidates emitted, [1] running maximum
-    const uint32_t preset[2] = {0u, meta_h[1]};
-    HIPC(hipMemcpyAsync(meta, preset, 8, hipMemcpyHostToDevice, st));
+    uint32_t *meta = cn + cap;  // [0] candidates emitted, [1] running maximum, [2..3] votes
+    const uint32_t preset[4] = {0u, meta_h[1], 0u, 0u};
+    HIPC(hipMemcpyAsync(meta, preset, 16, hipMemcpyHostToDevice, st));
     ppfk::VoteJob J{};
     J.sx = s->cloud.x();
     J.sy = s->cloud.y();
@@ -503,11 +512,16 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
     J.cand_count = cn;
     J.cand_n = meta;
     J.cand_cap = cap;
+    J.votes = reinterpret_cast<unsigned long long *>(meta + 2);
     J.item_begin = 0;
     float ms = 0.0f;
     TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)s->R, m->TM, s->cloud.n, p->exact_votes != 0, &ms));
     vote_ms += ms;
-    HIPC(hipMemcpy(meta_h, meta, 8, hipMemcpyDeviceToHost));
+    uint32_t mh[4];
+    HIPC(hipMemcpy(mh, meta, 16, hipMemcpyDeviceToHost));
+    meta_h[0] = mh[0];
+    meta_h[1] = mh[1];
+    votes = (int64_t)(((uint64_t)mh[3] << 32) | mh[2]);
     if (meta_h[0] <= cap) break;
     if (attempt >= 2 || meta_h[0] > CAND_CAP_MAX) {
       set_error("ppf_lookup: kept-vote buffer overflow (" + std::to_string(meta_h[0]) + " candidates)");
@@ -517,6 +531,7 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   }
   const uint32_t n_cand = meta_h[0];
   r->info.lookups = (int64_t)s->R * (s->cloud.n - 1);
+  r->info.votes = votes;
   r->info.max_count = meta_h[1];
   r->info.n_candidates = n_cand;
   r->info.vote_ms = vote_ms;
@@ -712,6 +727,39 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
   TRY(ppfk::debug_numerics(ctx, op, da.as<float>(), db.as<float>(), dout.as<float>(), n));
   HIPC(hipMemcpyAsync(out, dout.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
+  return PPF_OK;
+}
+
+int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
+                        int32_t bit_lo, int32_t bit_hi) {
+  ARG_CHECK(ctx && keys_io && n >= 0, "ppf_debug_primitive: bad arguments");
+  ARG_CHECK(op == 0 || op == 1, "ppf_debug_primitive: unknown op");
+  TRY(set_device(ctx));
+  hipStream_t st = ctx->stream;
+  if (op == 0) {
+    ARG_CHECK(vals_io, "ppf_debug_primitive: scan needs vals_io[0] for the total");
+    ppfi::DevBuf a, b, tmp;
+    TRY(a.ensure((size_t)n * 4 + 8));
+    TRY(b.ensure((size_t)n * 4 + 8));
+    HIPC(hipMemcpy(a.ptr, keys_io, (size_t)n * 4, hipMemcpyHostToDevice));
+    uint32_t *tot = b.as<uint32_t>() + n;
+    TRY(ppfi::exclusive_scan_u32(a.as<uint32_t>(), b.as<uint32_t>(), n, tmp, st, tot));
+    HIPC(hipMemcpyAsync(keys_io, b.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(vals_io, tot, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return PPF_OK;
+  }
+  ARG_CHECK(bit_lo >= 0 && bit_hi <= 64 && bit_lo <= bit_hi, "ppf_debug_primitive: bad bit range");
+  ppfi::DevBuf k, v;
+  TRY(k.ensure((size_t)n * 8 + 8));
+  if (vals_io) TRY(v.ensure((size_t)n * 4 + 8));
+  HIPC(hipMemcpy(k.ptr, keys_io, (size_t)n * 8, hipMemcpyHostToDevice));
+  if (vals_io) HIPC(hipMemcpy(v.ptr, vals_io, (size_t)n * 4, hipMemcpyHostToDevice));
+  TRY(ppfi::radix_sort_u64(k.as<uint64_t>(), vals_io ? v.as<uint32_t>() : nullptr, n, bit_lo, bit_hi,
+                           ctx->sort, st));
+  HIPC(hipMemcpyAsync(keys_io, k.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+  if (vals_io) HIPC(hipMemcpyAsync(vals_io, v.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
   return PPF_OK;
 }
 

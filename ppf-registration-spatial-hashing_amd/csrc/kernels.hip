@@ -33,11 +33,29 @@ namespace {
 
 constexpr int B256 = 256;
 
+// Load through a global-address-space pointer.  Pointers read from the job
+// table are generic (flat) to the compiler; flat loads also wait on the LDS
+// counter, which serialises the vote loop.
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 gld2(const uint32_t *p) {
+  const u32x2 v = *(const __attribute__((address_space(1))) u32x2 *)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 gld4(const uint4 *p) {
+  const u32x4 v = *(const __attribute__((address_space(1))) u32x4 *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ uint32_t htab_find(const uint32_t *__restrict__ htab, uint32_t hshift,
                                               uint32_t hmask, uint32_t key) {
   uint32_t slot = (key * 2654435761u) >> hshift;
   for (;;) {
-    const uint2 e = *reinterpret_cast<const uint2 *>(htab + 2 * slot);
+    const uint2 e = gld2(htab + 2 * slot);
     if (e.x == key) return e.y;
     if (e.x == 0u) return 0xffffffffu;
     slot = (slot + 1u) & hmask;
@@ -232,13 +250,18 @@ __global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const fl
 // --------------------------------------------------------------- voting
 // Fast alpha bin: with theta = phi_v - phi_u the exact angle between the y/z
 // parts of u and v, the float pipeline of alpha_bin() yields
-// floor((theta + pi_f)/D) except within ~1e-6 rad of a bin edge (rounding of
+// floor((theta + pi_f)/D) except within ~1.2e-6 rad of a bin edge (rounding of
 // cross/dot, atan2f error <= 3.2 ulp, rounding of alpha + pi).  phi_u carries
-// <= 3e-6 rad quantization, phi_v the float atan2 error.  Whenever theta is
-// farther than MARGIN (2e-5 rad) from every nominal edge k*2pi/30 (nominal and
-// true edges differ by < 3e-7 rad) the bin is decided; otherwise, and for
-// degenerate u or v, the exact float pipeline runs.
-constexpr uint32_t FAST_MARGIN = 410166u;  // 2e-5 rad * 30/(2 pi) * 2^32
+// <= 3e-6 rad of quantization, phi_v the float atan2 error (<= 8e-7 rad), the
+// float position below <= 2e-6 bin, and nominal edges k*2pi/30 differ from the
+// true ones by < 3e-7 rad.  Together < 4e-5 bin; whenever the position is
+// farther than FAST_MARGIN_BINS (1.2e-4 bin = 2.5e-5 rad) from a nominal edge
+// the bin is decided, otherwise (and for degenerate u or v) the exact float
+// pipeline runs.  The wrap point theta = +-pi (position 0 or 30) is always
+// within the margin, so bin 30 only comes from the exact path.
+constexpr float BIN_SCALE = 30.0f / 4294967296.0f;
+constexpr float FAST_MARGIN_BINS = 1.2e-4f;
+constexpr int VOTE_UNROLL = 4;
 
 template <bool EXACT>
 __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
@@ -281,13 +304,14 @@ __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__
     const uint32_t r = (uint32_t)(item - J.item_begin);
     const int32_t s_r = (int32_t)r * J.rpdf;
     const float *Fs = J.sframe + (size_t)r * 16;
-    const float srx = J.sx[s_r], sry = J.sy[s_r], srz = J.sz[s_r];
-    const float snx = J.snx[s_r], sny = J.sny[s_r], snz = J.snz[s_r], snn = J.snn[s_r];
+    const float srx = gld(J.sx + s_r), sry = gld(J.sy + s_r), srz = gld(J.sz + s_r);
+    const float snx = gld(J.snx + s_r), sny = gld(J.sny + s_r), snz = gld(J.snz + s_r);
+    const float snn = gld(J.snn + s_r);
     float row1[4], row2[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      row1[k] = Fs[4 + k];
-      row2[k] = Fs[8 + k];
+      row1[k] = gld(Fs + 4 + k);
+      row2[k] = gld(Fs + 8 + k);
     }
     const float d_dist = J.d_dist;
     const uint32_t *__restrict__ htab = J.htab;
@@ -299,10 +323,10 @@ __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__
       bool hit = false;
       uint4 hr;
       if (s_i < n_s && s_i != s_r) {
-        const float px = J.sx[s_i], py = J.sy[s_i], pz = J.sz[s_i];
+        const float px = gld(J.sx + s_i), py = gld(J.sy + s_i), pz = gld(J.sz + s_i);
         float f[4];
-        ppfg::ppf_feature_q(srx, sry, srz, snx, sny, snz, snn, px, py, pz, J.snx[s_i], J.sny[s_i],
-                            J.snz[s_i], J.snn[s_i], d_dist, f);
+        ppfg::ppf_feature_q(srx, sry, srz, snx, sny, snz, snn, px, py, pz, gld(J.snx + s_i),
+                            gld(J.sny + s_i), gld(J.snz + s_i), gld(J.snn + s_i), d_dist, f);
         const uint32_t key = ppfm::ppf_key(f[0], f[1], f[2], f[3]);
         if (key != 0u) {
           const uint32_t u = htab_find(htab, hshift, hmask, key);
@@ -333,6 +357,7 @@ __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__
     const uint32_t nh = s_nh;
 
     // ---- phase 2: per model-point tile, accumulate then emit
+    unsigned long long wave_votes = 0;
     const int32_t T = J.T, TM = J.TM, n_m = J.n_m;
     const uint32_t *__restrict__ toff = J.toff;
     const uint32_t *__restrict__ rec = J.rec;
@@ -348,33 +373,53 @@ __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__
         if (lane == 0) h = atomicAdd(&s_next, 1u);
         h = __shfl(h, 0, WAVE);
         if (h >= nh) break;
-        const uint4 hr = hits[h];
+        const uint4 hr = gld4(hits + h);
         const uint32_t u = hr.x & 0x7fffffffu;
         const bool slow_all = EXACT || (hr.x >> 31) != 0u;
-        const uint32_t b = toff[(size_t)u * (T + 1) + t], e = toff[(size_t)u * (T + 1) + t + 1];
+        const uint32_t b = gld(toff + (size_t)u * (T + 1) + t);
+        const uint32_t e = gld(toff + (size_t)u * (T + 1) + t + 1);
+        wave_votes += e - b;
         const uint32_t v32 = hr.y;
         const float vy = ppfm::bitsf(hr.z), vz = ppfm::bitsf(hr.w);
-        for (uint32_t j = b + lane; j < e; j += WAVE) {
-          const uint32_t rc = rec[j];
-          uint32_t bin = 0, mloc = rc & REC_M_MASK;
-          bool ok = false;
-          if (!slow_all && !(rc & REC_SLOW_BIT)) {
-            const uint32_t a32 = v32 - (rc & REC_PHI_MASK);
-            const uint64_t p = (uint64_t)a32 * 30ull;
-            const uint32_t frac = (uint32_t)p;
-            bin = (uint32_t)(p >> 32);
-            ok = (frac - FAST_MARGIN) <= (0xffffffffu - 2u * FAST_MARGIN);
+        for (uint32_t j0 = b + lane; j0 < e; j0 += VOTE_UNROLL * WAVE) {
+          uint32_t rc[VOTE_UNROLL];
+#pragma unroll
+          for (int q = 0; q < VOTE_UNROLL; ++q) {
+            const uint32_t j = j0 + q * WAVE;
+            rc[q] = j < e ? gld(rec + j) : 0u;
           }
-          if (!ok) {
-            const uint32_t pr = J.pair[j];
-            const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
-            const float *Fm = J.mframe + (size_t)m_r * 16;
-            float uy, uz;
-            ppfg::frame_apply_yz(Fm + 4, Fm + 8, J.mx[m_i], J.my[m_i], J.mz[m_i], uy, uz);
-            bin = ppfg::alpha_bin(uy, uz, vy, vz);
-            mloc = m_r - (uint32_t)tile_base;
+#pragma unroll
+          for (int q = 0; q < VOTE_UNROLL; ++q) {
+            const uint32_t j = j0 + q * WAVE;
+            if (j < e) {
+              uint32_t bin = 0, mloc = rc[q] & REC_M_MASK;
+              bool ok = false;
+              if (!slow_all && !(rc[q] & REC_SLOW_BIT)) {
+                const uint32_t a32 = v32 - (rc[q] & REC_PHI_MASK);
+                const float x = (float)a32 * BIN_SCALE;
+                bin = (uint32_t)x;
+                const float fr = x - (float)bin;
+                ok = fabsf(fr - 0.5f) < 0.5f - FAST_MARGIN_BINS;
+              }
+              if (!ok) {
+                const uint32_t pr = gld(J.pair + j);
+                const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
+                const float *Fm = J.mframe + (size_t)m_r * 16;
+                float r1[4], r2[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                  r1[k] = gld(Fm + 4 + k);
+                  r2[k] = gld(Fm + 8 + k);
+                }
+                float uy, uz;
+                ppfg::frame_apply_yz(r1, r2, gld(J.mx + m_i), gld(J.my + m_i), gld(J.mz + m_i), uy,
+                                     uz);
+                bin = ppfg::alpha_bin(uy, uz, vy, vz);
+                mloc = m_r - (uint32_t)tile_base;
+              }
+              atomicAdd(&acc[__umul24(mloc, (uint32_t)N_BINS) + bin], 1u);
+            }
           }
-          atomicAdd(&acc[mloc * N_BINS + bin], 1u);
         }
       }
       __syncthreads();
@@ -413,6 +458,7 @@ __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__
       }
       __syncthreads();
     }
+    if (lane == 0 && wave_votes) atomicAdd(J.votes, wave_votes);
   }
 }
 
@@ -533,66 +579,165 @@ __global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long 
                                                          : ppfm::cell_hash(c[0] + a, c[1] + b, c[2] + k);
 }
 
-__global__ void k_cluster_votes(const float *__restrict__ trans, const float *__restrict__ quats,
-                                const float *__restrict__ w, const uint32_t *__restrict__ adj,
-                                const uint32_t *__restrict__ ukeys, const uint32_t *__restrict__ ufirst,
-                                const uint32_t *__restrict__ ucount, int64_t U,
-                                const uint32_t *__restrict__ members, int64_t P, float trans_thresh,
-                                int use_l1, int use_avg, float *__restrict__ vc_out,
-                                float *__restrict__ trans_out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P) return;
+// Work items of the tiled clustering: (unique cell hash u, 256-pose chunk).
+__global__ void k_cluster_chunks(const uint32_t *__restrict__ ucount, int64_t U,
+                                 uint32_t *__restrict__ nchunk) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  nchunk[u] = (ucount[u] + 255u) / 256u;
+}
+
+// Poses that share a voxel cell share the 26 neighbour hashes, so one block
+// takes up to 256 poses of one cell and streams every neighbour bucket through
+// LDS once; each thread walks the bucket in order, which keeps the summation
+// order of rot_clustering_kernel (kernel.cu:724-756).  Chunks whose poses do
+// not all lie in one cell (an FNV collision between cells) walk their own
+// neighbour lists per thread.
+constexpr int CL_BLOCK = 256;
+
+__device__ __forceinline__ void cluster_test(float tx, float ty, float tz, float qx, float qy,
+                                             float qz, float qw, float jx, float jy, float jz,
+                                             float jqx, float jqy, float jqz, float jqw, float wj,
+                                             float trans_thresh, int use_l1, int use_avg,
+                                             float &vc, float &ox, float &oy, float &oz) {
   const float rot_thresh_sq = ppfm::ROT_THRESH * ppfm::ROT_THRESH;
-  const float tx = trans[3 * i], ty = trans[3 * i + 1], tz = trans[3 * i + 2];
-  const float qx = quats[4 * i], qy = quats[4 * i + 1], qz = quats[4 * i + 2], qw = quats[4 * i + 3];
-  float vc = 1.0f;
-  float ox = tx, oy = ty, oz = tz;
-  for (int blk = 0; blk < 27; ++blk) {
-    const uint32_t h = adj[27 * i + blk];
-    if (h == 0u) continue;
-    int64_t lo = 0, hi = U;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (ukeys[mid] < h)
-        lo = mid + 1;
-      else
-        hi = mid;
+  const float d4 = qx * jqx + qy * jqy + qz * jqz + qw * jqw;
+  const float qd = fabsf(8.0f * (1.0f - d4));
+  if (qd < rot_thresh_sq) {
+    if (!use_l1) {
+      const float dx = tx - jx, dy = ty - jy, dz = tz - jz;
+      const float nd = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (!(nd < trans_thresh)) return;
     }
-    if (lo >= U || ukeys[lo] != h) continue;
-    const uint32_t f = ufirst[lo], n = ucount[lo];
-    for (uint32_t jj = 0; jj < n; ++jj) {
-      const uint32_t j = members[f + jj];
-      const float wj = w[j];
-      const float d4 = qx * quats[4 * j] + qy * quats[4 * j + 1] + qz * quats[4 * j + 2] +
-                       qw * quats[4 * j + 3];
-      const float qd = fabsf(8.0f * (1.0f - d4));
-      if (qd < rot_thresh_sq) {
-        const float jx = trans[3 * j], jy = trans[3 * j + 1], jz = trans[3 * j + 2];
-        if (!use_l1) {
-          const float dx = tx - jx, dy = ty - jy, dz = tz - jz;
-          const float nd = sqrtf(dx * dx + dy * dy + dz * dz);
-          if (!(nd < trans_thresh)) continue;
+    if (use_avg) {
+      ox = vc * ox;
+      oy = vc * oy;
+      oz = vc * oz;
+      ox = ox + wj * jx;
+      oy = oy + wj * jy;
+      oz = oz + wj * jz;
+      const float s = 1.0f / (vc + wj);
+      ox = s * ox;
+      oy = s * oy;
+      oz = s * oz;
+    }
+    vc += wj;
+  }
+}
+
+__device__ __forceinline__ int64_t cell_lower_bound(const uint32_t *__restrict__ ukeys, int64_t U,
+                                                    uint32_t h) {
+  int64_t lo = 0, hi = U;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ukeys[mid] < h)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < U && ukeys[lo] == h) ? lo : -1;
+}
+
+__global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
+    const float *__restrict__ trans, const float *__restrict__ quats, const float *__restrict__ w,
+    const uint32_t *__restrict__ adj, const uint32_t *__restrict__ ukeys,
+    const uint32_t *__restrict__ ufirst, const uint32_t *__restrict__ ucount, int64_t U,
+    const uint32_t *__restrict__ chunk_start, const uint32_t *__restrict__ members, float d_dist,
+    int use_l1, int use_avg, float *__restrict__ vc_out, float *__restrict__ trans_out) {
+  __shared__ float s_q[CL_BLOCK][4];
+  __shared__ float s_t[CL_BLOCK][4];
+  __shared__ int s_u, s_c, s_uniform;
+  __shared__ uint32_t s_adj[27];
+  __shared__ int64_t s_bucket;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    // which (cell, chunk) this block owns
+    const uint32_t b = blockIdx.x;
+    int64_t lo = 0, hi = U - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (chunk_start[mid] <= b)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    s_u = (int)lo;
+    s_c = (int)(b - chunk_start[lo]);
+    s_uniform = 1;
+  }
+  __syncthreads();
+  const int u = s_u;
+  const uint32_t f = ufirst[u], n = ucount[u];
+  const uint32_t k = (uint32_t)s_c * CL_BLOCK + tid;
+  const bool have = k < n;
+  const uint32_t i = have ? members[f + k] : members[f];
+  const uint32_t i0 = members[f + (uint32_t)s_c * CL_BLOCK];
+  // neighbour lists equal the chunk's first pose's list?
+  if (have) {
+    for (int q = 0; q < 27; ++q)
+      if (adj[27 * (size_t)i + q] != adj[27 * (size_t)i0 + q]) s_uniform = 0;
+  }
+  if (tid < 27) s_adj[tid] = adj[27 * (size_t)i0 + tid];
+  __syncthreads();
+  const float tx = trans[3 * (size_t)i], ty = trans[3 * (size_t)i + 1], tz = trans[3 * (size_t)i + 2];
+  const float qx = quats[4 * (size_t)i], qy = quats[4 * (size_t)i + 1], qz = quats[4 * (size_t)i + 2],
+              qw = quats[4 * (size_t)i + 3];
+  float vc = 1.0f, ox = tx, oy = ty, oz = tz;
+  if (s_uniform) {
+    for (int blk = 0; blk < 27; ++blk) {
+      const uint32_t h = s_adj[blk];
+      if (h == 0u) continue;
+      if (tid == 0) s_bucket = cell_lower_bound(ukeys, U, h);
+      __syncthreads();
+      const int64_t bu = s_bucket;
+      __syncthreads();
+      if (bu < 0) continue;
+      const uint32_t bf = ufirst[bu], bn = ucount[bu];
+      for (uint32_t c0 = 0; c0 < bn; c0 += CL_BLOCK) {
+        const uint32_t cn = min((uint32_t)CL_BLOCK, bn - c0);
+        if ((uint32_t)tid < cn) {
+          const uint32_t j = members[bf + c0 + tid];
+          s_q[tid][0] = quats[4 * (size_t)j];
+          s_q[tid][1] = quats[4 * (size_t)j + 1];
+          s_q[tid][2] = quats[4 * (size_t)j + 2];
+          s_q[tid][3] = quats[4 * (size_t)j + 3];
+          s_t[tid][0] = trans[3 * (size_t)j];
+          s_t[tid][1] = trans[3 * (size_t)j + 1];
+          s_t[tid][2] = trans[3 * (size_t)j + 2];
+          s_t[tid][3] = w[j];
         }
-        if (use_avg) {
-          ox = vc * ox;
-          oy = vc * oy;
-          oz = vc * oz;
-          ox = ox + wj * jx;
-          oy = oy + wj * jy;
-          oz = oz + wj * jz;
-          const float s = 1.0f / (vc + wj);
-          ox = s * ox;
-          oy = s * oy;
-          oz = s * oz;
+        __syncthreads();
+        if (have) {
+          for (uint32_t jj = 0; jj < cn; ++jj)
+            cluster_test(tx, ty, tz, qx, qy, qz, qw, s_t[jj][0], s_t[jj][1], s_t[jj][2], s_q[jj][0],
+                         s_q[jj][1], s_q[jj][2], s_q[jj][3], s_t[jj][3], d_dist, use_l1, use_avg, vc,
+                         ox, oy, oz);
         }
-        vc += wj;
+        __syncthreads();
+      }
+    }
+  } else if (have) {
+    for (int blk = 0; blk < 27; ++blk) {
+      const uint32_t h = adj[27 * (size_t)i + blk];
+      if (h == 0u) continue;
+      const int64_t bu = cell_lower_bound(ukeys, U, h);
+      if (bu < 0) continue;
+      const uint32_t bf = ufirst[bu], bn = ucount[bu];
+      for (uint32_t jj = 0; jj < bn; ++jj) {
+        const uint32_t j = members[bf + jj];
+        cluster_test(tx, ty, tz, qx, qy, qz, qw, trans[3 * (size_t)j], trans[3 * (size_t)j + 1],
+                     trans[3 * (size_t)j + 2], quats[4 * (size_t)j], quats[4 * (size_t)j + 1],
+                     quats[4 * (size_t)j + 2], quats[4 * (size_t)j + 3], w[j], d_dist, use_l1, use_avg,
+                     vc, ox, oy, oz);
       }
     }
   }
-  vc_out[i] = vc;
-  trans_out[3 * i] = ox;
-  trans_out[3 * i + 1] = oy;
-  trans_out[3 * i + 2] = oz;
+  if (have) {
+    vc_out[i] = vc;
+    trans_out[3 * (size_t)i] = ox;
+    trans_out[3 * (size_t)i + 1] = oy;
+    trans_out[3 * (size_t)i + 2] = oz;
+  }
 }
 
 // first index of the maximum (thrust::max_element), one block
@@ -763,15 +908,17 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
     set_error("run_votes: accumulator tile exceeds the LDS limit");
     return PPF_ERR_ARG;
   }
-  const int G = ctx->n_cu;
+  auto kern = exact ? k_vote<true> : k_vote<false>;
+  PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  PPF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, VOTE_BLOCK, lds));
+  const int G = ctx->n_cu * std::max(1, per_cu);
   PPF_TRY(ctx->tmp_a.ensure(sizeof(VoteJob) * n_jobs));
   PPF_TRY(ctx->queue.ensure(16));
   PPF_TRY(ctx->hits.ensure((size_t)G * (size_t)max_ns * 16));
   PPF_HIP(hipMemcpyAsync(ctx->tmp_a.ptr, h_jobs, sizeof(VoteJob) * n_jobs, hipMemcpyHostToDevice,
                          ctx->stream));
   PPF_HIP(hipMemsetAsync(ctx->queue.ptr, 0, 16, ctx->stream));
-  auto kern = exact ? k_vote<true> : k_vote<false>;
-  PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream,
                      ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items,
@@ -875,9 +1022,20 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   int64_t U = 0;
   PPF_TRY(unique_count(ctx, words, P, &U));
   PPF_TRY(unique_emit(ctx, words, P, U, ukeys, ufirst, ucount, members));
-  hipLaunchKernelGGL(k_cluster_votes, dim3(nblk(P, 128)), dim3(128), 0, ctx->stream, trans, quats, w,
-                     adj, ukeys, ufirst, ucount, U, members, P, d_dist, use_l1 ? 1 : 0,
-                     use_avg ? 1 : 0, vc, trans_out);
+  // chunk table: chunk_start[u] = exclusive scan of ceil(count/256)
+  PPF_TRY(ctx->tmp_a.ensure((size_t)(U + 1) * 8 + 64));
+  uint32_t *nch = ctx->tmp_a.as<uint32_t>();
+  uint32_t *cstart = nch + U + 1;
+  uint32_t *ctot = cstart + U;
+  hipLaunchKernelGGL(k_cluster_chunks, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, ucount, U, nch);
+  PPF_TRY(exclusive_scan_u32(nch, cstart, U, ctx->tmp_d, ctx->stream, ctot));
+  uint32_t n_items = 0;
+  PPF_HIP(hipMemcpyAsync(&n_items, ctot, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PPF_HIP(hipStreamSynchronize(ctx->stream));
+  if (n_items > 0)
+    hipLaunchKernelGGL(k_cluster_tiled, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, w,
+                       adj, ukeys, ufirst, ucount, U, cstart, members, d_dist, use_l1 ? 1 : 0,
+                       use_avg ? 1 : 0, vc, trans_out);
   hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
   PPF_HIP(hipGetLastError());
   long long mi = -1;

@@ -15,7 +15,7 @@ constexpr uint32_t REC_M_MASK = 0x7ffu;
 constexpr uint32_t REC_SLOW_BIT = 0x800u;
 constexpr uint32_t REC_PHI_MASK = 0xfffff000u;
 constexpr int MAX_TILE_POINTS = 2048;
-constexpr int DEFAULT_TILE_POINTS = 1024;
+constexpr int DEFAULT_TILE_POINTS = 512;
 constexpr int N_BINS = 31;
 constexpr int VOTE_BLOCK = 1024;
 
@@ -118,6 +118,7 @@ struct VoteJob {
   uint32_t *cand_count;
   uint32_t *cand_n;
   uint32_t cand_cap, pad2;
+  unsigned long long *votes;  // total votes of the job (statistics)
   uint64_t item_begin;
 };
 

@@ -211,24 +211,38 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_down(const uint32_t *__restri
 
 }  // namespace
 
+// Scan of n <= SC_CHUNK^2 elements using `sums` (>= nb entries) as scratch.
+static int scan_level(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *sums, hipStream_t st,
+                      uint32_t *total_out_dev) {
+  const int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BLOCK), 0, st, in, n, sums);
+  const uint32_t *scanned = sums;
+  if (nb <= SC_CHUNK) {
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SC_BLOCK), 0, st, sums, nb, total_out_dev);
+  } else {
+    // one more level: scan the chunk sums into sums + nb
+    PPF_TRY(scan_level(sums, sums + nb, nb, sums + 2 * nb, st, total_out_dev));
+    scanned = sums + nb;
+  }
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SC_BLOCK), 0, st, in, n, scanned, out);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp, hipStream_t st,
                        uint32_t *total_out_dev) {
   if (n <= 0) {
     if (total_out_dev) PPF_HIP(hipMemsetAsync(total_out_dev, 0, 4, st));
     return 0;
   }
-  const int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
-  if (nb > SC_CHUNK) {
+  if (n > ((int64_t)1 << 32) - 1) {
     set_error("exclusive_scan_u32: input too long");
     return 1;
   }
-  PPF_TRY(tmp.ensure((size_t)nb * 4));
-  uint32_t *sums = tmp.as<uint32_t>();
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BLOCK), 0, st, in, n, sums);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SC_BLOCK), 0, st, sums, nb, total_out_dev);
-  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SC_BLOCK), 0, st, in, n, sums, out);
-  PPF_HIP(hipGetLastError());
-  return 0;
+  // scratch for every level: 2*nb + 2*nb/SC_CHUNK + ... (< 3*nb + 8)
+  const int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
+  PPF_TRY(tmp.ensure((size_t)(3 * nb + 16) * 4));
+  return scan_level(in, out, n, tmp.as<uint32_t>(), st, total_out_dev);
 }
 
 int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,

@@ -13,8 +13,10 @@ present, the calls raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -51,7 +53,8 @@ class LookupParams(C.Structure):
 
 
 class ResultInfo(C.Structure):
-    _fields_ = [("status", C.c_int32), ("lookups", C.c_int64), ("max_count", C.c_int64),
+    _fields_ = [("status", C.c_int32), ("lookups", C.c_int64), ("votes", C.c_int64),
+                ("max_count", C.c_int64),
                 ("n_candidates", C.c_int64), ("n_kept", C.c_int64), ("max_idx", C.c_int64),
                 ("n_clusters", C.c_int64), ("vote_ms", C.c_double), ("total_ms", C.c_double)]
 
@@ -95,6 +98,13 @@ def lib():
                                    C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.c_int32, _vp,
                                    _f32p, _vp]
     L.ppf_debug_numerics.argtypes = [_vp, C.c_int32, _f32p, _vp, _f32p, C.c_int64]
+    L.ppf_debug_primitive.argtypes = [_vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32]
+    sizes = (C.c_uint32 * 3)()
+    L.ppf_abi_sizes(sizes)
+    want = (C.sizeof(ModelInfo), C.sizeof(LookupParams), C.sizeof(ResultInfo))
+    if tuple(sizes) != want:
+        raise PPFError(PPF_ERR_STATE, f"{LIB_PATH} was built from a different include/ppf_amd.h "
+                                      f"(struct sizes {tuple(sizes)} != {want}): rebuild it")
     _lib = L
     return L
 
@@ -116,16 +126,52 @@ def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data
 
 
+_live_contexts: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+class _Owned:
+    """Base of objects created from a Context: destroyed before their context."""
+
+    handle = None
+    _destroy = None
+
+    def _register(self, ctx: "Context"):
+        self.ctx = ctx
+        ctx._children.add(self)
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h:
+            self.handle = None
+            if self.ctx.handle:
+                getattr(lib(), self._destroy)(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
-    """One device + one HIP stream (the reference's device selection, ppf.cu:37-49)."""
+    """One device + one HIP stream (the reference's device selection, ppf.cu:37-49).
+
+    Closing a context first closes every scene, model and result made from it."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
         h = _vp()
         _check(lib().ppf_ctx_create(device, hbm_budget_bytes, C.byref(h)))
         self.handle = h
+        self._children: "weakref.WeakSet[_Owned]" = weakref.WeakSet()
+        _live_contexts.add(self)
 
     def close(self):
         if getattr(self, "handle", None):
+            # results before models/scenes (they are independent on the device,
+            # but keep a deterministic order)
+            kids = sorted(list(self._children), key=lambda o: 0 if isinstance(o, LookupResult) else 1)
+            for k in kids:
+                k.close()
             lib().ppf_ctx_destroy(self.handle)
             self.handle = None
 
@@ -147,6 +193,19 @@ class Context:
         return out
 
 
+    def debug_scan(self, a: np.ndarray):
+        x = np.ascontiguousarray(a, dtype=np.uint32).copy()
+        tot = np.zeros(1, np.uint32)
+        _check(lib().ppf_debug_primitive(self.handle, 0, x.ctypes.data, tot.ctypes.data, len(x), 0, 0))
+        return x, int(tot[0])
+
+    def debug_sort(self, keys: np.ndarray, vals: Optional[np.ndarray], bit_lo: int, bit_hi: int):
+        k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+        v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
+        _check(lib().ppf_debug_primitive(self.handle, 1, k.ctypes.data, _ptr(v), len(k), bit_lo, bit_hi))
+        return k, v
+
+
 _default_ctx: Optional[Context] = None
 
 
@@ -157,7 +216,7 @@ def default_context() -> Context:
     return _default_ctx
 
 
-class Scene:
+class Scene(_Owned):
     """Scene(cloud, d_dist, ref_point_downsample_factor=1) -- include/scene.h:15-16.
 
     d_dist is accepted for signature parity; scene features are quantized with
@@ -166,14 +225,16 @@ class Scene:
 
     def __init__(self, cloud, d_dist: float = 0.0, ref_point_downsample_factor: int = 1,
                  ctx: Optional[Context] = None):
-        self.ctx = ctx or default_context()
+        ctx = ctx or default_context()
         p, n = _cloud(*cloud)
         self.n = len(p)
         self.d_dist = d_dist
         self.rpdf = int(ref_point_downsample_factor)
         h = _vp()
-        _check(lib().ppf_scene_upload(self.ctx.handle, p, n, self.n, self.rpdf, C.byref(h)))
+        _check(lib().ppf_scene_upload(ctx.handle, p, n, self.n, self.rpdf, C.byref(h)))
         self.handle = h
+        self._destroy = "ppf_scene_destroy"
+        self._register(ctx)
 
     def numPoints(self) -> int:
         return self.n
@@ -184,21 +245,15 @@ class Scene:
         _check(lib().ppf_scene_keys(self.handle, float(d_dist if d_dist else self.d_dist), out))
         return out
 
-    def close(self):
-        if getattr(self, "handle", None):
-            lib().ppf_scene_destroy(self.handle)
-            self.handle = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
-class LookupResult:
-    def __init__(self, handle, cpu: bool):
+class LookupResult(_Owned):
+    """Outputs of one Model::ppf_lookup; owns its device arrays."""
+
+    def __init__(self, handle, cpu: bool, ctx: "Context"):
         self.handle = handle
+        self._destroy = "ppf_result_destroy"
+        self._register(ctx)
         self.cpu = cpu
         info = ResultInfo()
         _check(lib().ppf_result_info(handle, C.byref(info)))
@@ -236,26 +291,16 @@ class LookupResult:
             _check(rc)
         return out.reshape(4, 4)
 
-    def close(self):
-        if getattr(self, "handle", None):
-            lib().ppf_result_destroy(self.handle)
-            self.handle = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
-class Model:
+class Model(_Owned):
     """Model(cloud, d_dist, vote_count_threshold, cpu_clustering, use_l1_norm,
     use_averaged_clusters) -- include/model.h:17-19, src/cuda/model.cu:43-82."""
 
     def __init__(self, cloud, d_dist: float, vote_count_threshold: float = 0.4,
                  cpu_clustering: bool = False, use_l1_norm: bool = False,
                  use_averaged_clusters: bool = False, ctx: Optional[Context] = None):
-        self.ctx = ctx or default_context()
+        ctx = ctx or default_context()
         p, n = _cloud(*cloud)
         self.n = len(p)
         self.d_dist = float(d_dist)
@@ -264,8 +309,10 @@ class Model:
         self.use_l1_norm = bool(use_l1_norm)
         self.use_averaged_clusters = bool(use_averaged_clusters)
         h = _vp()
-        _check(lib().ppf_model_build(self.ctx.handle, p, n, self.n, self.d_dist, C.byref(h)))
+        _check(lib().ppf_model_build(ctx.handle, p, n, self.n, self.d_dist, C.byref(h)))
         self.handle = h
+        self._destroy = "ppf_model_destroy"
+        self._register(ctx)
         self.result: Optional[LookupResult] = None
 
     def info(self) -> dict:
@@ -295,7 +342,7 @@ class Model:
                            int(exact_votes))
         h = _vp()
         _check(lib().ppf_lookup(self.handle, scene.handle, C.byref(prm), C.byref(h)))
-        self.result = LookupResult(h, self.cpu_clustering)
+        self.result = LookupResult(h, self.cpu_clustering, self.ctx)
         return self.result
 
     # reference-style accessors of the last lookup
@@ -306,18 +353,18 @@ class Model:
     def getTransformations(self) -> np.ndarray:
         return self.result.transformations()
 
-    def close(self):
-        if self.result is not None:
-            self.result.close()
-        if getattr(self, "handle", None):
-            lib().ppf_model_destroy(self.handle)
-            self.handle = None
 
-    def __del__(self):
+
+def _close_all_contexts():
+    for c in list(_live_contexts):
         try:
-            self.close()
+            c.close()
         except Exception:
             pass
+
+
+# tear down before the HIP runtime's own exit-time destructors run
+atexit.register(_close_all_contexts)
 
 
 def ppf_registration(scene_clouds: Sequence, model_clouds: Sequence, model_d_dists: Sequence[float],

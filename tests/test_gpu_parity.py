@@ -226,3 +226,32 @@ def test_budget_rejects_allocation():
         ppf_amd.Model(m, 0.05, ctx=c)
     assert e.value.code == ppf_amd.PPF_ERR_NOMEM
     c.close()
+
+
+@pytest.mark.parametrize("n", [1, 1000, 4096 * 4096 + 12345])
+def test_device_exclusive_scan(ctx, n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 4, n).astype(np.uint32)
+    out, tot = ctx.debug_scan(a)
+    want = np.concatenate([[0], np.cumsum(a, dtype=np.uint64)[:-1]]).astype(np.uint32)
+    assert np.array_equal(out, want) and tot == int(a.sum())
+
+
+@pytest.mark.parametrize("case", ["random", "equal", "few", "descending"])
+def test_device_radix_sort_is_stable(ctx, case):
+    rng = np.random.default_rng(1)
+    n = 300001
+    if case == "random":
+        k = rng.integers(0, 2**63, n, dtype=np.uint64)
+    elif case == "equal":
+        k = np.full(n, 7, np.uint64)
+    elif case == "few":
+        k = rng.integers(0, 5, n).astype(np.uint64) << np.uint64(32)
+    else:
+        k = np.arange(n, dtype=np.uint64)[::-1].copy() << np.uint64(20)
+    v = np.arange(n, dtype=np.uint32)
+    for lo, hi in ((0, 64), (32, 64), (20, 44)):
+        sk, sv = ctx.debug_sort(k, v, lo, hi)
+        digit = (k >> np.uint64(lo)) & np.uint64((1 << (hi - lo)) - 1) if hi - lo < 64 else k
+        order = np.argsort(digit, kind="stable")
+        assert np.array_equal(sv, v[order]) and np.array_equal(sk, k[order])
