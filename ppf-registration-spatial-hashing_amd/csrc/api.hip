@@ -337,6 +337,10 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   TRY(m->hcount.alloc(ctx, (size_t)U * 4));
   TRY(ppfk::unique_emit(ctx, words.as<uint64_t>(), N, U, m->hkeys.as<uint32_t>(),
                         m->hfirst.as<uint32_t>(), m->hcount.as<uint32_t>(), m->pair.as<uint32_t>()));
+  // vote order: (key, tile, phi_u), pair indices in vpair
+  TRY(m->vpair.alloc(ctx, (size_t)N * 4));
+  TRY(ppfk::vote_order(ctx, words.as<uint64_t>(), m->vpair.as<uint32_t>(), N, m->frames.as<float>(),
+                       m->cloud, m->TM));
   HIPC(hipStreamSynchronize(st));
   words.release();
   // key -> unique index table
@@ -349,9 +353,9 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   // per-tile sub-ranges and packed vote records
   TRY(m->toff.alloc(ctx, (size_t)U * (m->T + 1) * 4));
   TRY(ppfk::build_tile_offsets(ctx, m->hfirst.as<uint32_t>(), m->hcount.as<uint32_t>(),
-                               m->pair.as<uint32_t>(), U, (int32_t)n, m->TM, m->T, m->toff.as<uint32_t>()));
+                               m->vpair.as<uint32_t>(), U, (int32_t)n, m->TM, m->T, m->toff.as<uint32_t>()));
   TRY(m->rec.alloc(ctx, (size_t)N * 4));
-  TRY(ppfk::build_records(ctx, m->pair.as<uint32_t>(), N, m->frames.as<float>(), m->cloud, m->TM,
+  TRY(ppfk::build_records(ctx, m->vpair.as<uint32_t>(), N, m->frames.as<float>(), m->cloud, m->TM,
                           m->rec.as<uint32_t>()));
   TRY(m->weights.alloc(ctx, (size_t)n * 4));
   std::vector<float> ones((size_t)n, 1.0f);
@@ -378,7 +382,7 @@ int ppf_model_info(ppf_model m, ppf_model_info_t *info) {
   info->tile_points = m->TM;
   info->n_tiles = m->T;
   info->device_bytes = m->cloud.soa.bytes + m->frames.bytes + m->hkeys.bytes + m->hcount.bytes +
-                       m->hfirst.bytes + m->pair.bytes + m->rec.bytes + m->toff.bytes +
+                       m->hfirst.bytes + m->pair.bytes + m->vpair.bytes + m->rec.bytes + m->toff.bytes +
                        m->htab.bytes + m->weights.bytes;
   return PPF_OK;
 }
@@ -497,7 +501,7 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
     J.hmask = m->hcap - 1;
     J.toff = m->toff.as<uint32_t>();
     J.rec = m->rec.as<uint32_t>();
-    J.pair = m->pair.as<uint32_t>();
+    J.pair = m->vpair.as<uint32_t>();
     J.mframe = m->frames.as<float>();
     J.mx = m->cloud.x();
     J.my = m->cloud.y();

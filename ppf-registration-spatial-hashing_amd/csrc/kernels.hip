@@ -247,6 +247,28 @@ __global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const fl
   rec[j] = (q << 12) | slow | (m_r % (uint32_t)TM);
 }
 
+// Sort words of the vote-record order: (key, tile, phi_u).  Within a bucket
+// and tile, entries ordered by angle instead of by pair index make the lanes
+// of one wave hit different model points, so their LDS accumulator updates do
+// not collide on one address.
+__global__ void k_vote_order_words(uint64_t *__restrict__ words, uint32_t *__restrict__ vals,
+                                   int64_t N, const float *__restrict__ mframe,
+                                   const float *__restrict__ soa, int32_t n_m, int32_t TM) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint64_t w = words[j];
+  const uint32_t p = (uint32_t)w;
+  const uint32_t m_r = p / (uint32_t)n_m, m_i = p - m_r * (uint32_t)n_m;
+  const float *F = mframe + (int64_t)m_r * 16;
+  float uy, uz;
+  ppfg::frame_apply_yz(F + 4, F + 8, soa[m_i], soa[n_m + m_i], soa[2 * n_m + m_i], uy, uz);
+  const double phi = atan2((double)uz, (double)uy);
+  const uint32_t q = (uint32_t)(int32_t)llrint(phi * (1048576.0 / 6.283185307179586)) & 0xfffffu;
+  const uint32_t tile = m_r / (uint32_t)TM;
+  words[j] = (w & 0xffffffff00000000ull) | ((uint64_t)tile << 20) | (uint64_t)q;
+  vals[j] = p;
+}
+
 // --------------------------------------------------------------- voting
 // Fast alpha bin: with theta = phi_v - phi_u the exact angle between the y/z
 // parts of u and v, the float pipeline of alpha_bin() yields
@@ -262,22 +284,29 @@ __global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const fl
 constexpr float BIN_SCALE = 30.0f / 4294967296.0f;
 constexpr float FAST_MARGIN_BINS = 1.2e-4f;
 constexpr int VOTE_UNROLL = 4;
+constexpr int GT_LOG2 = 12;
+constexpr int GT_SIZE = 1 << GT_LOG2;  // hit-grouping table slots (2 x 16 KB of LDS)
 
 template <bool EXACT>
-__global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
+__global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
                                                      uint64_t n_items,
                                                      unsigned long long *__restrict__ queue,
                                                      uint4 *__restrict__ hit_scratch,
-                                                     uint32_t hit_cap) {
+                                                     uint32_t hit_cap, uint32_t stride_u4) {
   extern __shared__ uint32_t acc[];
   __shared__ unsigned long long s_item;
   __shared__ int s_job;
-  __shared__ uint32_t s_nh, s_next;
+  __shared__ uint32_t s_nh, s_next, s_ovf, s_ngroups;
   __shared__ uint32_t s_wmax[VOTE_BLOCK / WAVE];
+  __shared__ uint32_t s_wsum[VOTE_BLOCK / WAVE][2];
   __shared__ float s_thr;
   const int tid = threadIdx.x, wv = tid / WAVE, lane = tid % WAVE;
   const uint64_t lt = lanemask_lt();
-  uint4 *hits = hit_scratch + (size_t)blockIdx.x * hit_cap;
+  // per-block scratch: hits, hits in bucket order, slot of each hit, groups
+  uint4 *hits = hit_scratch + (size_t)blockIdx.x * stride_u4;
+  uint4 *sorted = hits + hit_cap;
+  uint32_t *slotid = reinterpret_cast<uint32_t *>(sorted + hit_cap);
+  uint2 *groups = reinterpret_cast<uint2 *>(slotid + hit_cap + (hit_cap & 1u));
 
   for (;;) {
     if (tid == 0) {
@@ -356,6 +385,86 @@ __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__
     __syncthreads();
     const uint32_t nh = s_nh;
 
+    // ---- phase 1b: group hits by model bucket.  Partners of one reference
+    // point hit few distinct buckets (tens of hits per bucket is common), so
+    // phase 2 reads each bucket's records once per group.  LDS hash table over
+    // u (in the accumulator space, not yet in use), then a counting sort.
+    uint32_t *tkey = acc;
+    uint32_t *tcnt = acc + GT_SIZE;
+    for (int i = tid; i < GT_SIZE; i += VOTE_BLOCK) {
+      tkey[i] = 0u;
+      tcnt[i] = 0u;
+    }
+    if (tid == 0) s_ovf = 0u;
+    __syncthreads();
+    for (uint32_t h = tid; h < nh; h += VOTE_BLOCK) {
+      const uint32_t u = gld(reinterpret_cast<const uint32_t *>(hits + h)) & 0x7fffffffu;
+      const uint32_t key = u + 1u;
+      uint32_t slot = (u * 2654435761u) >> (32 - GT_LOG2);
+      for (uint32_t probes = 0;; ++probes) {
+        if (probes == (uint32_t)GT_SIZE) {
+          slot = 0xffffffffu;
+          break;
+        }
+        const uint32_t prev = atomicCAS(&tkey[slot], 0u, key);
+        if (prev == 0u || prev == key) break;
+        slot = (slot + 1u) & (GT_SIZE - 1);
+      }
+      if (slot == 0xffffffffu)
+        s_ovf = 1u;
+      else
+        atomicAdd(&tcnt[slot], 1u);
+      slotid[h] = slot;
+    }
+    __syncthreads();
+    const bool identity = s_ovf != 0u;  // table full: one group per hit
+    if (!identity) {
+      // exclusive scans of the slot counts (group starts) and of the used
+      // slots (group ids); each thread owns GT_SIZE / VOTE_BLOCK slots
+      constexpr int PER = GT_SIZE / VOTE_BLOCK;
+      uint32_t c[PER], cs = 0, us = 0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        c[q] = tcnt[tid * PER + q];
+        cs += c[q];
+        us += c[q] ? 1u : 0u;
+      }
+      const uint32_t ci = wave_incl_scan_u32(cs), ui = wave_incl_scan_u32(us);
+      if (lane == WAVE - 1) {
+        s_wsum[wv][0] = ci;
+        s_wsum[wv][1] = ui;
+      }
+      __syncthreads();
+      uint32_t cb = ci - cs, ub = ui - us, utot = 0;
+      for (int k = 0; k < VOTE_BLOCK / WAVE; ++k) {
+        if (k < wv) {
+          cb += s_wsum[k][0];
+          ub += s_wsum[k][1];
+        }
+        utot += s_wsum[k][1];
+      }
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (c[q]) {
+          groups[ub] = make_uint2(cb, c[q]);
+          ++ub;
+        }
+        tcnt[tid * PER + q] = cb;  // cursor
+        cb += c[q];
+      }
+      if (tid == 0) s_ngroups = utot;
+      __syncthreads();
+      for (uint32_t h = tid; h < nh; h += VOTE_BLOCK) {
+        const uint32_t pos = atomicAdd(&tcnt[slotid[h]], 1u);
+        sorted[pos] = gld4(hits + h);
+      }
+    } else if (tid == 0) {
+      s_ngroups = nh;
+    }
+    __syncthreads();
+    const uint32_t ngroups = s_ngroups;
+    const uint4 *ghits = identity ? hits : sorted;
+
     // ---- phase 2: per model-point tile, accumulate then emit
     unsigned long long wave_votes = 0;
     const int32_t T = J.T, TM = J.TM, n_m = J.n_m;
@@ -369,55 +478,70 @@ __global__ __launch_bounds__(VOTE_BLOCK) void k_vote(const VoteJob *__restrict__
       if (tid == 0) s_next = 0;
       __syncthreads();
       for (;;) {
-        uint32_t h = 0;
-        if (lane == 0) h = atomicAdd(&s_next, 1u);
-        h = __shfl(h, 0, WAVE);
-        if (h >= nh) break;
-        const uint4 hr = gld4(hits + h);
-        const uint32_t u = hr.x & 0x7fffffffu;
-        const bool slow_all = EXACT || (hr.x >> 31) != 0u;
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(&s_next, 1u);
+        g = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(g, 0, WAVE));
+        if (g >= ngroups) break;
+        uint32_t gs = g, gc = 1;
+        if (!identity) {
+          const uint2 gr = gld2(reinterpret_cast<const uint32_t *>(groups + g));
+          gs = gr.x;
+          gc = gr.y;
+        }
+        const uint32_t u = gld(reinterpret_cast<const uint32_t *>(ghits + gs)) & 0x7fffffffu;
         const uint32_t b = gld(toff + (size_t)u * (T + 1) + t);
         const uint32_t e = gld(toff + (size_t)u * (T + 1) + t + 1);
-        wave_votes += e - b;
-        const uint32_t v32 = hr.y;
-        const float vy = ppfm::bitsf(hr.z), vz = ppfm::bitsf(hr.w);
-        for (uint32_t j0 = b + lane; j0 < e; j0 += VOTE_UNROLL * WAVE) {
-          uint32_t rc[VOTE_UNROLL];
+        if (b >= e) continue;
+        wave_votes += (unsigned long long)(e - b) * gc;
+        for (uint32_t s0 = 0; s0 < gc; s0 += WAVE) {
+          const uint32_t sc = min((uint32_t)WAVE, gc - s0);
+          uint4 my = make_uint4(0u, 0u, 0u, 0u);
+          if ((uint32_t)lane < sc) my = gld4(ghits + gs + s0 + lane);
+          for (uint32_t j0 = b + lane; j0 < e; j0 += VOTE_UNROLL * WAVE) {
+            uint32_t rc[VOTE_UNROLL];
 #pragma unroll
-          for (int q = 0; q < VOTE_UNROLL; ++q) {
-            const uint32_t j = j0 + q * WAVE;
-            rc[q] = j < e ? gld(rec + j) : 0u;
-          }
+            for (int q = 0; q < VOTE_UNROLL; ++q) {
+              const uint32_t j = j0 + q * WAVE;
+              rc[q] = j < e ? gld(rec + j) : 0u;
+            }
+            for (uint32_t k = 0; k < sc; ++k) {
+              const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)my.x, (int)k);
+              const uint32_t v32 = (uint32_t)__builtin_amdgcn_readlane((int)my.y, (int)k);
+              const bool slow_all = EXACT || (hx >> 31) != 0u;
 #pragma unroll
-          for (int q = 0; q < VOTE_UNROLL; ++q) {
-            const uint32_t j = j0 + q * WAVE;
-            if (j < e) {
-              uint32_t bin = 0, mloc = rc[q] & REC_M_MASK;
-              bool ok = false;
-              if (!slow_all && !(rc[q] & REC_SLOW_BIT)) {
-                const uint32_t a32 = v32 - (rc[q] & REC_PHI_MASK);
-                const float x = (float)a32 * BIN_SCALE;
-                bin = (uint32_t)x;
-                const float fr = x - (float)bin;
-                ok = fabsf(fr - 0.5f) < 0.5f - FAST_MARGIN_BINS;
-              }
-              if (!ok) {
-                const uint32_t pr = gld(J.pair + j);
-                const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
-                const float *Fm = J.mframe + (size_t)m_r * 16;
-                float r1[4], r2[4];
+              for (int q = 0; q < VOTE_UNROLL; ++q) {
+                const uint32_t j = j0 + q * WAVE;
+                if (j < e) {
+                  uint32_t bin = 0, mloc = rc[q] & REC_M_MASK;
+                  bool ok = false;
+                  if (!slow_all && !(rc[q] & REC_SLOW_BIT)) {
+                    const uint32_t a32 = v32 - (rc[q] & REC_PHI_MASK);
+                    const float x = (float)a32 * BIN_SCALE;
+                    bin = (uint32_t)x;
+                    const float fr = x - (float)bin;
+                    ok = fabsf(fr - 0.5f) < 0.5f - FAST_MARGIN_BINS;
+                  }
+                  if (!ok) {
+                    const float vy = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.z, (int)k));
+                    const float vz = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.w, (int)k));
+                    const uint32_t pr = gld(J.pair + j);
+                    const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
+                    const float *Fm = J.mframe + (size_t)m_r * 16;
+                    float r1[4], r2[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                  r1[k] = gld(Fm + 4 + k);
-                  r2[k] = gld(Fm + 8 + k);
+                    for (int kk = 0; kk < 4; ++kk) {
+                      r1[kk] = gld(Fm + 4 + kk);
+                      r2[kk] = gld(Fm + 8 + kk);
+                    }
+                    float uy, uz;
+                    ppfg::frame_apply_yz(r1, r2, gld(J.mx + m_i), gld(J.my + m_i), gld(J.mz + m_i),
+                                         uy, uz);
+                    bin = ppfg::alpha_bin(uy, uz, vy, vz);
+                    mloc = m_r - (uint32_t)tile_base;
+                  }
+                  atomicAdd(&acc[__umul24(mloc, (uint32_t)N_BINS) + bin], 1u);
                 }
-                float uy, uz;
-                ppfg::frame_apply_yz(r1, r2, gld(J.mx + m_i), gld(J.my + m_i), gld(J.mz + m_i), uy,
-                                     uz);
-                bin = ppfg::alpha_bin(uy, uz, vy, vz);
-                mloc = m_r - (uint32_t)tile_base;
               }
-              atomicAdd(&acc[__umul24(mloc, (uint32_t)N_BINS) + bin], 1u);
             }
           }
         }
@@ -890,6 +1014,15 @@ int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcou
   return 0;
 }
 
+int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const float *mframe,
+               const CloudDev &m, int32_t TM) {
+  if (N <= 0) return 0;
+  hipLaunchKernelGGL(k_vote_order_words, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, vals, N,
+                     mframe, m.soa.as<float>(), (int32_t)m.n, TM);
+  PPF_HIP(hipGetLastError());
+  return radix_sort_u64(words, vals, N, 0, 64, ctx->sort, ctx->stream);
+}
+
 int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
                   const CloudDev &m, int32_t TM, uint32_t *rec) {
   if (N <= 0) return 0;
@@ -903,7 +1036,7 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
               int64_t max_ns, bool exact, float *vote_ms) {
   *vote_ms = 0.0f;
   if (n_items == 0 || n_jobs <= 0) return 0;
-  const size_t lds = (size_t)max_tm * N_BINS * 4;
+  const size_t lds = std::max((size_t)max_tm * N_BINS * 4, (size_t)2 * GT_SIZE * 4);
   if (lds > ctx->max_lds) {
     set_error("run_votes: accumulator tile exceeds the LDS limit");
     return PPF_ERR_ARG;
@@ -915,14 +1048,18 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
   const int G = ctx->n_cu * std::max(1, per_cu);
   PPF_TRY(ctx->tmp_a.ensure(sizeof(VoteJob) * n_jobs));
   PPF_TRY(ctx->queue.ensure(16));
-  PPF_TRY(ctx->hits.ensure((size_t)G * (size_t)max_ns * 16));
+  const uint32_t hit_cap = (uint32_t)max_ns;
+  // hits + sorted hits (16 B each), slot ids (4 B), groups (8 B x GT_SIZE)
+  const uint32_t stride_u4 =
+      2 * hit_cap + (uint32_t)(((size_t)(hit_cap + (hit_cap & 1u)) * 4 + (size_t)GT_SIZE * 8 + 15) / 16);
+  PPF_TRY(ctx->hits.ensure((size_t)G * stride_u4 * 16));
   PPF_HIP(hipMemcpyAsync(ctx->tmp_a.ptr, h_jobs, sizeof(VoteJob) * n_jobs, hipMemcpyHostToDevice,
                          ctx->stream));
   PPF_HIP(hipMemsetAsync(ctx->queue.ptr, 0, 16, ctx->stream));
   PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream,
                      ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items,
-                     ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), (uint32_t)max_ns);
+                     ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4);
   PPF_HIP(hipGetLastError());
   PPF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   PPF_HIP(hipEventSynchronize(ctx->ev1));

@@ -72,8 +72,9 @@ struct ppf_model_s {
   TBuf hkeys;    // [U] unique keys (ascending)
   TBuf hcount;   // [U]
   TBuf hfirst;   // [U]
-  TBuf pair;     // [n*n] pair index m_r*n + m_i, sorted by (key, pair)
-  TBuf rec;      // [n*n] packed vote records, same order
+  TBuf pair;     // [n*n] pair index m_r*n + m_i, sorted by (key, pair): hashkeyToDataMap
+  TBuf vpair;    // [n*n] pair index in vote order (key, tile, phi_u)
+  TBuf rec;      // [n*n] packed vote records, vote order
   TBuf toff;     // [U][T+1] absolute start of each tile's sub-range
   TBuf htab;     // [hcap][2] (key, u) open addressing, key 0 = empty
   TBuf weights;  // [n]
@@ -138,6 +139,10 @@ int build_htable(ppf_ctx ctx, const uint32_t *hkeys, int64_t U, uint32_t *htab, 
 int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcount,
                        const uint32_t *pair, int64_t U, int32_t n_m, int32_t TM, int32_t T,
                        uint32_t *toff);
+// words (key << 32 | pair, sorted by key) -> vote order (key, tile, phi_u);
+// vals receives the pair index of each entry in that order.
+int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const float *mframe,
+               const CloudDev &m, int32_t TM);
 int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
                   const CloudDev &m, int32_t TM, uint32_t *rec);
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
