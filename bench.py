@@ -82,7 +82,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-votes", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        ctx = ppf_amd.Context(0)
+        a = np.arange(1 << 26, dtype=np.uint32)
+        for _ in range(3):
+            b = ctx.debug_copy(a)
+        assert np.array_equal(a, b)
+        log(f"copied {a.nbytes} bytes x3")
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -126,6 +126,10 @@ typedef struct {
   int64_t n_clusters;      /* CPU clustering: number of clusters */
   double vote_ms;          /* device time of the vote kernel (HIP events) */
   double total_ms;         /* device time of the whole lookup */
+  double select_ms;        /* threshold + ordering of the kept votes */
+  double pose_ms;          /* one pose per kept vote */
+  double cluster_ms;       /* clustering (GPU or host) and pose extraction */
+  int64_t cluster_tests;   /* GPU clustering: (pose, neighbour pose) pairs tested */
 } ppf_result_info_t;
 int ppf_result_info(ppf_result result, ppf_result_info_t *info);
 
@@ -173,7 +177,9 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
  * op 0: exclusive scan of n uint32 (keys_io as uint32[n], result in place; the
  *       total is returned in vals_io[0]);
  * op 1: stable radix sort of n uint64 keys_io carrying uint32 vals_io (may be
- *       NULL) on bits [bit_lo, bit_hi). */
+ *       NULL) on bits [bit_lo, bit_hi);
+ * op 2: plain device copy kernel of n uint32 (keys_io in, vals_io out), a
+ *       known-size reference for hardware traffic counters. */
 int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
                         int32_t bit_lo, int32_t bit_hi);
 

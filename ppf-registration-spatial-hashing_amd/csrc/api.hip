@@ -276,6 +276,9 @@ int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipEventCreate(&c->ev2);
   if (e == hipSuccess) e = hipEventCreate(&c->ev3);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev4);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev5);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev6);
   if (e != hipSuccess) {
     set_error(std::string("ppf_ctx_create: ") + hipGetErrorString(e));
     delete c;
@@ -293,7 +296,7 @@ int ppf_ctx_destroy(ppf_ctx ctx) {
   ctx->sort.scan.release();
   ctx->sort.tmp_keys.release();
   ctx->sort.tmp_vals.release();
-  for (hipEvent_t ev : {ctx->ev0, ctx->ev1, ctx->ev2, ctx->ev3})
+  for (hipEvent_t ev : {ctx->ev0, ctx->ev1, ctx->ev2, ctx->ev3, ctx->ev4, ctx->ev5, ctx->ev6})
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -550,8 +553,10 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   unsigned long long *scode = sel.as<unsigned long long>();
   uint32_t *scount = reinterpret_cast<uint32_t *>(scode + sel_n);
   int64_t P = 0;
+  HIPC(hipEventRecord(ctx->ev4, st));
   TRY(ppfk::select_and_sort(ctx, ccode, ccount, n_cand, meta_h[1], p->vote_count_threshold,
                             s->cloud.n, m->cloud.n, scode, scount, &P));
+  HIPC(hipEventRecord(ctx->ev5, st));
   r->P = P;
   r->info.n_kept = P;
   if (P > 0) {
@@ -563,6 +568,7 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
     // ---- poses
     TRY(ppfk::compute_poses(ctx, r->codes.as<unsigned long long>(), P, m->frames.as<float>(),
                             s->frames.as<float>(), s->rpdf, r->T.as<float>()));
+    HIPC(hipEventRecord(ctx->ev6, st));
     // ---- clustering
     if (p->cpu_clustering) {
       std::vector<float> T((size_t)P * 16);
@@ -579,7 +585,8 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
       TRY(ppfk::cluster_spatial(ctx, r->T.as<float>(), r->codes.as<unsigned long long>(),
                                 r->counts.as<uint32_t>(), m->weights.as<float>(), P, m->d_dist,
                                 p->use_l1_norm != 0, p->use_averaged_clusters != 0, r->vc.as<float>(),
-                                r->trans.as<float>(), r->quats.as<float>(), &mi));
+                                r->trans.as<float>(), r->quats.as<float>(), &mi,
+                                &r->info.cluster_tests));
       r->info.max_idx = mi;
       float t3[3];
       HIPC(hipMemcpyAsync(r->pose, r->T.as<float>() + mi * 16, 64, hipMemcpyDeviceToHost, st));
@@ -599,6 +606,14 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   float tm = 0.0f;
   HIPC(hipEventElapsedTime(&tm, ctx->ev2, ctx->ev3));
   r->info.total_ms = tm;
+  HIPC(hipEventElapsedTime(&tm, ctx->ev4, ctx->ev5));
+  r->info.select_ms = tm;
+  if (P > 0) {
+    HIPC(hipEventElapsedTime(&tm, ctx->ev5, ctx->ev6));
+    r->info.pose_ms = tm;
+    HIPC(hipEventElapsedTime(&tm, ctx->ev6, ctx->ev3));
+    r->info.cluster_ms = tm;
+  }
   *out = r.release();
   return PPF_OK;
 }
@@ -737,9 +752,20 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
 int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
                         int32_t bit_lo, int32_t bit_hi) {
   ARG_CHECK(ctx && keys_io && n >= 0, "ppf_debug_primitive: bad arguments");
-  ARG_CHECK(op == 0 || op == 1, "ppf_debug_primitive: unknown op");
+  ARG_CHECK(op >= 0 && op <= 2, "ppf_debug_primitive: unknown op");
   TRY(set_device(ctx));
   hipStream_t st = ctx->stream;
+  if (op == 2) {
+    ARG_CHECK(vals_io && n % 4 == 0, "ppf_debug_primitive: copy needs an output and n % 4 == 0");
+    ppfi::DevBuf a, b;
+    TRY(a.ensure((size_t)n * 4 + 16));
+    TRY(b.ensure((size_t)n * 4 + 16));
+    HIPC(hipMemcpy(a.ptr, keys_io, (size_t)n * 4, hipMemcpyHostToDevice));
+    TRY(ppfk::debug_copy(ctx, a.as<uint32_t>(), b.as<uint32_t>(), n));
+    HIPC(hipMemcpyAsync(vals_io, b.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return PPF_OK;
+  }
   if (op == 0) {
     ARG_CHECK(vals_io, "ppf_debug_primitive: scan needs vals_io[0] for the total");
     ppfi::DevBuf a, b, tmp;

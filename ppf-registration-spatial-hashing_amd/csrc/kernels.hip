@@ -703,49 +703,59 @@ __global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long 
                                                          : ppfm::cell_hash(c[0] + a, c[1] + b, c[2] + k);
 }
 
-// Work items of the tiled clustering: (unique cell hash u, 256-pose chunk).
+// Work items of the tiled clustering: (unique cell hash u, 512-pose chunk).
 __global__ void k_cluster_chunks(const uint32_t *__restrict__ ucount, int64_t U,
                                  uint32_t *__restrict__ nchunk) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= U) return;
-  nchunk[u] = (ucount[u] + 255u) / 256u;
+  nchunk[u] = (ucount[u] + 511u) / 512u;
 }
 
 // Poses that share a voxel cell share the 26 neighbour hashes, so one block
-// takes up to 256 poses of one cell and streams every neighbour bucket through
-// LDS once; each thread walks the bucket in order, which keeps the summation
-// order of rot_clustering_kernel (kernel.cu:724-756).  Chunks whose poses do
-// not all lie in one cell (an FNV collision between cells) walk their own
-// neighbour lists per thread.
+// takes up to 512 poses of one cell (two per thread) and streams every
+// neighbour bucket through LDS once; each thread walks the bucket in order,
+// which keeps the summation order of rot_clustering_kernel
+// (kernel.cu:724-756).  Chunks whose poses do not all lie in one cell (an FNV
+// collision between cells) walk their own neighbour lists per pose.
+//
+// Translation test: sqrtf is correctly rounded and monotone, so
+// sqrtf(s) < d_dist holds exactly when s <= s_max, the largest float whose
+// square root rounds below d_dist (computed on the host).  s is the same
+// ((dx*dx + dy*dy) + dz*dz) sum as norm(); no square root is taken.
 constexpr int CL_BLOCK = 256;
+constexpr int CL_PER = 2;
+constexpr int CL_CHUNK = CL_BLOCK * CL_PER;
 
-__device__ __forceinline__ void cluster_test(float tx, float ty, float tz, float qx, float qy,
-                                             float qz, float qw, float jx, float jy, float jz,
-                                             float jqx, float jqy, float jqz, float jqw, float wj,
-                                             float trans_thresh, int use_l1, int use_avg,
-                                             float &vc, float &ox, float &oy, float &oz) {
+struct ClPose {
+  float tx, ty, tz, qx, qy, qz, qw;
+  float vc, ox, oy, oz;
+};
+
+__device__ __forceinline__ void cluster_test(ClPose &P, const float4 jq, const float4 jt,
+                                             float s_max, int use_l1, int use_avg) {
   const float rot_thresh_sq = ppfm::ROT_THRESH * ppfm::ROT_THRESH;
-  const float d4 = qx * jqx + qy * jqy + qz * jqz + qw * jqw;
+  const float d4 = P.qx * jq.x + P.qy * jq.y + P.qz * jq.z + P.qw * jq.w;
   const float qd = fabsf(8.0f * (1.0f - d4));
   if (qd < rot_thresh_sq) {
     if (!use_l1) {
-      const float dx = tx - jx, dy = ty - jy, dz = tz - jz;
-      const float nd = sqrtf(dx * dx + dy * dy + dz * dz);
-      if (!(nd < trans_thresh)) return;
+      const float dx = P.tx - jt.x, dy = P.ty - jt.y, dz = P.tz - jt.z;
+      const float ss = dx * dx + dy * dy + dz * dz;
+      if (!(ss <= s_max)) return;
     }
+    const float wj = jt.w;
     if (use_avg) {
-      ox = vc * ox;
-      oy = vc * oy;
-      oz = vc * oz;
-      ox = ox + wj * jx;
-      oy = oy + wj * jy;
-      oz = oz + wj * jz;
-      const float s = 1.0f / (vc + wj);
-      ox = s * ox;
-      oy = s * oy;
-      oz = s * oz;
+      P.ox = P.vc * P.ox;
+      P.oy = P.vc * P.oy;
+      P.oz = P.vc * P.oz;
+      P.ox = P.ox + wj * jt.x;
+      P.oy = P.oy + wj * jt.y;
+      P.oz = P.oz + wj * jt.z;
+      const float sc = 1.0f / (P.vc + wj);
+      P.ox = sc * P.ox;
+      P.oy = sc * P.oy;
+      P.oz = sc * P.oz;
     }
-    vc += wj;
+    P.vc += wj;
   }
 }
 
@@ -762,14 +772,30 @@ __device__ __forceinline__ int64_t cell_lower_bound(const uint32_t *__restrict__
   return (lo < U && ukeys[lo] == h) ? lo : -1;
 }
 
+__device__ __forceinline__ void load_pose(ClPose &P, const float *__restrict__ trans,
+                                          const float *__restrict__ quats, uint32_t i) {
+  P.tx = trans[3 * (size_t)i];
+  P.ty = trans[3 * (size_t)i + 1];
+  P.tz = trans[3 * (size_t)i + 2];
+  P.qx = quats[4 * (size_t)i];
+  P.qy = quats[4 * (size_t)i + 1];
+  P.qz = quats[4 * (size_t)i + 2];
+  P.qw = quats[4 * (size_t)i + 3];
+  P.vc = 1.0f;
+  P.ox = P.tx;
+  P.oy = P.ty;
+  P.oz = P.tz;
+}
+
 __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
     const float *__restrict__ trans, const float *__restrict__ quats, const float *__restrict__ w,
     const uint32_t *__restrict__ adj, const uint32_t *__restrict__ ukeys,
     const uint32_t *__restrict__ ufirst, const uint32_t *__restrict__ ucount, int64_t U,
-    const uint32_t *__restrict__ chunk_start, const uint32_t *__restrict__ members, float d_dist,
-    int use_l1, int use_avg, float *__restrict__ vc_out, float *__restrict__ trans_out) {
-  __shared__ float s_q[CL_BLOCK][4];
-  __shared__ float s_t[CL_BLOCK][4];
+    const uint32_t *__restrict__ chunk_start, const uint32_t *__restrict__ members, float s_max,
+    int use_l1, int use_avg, float *__restrict__ vc_out, float *__restrict__ trans_out,
+    unsigned long long *__restrict__ n_tests) {
+  __shared__ float4 s_q[CL_BLOCK];
+  __shared__ float4 s_t[CL_BLOCK];
   __shared__ int s_u, s_c, s_uniform;
   __shared__ uint32_t s_adj[27];
   __shared__ int64_t s_bucket;
@@ -792,21 +818,25 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   __syncthreads();
   const int u = s_u;
   const uint32_t f = ufirst[u], n = ucount[u];
-  const uint32_t k = (uint32_t)s_c * CL_BLOCK + tid;
-  const bool have = k < n;
-  const uint32_t i = have ? members[f + k] : members[f];
-  const uint32_t i0 = members[f + (uint32_t)s_c * CL_BLOCK];
-  // neighbour lists equal the chunk's first pose's list?
-  if (have) {
-    for (int q = 0; q < 27; ++q)
-      if (adj[27 * (size_t)i + q] != adj[27 * (size_t)i0 + q]) s_uniform = 0;
+  const uint32_t base = (uint32_t)s_c * CL_CHUNK;
+  const uint32_t i0 = members[f + base];
+  bool have[CL_PER];
+  uint32_t idx[CL_PER];
+  ClPose P[CL_PER];
+#pragma unroll
+  for (int r = 0; r < CL_PER; ++r) {
+    const uint32_t k = base + (uint32_t)(r * CL_BLOCK + tid);
+    have[r] = k < n;
+    idx[r] = have[r] ? members[f + k] : i0;
+    if (have[r]) {
+      for (int q = 0; q < 27; ++q)
+        if (adj[27 * (size_t)idx[r] + q] != adj[27 * (size_t)i0 + q]) s_uniform = 0;
+    }
+    load_pose(P[r], trans, quats, idx[r]);
   }
   if (tid < 27) s_adj[tid] = adj[27 * (size_t)i0 + tid];
   __syncthreads();
-  const float tx = trans[3 * (size_t)i], ty = trans[3 * (size_t)i + 1], tz = trans[3 * (size_t)i + 2];
-  const float qx = quats[4 * (size_t)i], qy = quats[4 * (size_t)i + 1], qz = quats[4 * (size_t)i + 2],
-              qw = quats[4 * (size_t)i + 3];
-  float vc = 1.0f, ox = tx, oy = ty, oz = tz;
+  unsigned long long tests = 0;
   if (s_uniform) {
     for (int blk = 0; blk < 27; ++blk) {
       const uint32_t h = s_adj[blk];
@@ -817,50 +847,67 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
       __syncthreads();
       if (bu < 0) continue;
       const uint32_t bf = ufirst[bu], bn = ucount[bu];
+#pragma unroll
+      for (int r = 0; r < CL_PER; ++r) tests += have[r] ? bn : 0u;
       for (uint32_t c0 = 0; c0 < bn; c0 += CL_BLOCK) {
         const uint32_t cn = min((uint32_t)CL_BLOCK, bn - c0);
         if ((uint32_t)tid < cn) {
           const uint32_t j = members[bf + c0 + tid];
-          s_q[tid][0] = quats[4 * (size_t)j];
-          s_q[tid][1] = quats[4 * (size_t)j + 1];
-          s_q[tid][2] = quats[4 * (size_t)j + 2];
-          s_q[tid][3] = quats[4 * (size_t)j + 3];
-          s_t[tid][0] = trans[3 * (size_t)j];
-          s_t[tid][1] = trans[3 * (size_t)j + 1];
-          s_t[tid][2] = trans[3 * (size_t)j + 2];
-          s_t[tid][3] = w[j];
+          s_q[tid] = make_float4(quats[4 * (size_t)j], quats[4 * (size_t)j + 1],
+                                 quats[4 * (size_t)j + 2], quats[4 * (size_t)j + 3]);
+          s_t[tid] = make_float4(trans[3 * (size_t)j], trans[3 * (size_t)j + 1],
+                                 trans[3 * (size_t)j + 2], w[j]);
         }
         __syncthreads();
-        if (have) {
-          for (uint32_t jj = 0; jj < cn; ++jj)
-            cluster_test(tx, ty, tz, qx, qy, qz, qw, s_t[jj][0], s_t[jj][1], s_t[jj][2], s_q[jj][0],
-                         s_q[jj][1], s_q[jj][2], s_q[jj][3], s_t[jj][3], d_dist, use_l1, use_avg, vc,
-                         ox, oy, oz);
+        uint32_t jj = 0;
+        for (; jj + 2 <= cn; jj += 2) {
+          const float4 q0 = s_q[jj], t0 = s_t[jj], q1 = s_q[jj + 1], t1 = s_t[jj + 1];
+#pragma unroll
+          for (int r = 0; r < CL_PER; ++r) {
+            cluster_test(P[r], q0, t0, s_max, use_l1, use_avg);
+            cluster_test(P[r], q1, t1, s_max, use_l1, use_avg);
+          }
+        }
+        if (jj < cn) {
+          const float4 q0 = s_q[jj], t0 = s_t[jj];
+#pragma unroll
+          for (int r = 0; r < CL_PER; ++r) cluster_test(P[r], q0, t0, s_max, use_l1, use_avg);
         }
         __syncthreads();
       }
     }
-  } else if (have) {
-    for (int blk = 0; blk < 27; ++blk) {
-      const uint32_t h = adj[27 * (size_t)i + blk];
-      if (h == 0u) continue;
-      const int64_t bu = cell_lower_bound(ukeys, U, h);
-      if (bu < 0) continue;
-      const uint32_t bf = ufirst[bu], bn = ucount[bu];
-      for (uint32_t jj = 0; jj < bn; ++jj) {
-        const uint32_t j = members[bf + jj];
-        cluster_test(tx, ty, tz, qx, qy, qz, qw, trans[3 * (size_t)j], trans[3 * (size_t)j + 1],
-                     trans[3 * (size_t)j + 2], quats[4 * (size_t)j], quats[4 * (size_t)j + 1],
-                     quats[4 * (size_t)j + 2], quats[4 * (size_t)j + 3], w[j], d_dist, use_l1, use_avg,
-                     vc, ox, oy, oz);
+  } else {
+#pragma unroll
+    for (int r = 0; r < CL_PER; ++r) {
+      if (!have[r]) continue;
+      for (int blk = 0; blk < 27; ++blk) {
+        const uint32_t h = adj[27 * (size_t)idx[r] + blk];
+        if (h == 0u) continue;
+        const int64_t bu = cell_lower_bound(ukeys, U, h);
+        if (bu < 0) continue;
+        const uint32_t bf = ufirst[bu], bn = ucount[bu];
+        tests += bn;
+        for (uint32_t jj = 0; jj < bn; ++jj) {
+          const uint32_t j = members[bf + jj];
+          const float4 jq = make_float4(quats[4 * (size_t)j], quats[4 * (size_t)j + 1],
+                                        quats[4 * (size_t)j + 2], quats[4 * (size_t)j + 3]);
+          const float4 jt = make_float4(trans[3 * (size_t)j], trans[3 * (size_t)j + 1],
+                                        trans[3 * (size_t)j + 2], w[j]);
+          cluster_test(P[r], jq, jt, s_max, use_l1, use_avg);
+        }
       }
     }
   }
-  if (have) {
-    vc_out[i] = vc;
-    trans_out[3 * (size_t)i] = ox;
-    trans_out[3 * (size_t)i + 1] = oy;
-    trans_out[3 * (size_t)i + 2] = oz;
+  tests = wave_sum_u64(tests);
+  if ((tid % WAVE) == 0 && tests) atomicAdd(n_tests, tests);
+#pragma unroll
+  for (int r = 0; r < CL_PER; ++r) {
+    if (have[r]) {
+      vc_out[idx[r]] = P[r].vc;
+      trans_out[3 * (size_t)idx[r]] = P[r].ox;
+      trans_out[3 * (size_t)idx[r] + 1] = P[r].oy;
+      trans_out[3 * (size_t)idx[r] + 2] = P[r].oz;
+    }
   }
 }
 
@@ -897,6 +944,11 @@ __global__ __launch_bounds__(1024) void k_argmax(const float *__restrict__ v, in
 }
 
 // ---------------------------------------------------------- diagnostics
+__global__ void k_copy_u32(const uint4 *__restrict__ in, uint4 *__restrict__ out, int64_t n4) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n4) out[i] = in[i];
+}
+
 __global__ void k_debug_numerics(int op, const float *__restrict__ a, const float *__restrict__ b,
                                  float *__restrict__ out, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -918,6 +970,24 @@ __global__ void k_debug_numerics(int op, const float *__restrict__ a, const floa
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Largest float s with sqrtf(s) < t (t > 0 finite); sqrtf is correctly
+// rounded and monotone, so {s : sqrtf(s) < t} is a prefix of the floats.
+float sqrt_below_max(float t) {
+  float s = t * t;
+  if (sqrtf(s) < t) {
+    for (;;) {
+      const float nx = nextafterf(s, INFINITY);
+      if (!(sqrtf(nx) < t)) break;
+      s = nx;
+    }
+  } else {
+    do {
+      s = nextafterf(s, 0.0f);
+    } while (!(sqrtf(s) < t) && s > 0.0f);
+  }
+  return s;
+}
 
 int bitlen(uint64_t x) {
   int b = 0;
@@ -1128,11 +1198,13 @@ int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const
 int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
-                    int64_t *max_idx) {
+                    int64_t *max_idx, int64_t *pair_tests) {
   *max_idx = -1;
+  *pair_tests = 0;
   if (P <= 0) return 0;
   // trans (3P) w (P) words (P u64) adj (27P) ukeys/ufirst/ucount (3P) members (P) argmax (8)
-  const size_t bytes = (size_t)P * (12 + 4 + 8 + 108 + 12 + 4) + 64;
+  // test counter (8)
+  const size_t bytes = (size_t)P * (12 + 4 + 8 + 108 + 12 + 4) + 128;
   PPF_TRY(ctx->tmp_b.ensure(bytes));
   char *p = ctx->tmp_b.as<char>();
   uint64_t *words = reinterpret_cast<uint64_t *>(p);
@@ -1152,6 +1224,8 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   uint32_t *members = reinterpret_cast<uint32_t *>(p);
   p += (size_t)P * 4;
   long long *am = reinterpret_cast<long long *>(((uintptr_t)p + 7) & ~(uintptr_t)7);
+  unsigned long long *ntest = reinterpret_cast<unsigned long long *>(am + 1);
+  PPF_HIP(hipMemsetAsync(ntest, 0, 8, ctx->stream));
   hipLaunchKernelGGL(k_cluster_prep, dim3(nblk(P, 128)), dim3(128), 0, ctx->stream, T, codes, counts,
                      weights, P, d_dist, trans, quats, w, words, adj);
   PPF_HIP(hipGetLastError());
@@ -1171,14 +1245,24 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   PPF_HIP(hipStreamSynchronize(ctx->stream));
   if (n_items > 0)
     hipLaunchKernelGGL(k_cluster_tiled, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, w,
-                       adj, ukeys, ufirst, ucount, U, cstart, members, d_dist, use_l1 ? 1 : 0,
-                       use_avg ? 1 : 0, vc, trans_out);
+                       adj, ukeys, ufirst, ucount, U, cstart, members, sqrt_below_max(d_dist),
+                       use_l1 ? 1 : 0, use_avg ? 1 : 0, vc, trans_out, ntest);
   hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
   PPF_HIP(hipGetLastError());
-  long long mi = -1;
-  PPF_HIP(hipMemcpyAsync(&mi, am, 8, hipMemcpyDeviceToHost, ctx->stream));
+  long long mi_t[2] = {-1, 0};
+  PPF_HIP(hipMemcpyAsync(mi_t, am, 16, hipMemcpyDeviceToHost, ctx->stream));
   PPF_HIP(hipStreamSynchronize(ctx->stream));
-  *max_idx = mi;
+  *max_idx = mi_t[0];
+  *pair_tests = mi_t[1];
+  return 0;
+}
+
+int debug_copy(ppf_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n) {
+  const int64_t n4 = n / 4;
+  if (n4 > 0)
+    hipLaunchKernelGGL(k_copy_u32, dim3(nblk(n4, 256)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const uint4 *>(in), reinterpret_cast<uint4 *>(out), n4);
+  PPF_HIP(hipGetLastError());
   return 0;
 }
 

@@ -80,6 +80,11 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 // inclusive prefix sum across the wave
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
   int l = __lane_id();

@@ -29,6 +29,7 @@ struct ppf_ctx_s {
   ppfi::SortScratch sort;
   ppfi::DevBuf tmp_a, tmp_b, tmp_c, tmp_d, hits, queue;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
 };
 
 // Tracked device allocation owned by a model/scene/result.
@@ -155,7 +156,8 @@ int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const
 int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
-                    int64_t *max_idx);
+                    int64_t *max_idx, int64_t *pair_tests);
 int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *out, int64_t n);
+int debug_copy(ppf_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n);  // n multiple of 4
 
 }  // namespace ppfk

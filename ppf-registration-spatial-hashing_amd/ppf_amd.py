@@ -56,7 +56,9 @@ class ResultInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("lookups", C.c_int64), ("votes", C.c_int64),
                 ("max_count", C.c_int64),
                 ("n_candidates", C.c_int64), ("n_kept", C.c_int64), ("max_idx", C.c_int64),
-                ("n_clusters", C.c_int64), ("vote_ms", C.c_double), ("total_ms", C.c_double)]
+                ("n_clusters", C.c_int64), ("vote_ms", C.c_double), ("total_ms", C.c_double),
+                ("select_ms", C.c_double), ("pose_ms", C.c_double), ("cluster_ms", C.c_double),
+                ("cluster_tests", C.c_int64)]
 
 
 _lib = None
@@ -198,6 +200,12 @@ class Context:
         tot = np.zeros(1, np.uint32)
         _check(lib().ppf_debug_primitive(self.handle, 0, x.ctypes.data, tot.ctypes.data, len(x), 0, 0))
         return x, int(tot[0])
+
+    def debug_copy(self, a: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(a, dtype=np.uint32)
+        out = np.zeros_like(x)
+        _check(lib().ppf_debug_primitive(self.handle, 2, x.ctypes.data, out.ctypes.data, len(x), 0, 0))
+        return out
 
     def debug_sort(self, keys: np.ndarray, vals: Optional[np.ndarray], bit_lo: int, bit_hi: int):
         k = np.ascontiguousarray(keys, dtype=np.uint64).copy()

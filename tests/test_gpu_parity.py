@@ -255,3 +255,8 @@ def test_device_radix_sort_is_stable(ctx, case):
         digit = (k >> np.uint64(lo)) & np.uint64((1 << (hi - lo)) - 1) if hi - lo < 64 else k
         order = np.argsort(digit, kind="stable")
         assert np.array_equal(sv, v[order]) and np.array_equal(sk, k[order])
+
+
+def test_device_copy_kernel(ctx):
+    a = np.random.default_rng(0).integers(0, 2**32, 1 << 20, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(ctx.debug_copy(a), a)
