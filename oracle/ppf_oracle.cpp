@@ -920,22 +920,72 @@ void *oracle_model_create(const float *m_pts, const float *m_nrm, int64_t n_m, f
 
 void oracle_model_destroy(void *h) { delete static_cast<ModelDesc *>(h); }
 
+// Timing form of vote_histogram (mode 1) for a handful of reference points:
+// rows run one after another and each row's scene partners are split over the
+// OpenMP threads (per-thread [m_r][alpha] accumulators summed at the end of
+// the row), so a sample of a few rows still keeps every thread busy.
 int oracle_vote_throughput_h(void *h, const float *m_pts, const float *m_nrm, const float *s_pts,
                              const float *s_nrm, int64_t n_s, float d_dist, const int64_t *ref_list,
                              int64_t n_ref, int64_t *stats, double *seconds) {
   if (!h) return 1;
   const ModelDesc &md = *static_cast<ModelDesc *>(h);
-  std::vector<int64_t> refs(ref_list, ref_list + n_ref);
-  std::vector<VoteEntry> e;
-  int64_t L, H, V;
+  const int64_t n_m = md.n;
   auto t0 = std::chrono::steady_clock::now();
-  vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, 1, refs, e, &L, &H, &V);
+  std::vector<float> Tm((size_t)n_m * 16);
+  for (int64_t i = 0; i < n_m; ++i)
+    frame_to_global(pt(m_pts, i), pt(m_nrm, i), (float(*)[4]) & Tm[16 * i]);
+  const int nt = omp_get_max_threads();
+  std::vector<std::vector<uint32_t>> acc(nt, std::vector<uint32_t>((size_t)n_m * N_BINS));
+  std::vector<uint32_t> row((size_t)n_m * N_BINS);
+  int64_t L = 0, H = 0, V = 0, E = 0;
+  for (int64_t r = 0; r < n_ref; ++r) {
+    const int64_t s_r = ref_list[r];
+    float3 sr = pt(s_pts, s_r), nsr = pt(s_nrm, s_r);
+    float Ts[4][4];
+    frame_to_global(sr, nsr, Ts);
+#pragma omp parallel reduction(+ : L, H, V)
+    {
+      std::vector<uint32_t> &a = acc[omp_get_thread_num()];
+      std::fill(a.begin(), a.end(), 0u);
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t s_i = 0; s_i < n_s; ++s_i) {
+        if (s_i == s_r) continue;
+        ++L;
+        float3 si = pt(s_pts, s_i);
+        float4 f = compute_ppf(sr, nsr, si, pt(s_nrm, s_i));
+        uint32_t key = feature_key(disc_feature(f, d_dist, D_ANGLE0));
+        if (key == 0) continue;
+        int64_t u = find_key(md.ha, key);
+        if (u < 0) continue;
+        ++H;
+        uint64_t cnt = md.ha.counts[u], first = md.ha.first[u];
+        float3 v = dehomogenize(mat4f_vmul(Ts, homogenize(si)));
+        for (uint64_t i = 0; i < cnt; ++i) {
+          uint64_t pair = md.ha.key2data[first + i];
+          uint64_t m_r = pair / (uint64_t)n_m;
+          uint64_t m_i = pair - m_r * (uint64_t)n_m;
+          float3 uu =
+              dehomogenize(mat4f_vmul((float(*)[4]) & Tm[16 * m_r], homogenize(pt(m_pts, m_i))));
+          a[m_r * N_BINS + alpha_from_uv(uu, v)]++;
+          ++V;
+        }
+      }
+    }
+    std::fill(row.begin(), row.end(), 0u);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n_m * N_BINS; ++k) {
+      uint32_t c = 0;
+      for (int t = 0; t < nt; ++t) c += acc[t][k];
+      row[k] = c;
+    }
+    for (int64_t k = 0; k < n_m * N_BINS; ++k) E += row[k] != 0;
+  }
   auto t1 = std::chrono::steady_clock::now();
   *seconds = std::chrono::duration<double>(t1 - t0).count();
   stats[0] = L;
   stats[1] = H;
   stats[2] = V;
-  stats[3] = (int64_t)e.size();
+  stats[3] = E;
   return 0;
 }
 

@@ -210,7 +210,7 @@ def vote_throughput(model, scene, d_dist, refs):
                                       _c(scene[1], np.float32), len(scene[0]), d_dist, refs,
                                       len(refs), stats, C.byref(sec))
     assert rc == 0
-    return dict(lookups=int(stats[0]), hits=int(stats[1]), votes=int(stats[2]),
+    return dict(lookups=int(stats[0]), hits=int(stats[1]), votes=int(stats[2]), entries=int(stats[3]),
                 seconds=float(sec.value))
 
 
@@ -243,7 +243,7 @@ class OracleModel:
                                             len(scene[0]), self.d_dist, refs, len(refs), stats,
                                             C.byref(sec))
         assert rc == 0
-        return dict(lookups=int(stats[0]), hits=int(stats[1]), votes=int(stats[2]),
+        return dict(lookups=int(stats[0]), hits=int(stats[1]), votes=int(stats[2]), entries=int(stats[3]),
                     seconds=float(sec.value))
 
     def __del__(self):

@@ -248,3 +248,16 @@ def test_registration_recovers_ground_truth(cpu):
     # the reference's validation rule: translation < 0.1*diameter, rotation < 12 deg
     assert np.linalg.norm(pose[:3, 3] - truth[:3, 3]) < 0.1 * D
     assert ang < 12.0
+
+
+def test_timing_form_counts_match_vote_histogram():
+    """The row-parallel timing loop (used for the CPU baseline) does the same
+    work as the vote histogram: same lookups, hits, votes and nonzero bins."""
+    m, s, d, _ = tiny_workload(seed=5, n_model=80, n_scene=400)
+    refs = np.array([0, 7, 133, 399], np.int64)
+    a = O.vote_throughput(m, s, d, refs)
+    om = O.OracleModel(m, d)
+    b = om.vote_throughput(s, refs)
+    for k in ("lookups", "hits", "votes", "entries"):
+        assert a[k] == b[k], k
+    assert a["votes"] > 0 and a["lookups"] == len(refs) * (len(s[0]) - 1)
