@@ -161,6 +161,29 @@ int ppf_registration(ppf_ctx ctx, int32_t n_scenes, const float *const *scene_po
                      int32_t use_averaged_clusters, const float *const *model_weights,
                      float *poses_out, int32_t *status_out);
 
+/* ----------------------------------------------------------------- batched
+ * Many (model, scene) pairs in one call: the batched form of the loop in
+ * ppf_registration (src/cuda/ppf.cu:63-98) over device-resident models.
+ * All jobs are voted in one persistent launch (work items are (job, reference
+ * point), longest jobs first); the stages after voting run concurrently on
+ * the context's lanes.  results_out[j] is the result of (models[j],
+ * scenes[j]) and is bit-identical to ppf_lookup(models[j], scenes[j], params).
+ * A model or scene may appear in several jobs.  info may be NULL. */
+#define PPF_MAX_BATCH_JOBS 65536
+typedef struct {
+  int64_t n_jobs;
+  int64_t lookups;       /* sum over jobs */
+  int64_t votes;         /* sum over jobs */
+  int32_t vote_launches; /* 1, or more when a job's candidate buffer overflowed */
+  int32_t lanes;         /* lanes that finished the jobs */
+  double vote_ms;        /* device time of the vote launches */
+  double finish_ms;      /* wall time of threshold/order, poses and clustering */
+  double total_ms;       /* wall time of the call */
+} ppf_batch_info_t;
+int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const ppf_scene *scenes,
+                     const ppf_lookup_params *params, ppf_result *results_out,
+                     ppf_batch_info_t *info);
+
 /* sizeof(ppf_model_info_t), sizeof(ppf_lookup_params), sizeof(ppf_result_info_t):
  * lets a binding check that it was written against the same header. */
 void ppf_abi_sizes(uint32_t sizes_out[3]);

@@ -39,31 +39,89 @@ void DevBuf::release() {
 
 int TBuf::alloc(ppf_ctx c, size_t n) {
   release();
-  ctx = c;
+  ctx = c->root;
   size_t want = std::max<size_t>(n, 256);
-  if (c->budget && c->bytes.load() + want > c->budget) {
-    set_error("device memory budget exceeded (" + std::to_string(c->bytes.load() + want) + " > " +
-              std::to_string(c->budget) + " bytes)");
+  if (!ctx->reserve(want)) {
+    set_error("device memory budget exceeded (" + std::to_string(ctx->bytes.load() + want) + " > " +
+              std::to_string(ctx->budget) + " bytes)");
     return PPF_ERR_NOMEM;
   }
   hipError_t e = hipMalloc(&ptr, want);
   if (e != hipSuccess) {
     ptr = nullptr;
+    ctx->unreserve(want);
     set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
     return PPF_ERR_NOMEM;
   }
   bytes = want;
-  c->bytes += want;
   return 0;
 }
 void TBuf::release() {
   if (ptr) {
     (void)hipFree(ptr);
-    if (ctx) ctx->bytes -= bytes;
+    if (ctx) ctx->unreserve(bytes);
   }
   ptr = nullptr;
   bytes = 0;
 }
+
+namespace ppfi {
+int ctx_init(ppf_ctx_s *c, int device) {
+  c->device = device;
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+  int lds = 0;
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+  c->max_lds = (size_t)lds;
+  for (hipEvent_t *ev : {&c->ev0, &c->ev1, &c->ev2, &c->ev3, &c->ev4, &c->ev5, &c->ev6})
+    if (e == hipSuccess) e = hipEventCreate(ev);
+  if (e != hipSuccess) {
+    set_error(std::string("context set-up: ") + hipGetErrorString(e));
+    return PPF_ERR_HIP;
+  }
+  return 0;
+}
+
+void ctx_release(ppf_ctx_s *c) {
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (ppf_ctx_s *l : c->lanes) {
+    ctx_release(l);
+    delete l;
+  }
+  c->lanes.clear();
+  c->sort.counts.release();
+  c->sort.scan.release();
+  c->sort.tmp_keys.release();
+  c->sort.tmp_vals.release();
+  for (DevBuf *b : {&c->tmp_a, &c->tmp_b, &c->tmp_c, &c->tmp_d, &c->hits, &c->queue}) b->release();
+  for (hipEvent_t *ev : {&c->ev0, &c->ev1, &c->ev2, &c->ev3, &c->ev4, &c->ev5, &c->ev6})
+    if (*ev) {
+      (void)hipEventDestroy(*ev);
+      *ev = nullptr;
+    }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+int ctx_lanes(ppf_ctx_s *root, int n, std::vector<ppf_ctx_s *> &out) {
+  while ((int)root->lanes.size() < n) {
+    auto *l = new ppf_ctx_s();
+    l->root = root;
+    int rc = ctx_init(l, root->device);
+    if (rc) {
+      ctx_release(l);
+      delete l;
+      return rc;
+    }
+    root->lanes.push_back(l);
+  }
+  out.assign(root->lanes.begin(), root->lanes.begin() + n);
+  return 0;
+}
+}  // namespace ppfi
 
 #define ARG_CHECK(cond, msg)  \
   do {                        \
@@ -241,10 +299,132 @@ void greedy_cluster(const std::vector<float> &T, const std::vector<uint32_t> &co
   pose_out[15] = 1.0f;
 }
 
-constexpr uint32_t CAND_CAP_MIN = 1u << 23;
-constexpr uint32_t CAND_CAP_MAX = 1u << 28;
 
 }  // namespace
+
+namespace ppfi {
+ppfk::VoteJob vote_job(ppf_model m, ppf_scene s, float thr, unsigned long long *cand_code,
+                       uint32_t *cand_count, uint32_t *meta, uint32_t cap) {
+  ppfk::VoteJob J{};
+  J.sx = s->cloud.x();
+  J.sy = s->cloud.y();
+  J.sz = s->cloud.z();
+  J.snx = s->cloud.nx();
+  J.sny = s->cloud.ny();
+  J.snz = s->cloud.nz();
+  J.snn = s->cloud.nn();
+  J.sframe = s->frames.as<float>();
+  J.n_s = (int32_t)s->cloud.n;
+  J.rpdf = (int32_t)s->rpdf;
+  J.R = (int32_t)s->R;
+  J.r_begin = 0;
+  J.htab = m->htab.as<uint32_t>();
+  J.hshift = m->hshift;
+  J.hmask = m->hcap - 1;
+  J.toff = m->toff.as<uint32_t>();
+  J.rec = m->rec.as<uint32_t>();
+  J.pair = m->vpair.as<uint32_t>();
+  J.mframe = m->frames.as<float>();
+  J.mx = m->cloud.x();
+  J.my = m->cloud.y();
+  J.mz = m->cloud.z();
+  J.n_m = (int32_t)m->cloud.n;
+  J.TM = m->TM;
+  J.T = m->T;
+  J.d_dist = m->d_dist;
+  J.thr = thr;
+  J.max_count = meta + 1;
+  J.cand_code = cand_code;
+  J.cand_count = cand_count;
+  J.cand_n = meta;
+  J.cand_cap = cap;
+  J.votes = reinterpret_cast<unsigned long long *>(meta + 2);
+  J.item_begin = 0;
+  return J;
+}
+}  // namespace ppfi
+
+namespace ppfi {
+// Everything after voting for one (model, scene) pair, on one context or
+// lane: the final threshold and order of the candidates (model.cu:148-170),
+// one pose per kept vote (model.cu:191-200), then GPU spatial-hash or host
+// greedy clustering and the final pose (model.cu:202-266, ppf.cu:74-93).
+// Records ev4 (start), ev5, ev6 and ev3 (end) on the context's stream and
+// fills the timings and outputs of r; r->info vote fields are the caller's.
+int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *p,
+                  const unsigned long long *ccode, const uint32_t *ccount, uint32_t n_cand,
+                  uint32_t max_count, ppf_result_s *r) {
+  hipStream_t st = ctx->stream;
+  r->ctx = ctx->root;
+  r->n_m = m->cloud.n;
+  r->cpu = p->cpu_clustering != 0;
+  r->info.max_idx = -1;
+  ppfi::DevBuf sel;
+  const size_t sel_n = std::max<uint32_t>(n_cand, 1);
+  TRY(sel.ensure(sel_n * 12));
+  unsigned long long *scode = sel.as<unsigned long long>();
+  uint32_t *scount = reinterpret_cast<uint32_t *>(scode + sel_n);
+  int64_t P = 0;
+  HIPC(hipEventRecord(ctx->ev4, st));
+  TRY(ppfk::select_and_sort(ctx, ccode, ccount, n_cand, max_count, p->vote_count_threshold,
+                            s->cloud.n, m->cloud.n, scode, scount, &P));
+  HIPC(hipEventRecord(ctx->ev5, st));
+  r->P = P;
+  r->info.n_kept = P;
+  if (P > 0) {
+    TRY(r->codes.alloc(ctx, (size_t)P * 8));
+    TRY(r->counts.alloc(ctx, (size_t)P * 4));
+    TRY(r->T.alloc(ctx, (size_t)P * 64));
+    HIPC(hipMemcpyAsync(r->codes.ptr, scode, (size_t)P * 8, hipMemcpyDeviceToDevice, st));
+    HIPC(hipMemcpyAsync(r->counts.ptr, scount, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
+    TRY(ppfk::compute_poses(ctx, r->codes.as<unsigned long long>(), P, m->frames.as<float>(),
+                            s->frames.as<float>(), s->rpdf, r->T.as<float>()));
+    HIPC(hipEventRecord(ctx->ev6, st));
+    if (p->cpu_clustering) {
+      std::vector<float> T((size_t)P * 16);
+      std::vector<uint32_t> counts((size_t)P);
+      HIPC(hipMemcpyAsync(T.data(), r->T.ptr, (size_t)P * 64, hipMemcpyDeviceToHost, st));
+      HIPC(hipMemcpyAsync(counts.data(), r->counts.ptr, (size_t)P * 4, hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      greedy_cluster(T, counts, m->d_dist, ppfm::D_ANGLE0, r->pose, &r->info.n_clusters);
+    } else {
+      TRY(r->vc.alloc(ctx, (size_t)P * 4));
+      TRY(r->trans.alloc(ctx, (size_t)P * 12));
+      TRY(r->quats.alloc(ctx, (size_t)P * 16));
+      int64_t mi = -1;
+      TRY(ppfk::cluster_spatial(ctx, r->T.as<float>(), r->codes.as<unsigned long long>(),
+                                r->counts.as<uint32_t>(), m->weights.as<float>(), P, m->d_dist,
+                                p->use_l1_norm != 0, p->use_averaged_clusters != 0, r->vc.as<float>(),
+                                r->trans.as<float>(), r->quats.as<float>(), &mi,
+                                &r->info.cluster_tests));
+      r->info.max_idx = mi;
+      float t3[3];
+      HIPC(hipMemcpyAsync(r->pose, r->T.as<float>() + mi * 16, 64, hipMemcpyDeviceToHost, st));
+      HIPC(hipMemcpyAsync(t3, r->trans.as<float>() + mi * 3, 12, hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      // src/cuda/ppf.cu:78-93: translation column from transformation_trans
+      r->pose[3] = t3[0];
+      r->pose[7] = t3[1];
+      r->pose[11] = t3[2];
+    }
+    r->info.status = PPF_OK;
+  } else {
+    r->info.status = PPF_ERR_NO_VOTES;
+  }
+  HIPC(hipEventRecord(ctx->ev3, st));
+  HIPC(hipEventSynchronize(ctx->ev3));
+  float tm = 0.0f;
+  HIPC(hipEventElapsedTime(&tm, ctx->ev4, ctx->ev5));
+  r->info.select_ms = tm;
+  if (P > 0) {
+    HIPC(hipEventElapsedTime(&tm, ctx->ev5, ctx->ev6));
+    r->info.pose_ms = tm;
+    HIPC(hipEventElapsedTime(&tm, ctx->ev6, ctx->ev3));
+    r->info.cluster_ms = tm;
+  }
+  return 0;
+}
+}  // namespace ppfi
 
 extern "C" {
 
@@ -263,26 +443,12 @@ int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
   HIPC(hipGetDeviceCount(&n));
   ARG_CHECK(n > 0, "ppf_ctx_create: no HIP device");
   auto *c = new ppf_ctx_s();
-  c->device = std::min(n - 1, std::max(0, device));
   c->budget = hbm_budget_bytes;
-  hipError_t e = hipSetDevice(c->device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device);
-  int lds = 0;
-  if (e == hipSuccess)
-    e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-  c->max_lds = (size_t)lds;
-  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev2);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev3);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev4);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev5);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev6);
-  if (e != hipSuccess) {
-    set_error(std::string("ppf_ctx_create: ") + hipGetErrorString(e));
+  int rc = ctx_init(c, std::min(n - 1, std::max(0, device)));
+  if (rc) {
+    ctx_release(c);
     delete c;
-    return PPF_ERR_HIP;
+    return rc;
   }
   *out = c;
   return PPF_OK;
@@ -290,20 +456,13 @@ int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
 
 int ppf_ctx_destroy(ppf_ctx ctx) {
   if (!ctx) return PPF_OK;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  ctx->sort.counts.release();
-  ctx->sort.scan.release();
-  ctx->sort.tmp_keys.release();
-  ctx->sort.tmp_vals.release();
-  for (hipEvent_t ev : {ctx->ev0, ctx->ev1, ctx->ev2, ctx->ev3, ctx->ev4, ctx->ev5, ctx->ev6})
-    if (ev) (void)hipEventDestroy(ev);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  ARG_CHECK(ctx->root == ctx, "ppf_ctx_destroy: not a context created by ppf_ctx_create");
+  ctx_release(ctx);
   delete ctx;
   return PPF_OK;
 }
 
-uint64_t ppf_ctx_device_bytes(ppf_ctx ctx) { return ctx ? ctx->bytes.load() : 0; }
+uint64_t ppf_ctx_device_bytes(ppf_ctx ctx) { return ctx ? ctx->root->bytes.load() : 0; }
 
 // ------------------------------------------------------------------ model
 int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int64_t n,
@@ -456,6 +615,7 @@ int ppf_scene_keys(ppf_scene s, float d_dist, uint32_t *keys_out) {
   return PPF_OK;
 }
 
+
 // ----------------------------------------------------------------- lookup
 int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result *out) {
   ARG_CHECK(m && s && p && out, "ppf_lookup: NULL argument");
@@ -476,7 +636,7 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   ppfi::DevBuf cand;
   // candidates per reference point are bounded by the pruning; start at
   // 2048 per reference point (at least 8M), grow on overflow
-  uint32_t cap = (uint32_t)std::min<int64_t>(CAND_CAP_MAX, std::max<int64_t>(CAND_CAP_MIN, s->R * 2048));
+  uint32_t cap = (uint32_t)std::min<int64_t>(CAND_CAP_MAX, std::max<int64_t>(CAND_CAP_MIN, s->R * (int64_t)CAND_PER_REF));
   uint32_t meta_h[2] = {0, 0};
   int64_t votes = 0;
   float vote_ms = 0.0f;
@@ -487,40 +647,7 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
     uint32_t *meta = cn + cap;  // [0] candidates emitted, [1] running maximum, [2..3] votes
     const uint32_t preset[4] = {0u, meta_h[1], 0u, 0u};
     HIPC(hipMemcpyAsync(meta, preset, 16, hipMemcpyHostToDevice, st));
-    ppfk::VoteJob J{};
-    J.sx = s->cloud.x();
-    J.sy = s->cloud.y();
-    J.sz = s->cloud.z();
-    J.snx = s->cloud.nx();
-    J.sny = s->cloud.ny();
-    J.snz = s->cloud.nz();
-    J.snn = s->cloud.nn();
-    J.sframe = s->frames.as<float>();
-    J.n_s = (int32_t)s->cloud.n;
-    J.rpdf = (int32_t)s->rpdf;
-    J.R = (int32_t)s->R;
-    J.htab = m->htab.as<uint32_t>();
-    J.hshift = m->hshift;
-    J.hmask = m->hcap - 1;
-    J.toff = m->toff.as<uint32_t>();
-    J.rec = m->rec.as<uint32_t>();
-    J.pair = m->vpair.as<uint32_t>();
-    J.mframe = m->frames.as<float>();
-    J.mx = m->cloud.x();
-    J.my = m->cloud.y();
-    J.mz = m->cloud.z();
-    J.n_m = (int32_t)m->cloud.n;
-    J.TM = m->TM;
-    J.T = m->T;
-    J.d_dist = m->d_dist;
-    J.thr = p->vote_count_threshold;
-    J.max_count = meta + 1;
-    J.cand_code = cc;
-    J.cand_count = cn;
-    J.cand_n = meta;
-    J.cand_cap = cap;
-    J.votes = reinterpret_cast<unsigned long long *>(meta + 2);
-    J.item_begin = 0;
+    ppfk::VoteJob J = ppfi::vote_job(m, s, p->vote_count_threshold, cc, cn, meta, cap);
     float ms = 0.0f;
     TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)s->R, m->TM, s->cloud.n, p->exact_votes != 0, &ms));
     vote_ms += ms;
@@ -542,78 +669,12 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   r->info.max_count = meta_h[1];
   r->info.n_candidates = n_cand;
   r->info.vote_ms = vote_ms;
-  r->info.max_idx = -1;
-
-  // ---- threshold and order (count desc, code asc)
   const unsigned long long *ccode = cand.as<unsigned long long>();
   const uint32_t *ccount = reinterpret_cast<const uint32_t *>(ccode + cap);
-  ppfi::DevBuf sel;
-  const size_t sel_n = std::max<uint32_t>(n_cand, 1);
-  TRY(sel.ensure(sel_n * 12));
-  unsigned long long *scode = sel.as<unsigned long long>();
-  uint32_t *scount = reinterpret_cast<uint32_t *>(scode + sel_n);
-  int64_t P = 0;
-  HIPC(hipEventRecord(ctx->ev4, st));
-  TRY(ppfk::select_and_sort(ctx, ccode, ccount, n_cand, meta_h[1], p->vote_count_threshold,
-                            s->cloud.n, m->cloud.n, scode, scount, &P));
-  HIPC(hipEventRecord(ctx->ev5, st));
-  r->P = P;
-  r->info.n_kept = P;
-  if (P > 0) {
-    TRY(r->codes.alloc(ctx, (size_t)P * 8));
-    TRY(r->counts.alloc(ctx, (size_t)P * 4));
-    TRY(r->T.alloc(ctx, (size_t)P * 64));
-    HIPC(hipMemcpyAsync(r->codes.ptr, scode, (size_t)P * 8, hipMemcpyDeviceToDevice, st));
-    HIPC(hipMemcpyAsync(r->counts.ptr, scount, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
-    // ---- poses
-    TRY(ppfk::compute_poses(ctx, r->codes.as<unsigned long long>(), P, m->frames.as<float>(),
-                            s->frames.as<float>(), s->rpdf, r->T.as<float>()));
-    HIPC(hipEventRecord(ctx->ev6, st));
-    // ---- clustering
-    if (p->cpu_clustering) {
-      std::vector<float> T((size_t)P * 16);
-      std::vector<uint32_t> counts((size_t)P);
-      HIPC(hipMemcpyAsync(T.data(), r->T.ptr, (size_t)P * 64, hipMemcpyDeviceToHost, st));
-      HIPC(hipMemcpyAsync(counts.data(), r->counts.ptr, (size_t)P * 4, hipMemcpyDeviceToHost, st));
-      HIPC(hipStreamSynchronize(st));
-      greedy_cluster(T, counts, m->d_dist, ppfm::D_ANGLE0, r->pose, &r->info.n_clusters);
-    } else {
-      TRY(r->vc.alloc(ctx, (size_t)P * 4));
-      TRY(r->trans.alloc(ctx, (size_t)P * 12));
-      TRY(r->quats.alloc(ctx, (size_t)P * 16));
-      int64_t mi = -1;
-      TRY(ppfk::cluster_spatial(ctx, r->T.as<float>(), r->codes.as<unsigned long long>(),
-                                r->counts.as<uint32_t>(), m->weights.as<float>(), P, m->d_dist,
-                                p->use_l1_norm != 0, p->use_averaged_clusters != 0, r->vc.as<float>(),
-                                r->trans.as<float>(), r->quats.as<float>(), &mi,
-                                &r->info.cluster_tests));
-      r->info.max_idx = mi;
-      float t3[3];
-      HIPC(hipMemcpyAsync(r->pose, r->T.as<float>() + mi * 16, 64, hipMemcpyDeviceToHost, st));
-      HIPC(hipMemcpyAsync(t3, r->trans.as<float>() + mi * 3, 12, hipMemcpyDeviceToHost, st));
-      HIPC(hipStreamSynchronize(st));
-      // src/cuda/ppf.cu:303-313: translation column from transformation_trans
-      r->pose[3] = t3[0];
-      r->pose[7] = t3[1];
-      r->pose[11] = t3[2];
-    }
-    r->info.status = PPF_OK;
-  } else {
-    r->info.status = PPF_ERR_NO_VOTES;
-  }
-  HIPC(hipEventRecord(ctx->ev3, st));
-  HIPC(hipEventSynchronize(ctx->ev3));
+  TRY(ppfi::finish_lookup(ctx, m, s, p, ccode, ccount, n_cand, meta_h[1], r.get()));
   float tm = 0.0f;
   HIPC(hipEventElapsedTime(&tm, ctx->ev2, ctx->ev3));
   r->info.total_ms = tm;
-  HIPC(hipEventElapsedTime(&tm, ctx->ev4, ctx->ev5));
-  r->info.select_ms = tm;
-  if (P > 0) {
-    HIPC(hipEventElapsedTime(&tm, ctx->ev5, ctx->ev6));
-    r->info.pose_ms = tm;
-    HIPC(hipEventElapsedTime(&tm, ctx->ev6, ctx->ev3));
-    r->info.cluster_ms = tm;
-  }
   *out = r.release();
   return PPF_OK;
 }

@@ -1,0 +1,221 @@
+// batch.hip -- batched registration: many (model, scene) pairs of uneven
+// sizes voted in one persistent launch over device-resident model hash
+// arrays, then finished (threshold/order, poses, clustering) concurrently on
+// the context's lanes.
+//
+// The reference registers one pair at a time and rebuilds the model for every
+// scene (src/cuda/ppf.cu:63-98); each pair's result here is bit-identical to
+// ppf_lookup on the same pair, because a job's votes depend only on its own
+// (model, scene) and its candidates are pruned against its own running
+// maximum.
+#include <algorithm>
+#include <atomic>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ppf_batch.h"
+
+using namespace ppfi;
+
+namespace {
+
+#define B_ARG(cond, msg)    \
+  do {                      \
+    if (!(cond)) {          \
+      set_error(msg);       \
+      return PPF_ERR_ARG;   \
+    }                       \
+  } while (0)
+
+}  // namespace
+
+namespace ppfi {
+
+int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vote_ms,
+                int *launches) {
+  *vote_ms = 0.0f;
+  *launches = 0;
+  const size_t nj = jobs.size();
+  if (nj == 0) return 0;
+  TBuf meta;
+  PPF_TRY(meta.alloc(ctx, nj * 16));
+  uint32_t *md = meta.as<uint32_t>();
+  std::vector<uint32_t> mh(nj * 4, 0u);
+  std::vector<size_t> todo(nj);
+  std::iota(todo.begin(), todo.end(), (size_t)0);
+  for (int attempt = 0; !todo.empty(); ++attempt) {
+    std::vector<ppfk::VoteJob> vj;
+    uint64_t items = 0;
+    int max_tm = 0;
+    int64_t max_ns = 0;
+    for (size_t j : todo) {
+      BatchJob &b = jobs[j];
+      if (b.cand.bytes < (size_t)b.cap * 12) PPF_TRY(b.cand.alloc(ctx, (size_t)b.cap * 12));
+      auto *cc = b.cand.as<unsigned long long>();
+      auto *cn = reinterpret_cast<uint32_t *>(cc + b.cap);
+      ppfk::VoteJob J = vote_job(b.m, b.s, b.thr, cc, cn, md + 4 * j, b.cap);
+      J.item_begin = items;
+      items += (uint64_t)b.s->R;
+      max_tm = std::max(max_tm, (int)b.m->TM);
+      max_ns = std::max(max_ns, b.s->cloud.n);
+      vj.push_back(J);
+      // preset: no candidates yet, running maximum from the previous pass
+      mh[4 * j] = 0u;
+      mh[4 * j + 2] = mh[4 * j + 3] = 0u;
+    }
+    PPF_HIP(hipMemcpyAsync(md, mh.data(), nj * 16, hipMemcpyHostToDevice, ctx->stream));
+    float ms = 0.0f;
+    PPF_TRY(ppfk::run_votes(ctx, vj.data(), (int)vj.size(), items, max_tm, max_ns, exact, &ms));
+    *vote_ms += ms;
+    ++*launches;
+    PPF_HIP(hipMemcpy(mh.data(), md, nj * 16, hipMemcpyDeviceToHost));
+    std::vector<size_t> again;
+    for (size_t j : todo) {
+      BatchJob &b = jobs[j];
+      b.n_cand = mh[4 * j];
+      b.max_count = mh[4 * j + 1];
+      b.votes = (int64_t)(((uint64_t)mh[4 * j + 3] << 32) | mh[4 * j + 2]);
+      if (b.n_cand <= b.cap) continue;
+      if (attempt >= 2 || b.n_cand > CAND_CAP_MAX) {
+        set_error("batch: kept-vote buffer overflow (" + std::to_string(b.n_cand) + " candidates)");
+        return PPF_ERR_NOMEM;
+      }
+      // the rerun starts from the final maximum, so it emits exactly the
+      // kept votes, of which there are at most n_cand
+      b.cap = std::max(b.cap, b.n_cand);
+      again.push_back(j);
+    }
+    todo.swap(again);
+  }
+  return 0;
+}
+
+int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,
+                 std::vector<std::unique_ptr<ppf_result_s>> &results) {
+  const size_t nj = jobs.size();
+  results.clear();
+  results.resize(nj);
+  std::vector<ppf_ctx> lanes;
+  PPF_TRY(ctx_lanes(ctx->root, std::max(1, std::min<int>(n_lanes, (int)nj)), lanes));
+  // largest candidate sets first
+  std::vector<size_t> order(nj);
+  std::iota(order.begin(), order.end(), (size_t)0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return jobs[a].n_cand > jobs[b].n_cand; });
+  std::atomic<size_t> next{0};
+  std::vector<int> rc(lanes.size(), 0);
+  std::vector<std::string> err(lanes.size());
+  auto work = [&](size_t li) {
+    ppf_ctx lane = lanes[li];
+    if (hipSetDevice(lane->device) != hipSuccess) {
+      rc[li] = PPF_ERR_HIP;
+      err[li] = "batch: hipSetDevice failed";
+      return;
+    }
+    for (;;) {
+      const size_t k = next.fetch_add(1);
+      if (k >= nj || rc[li]) break;
+      BatchJob &b = jobs[order[k]];
+      std::unique_ptr<ppf_result_s> r(new ppf_result_s());
+      r->info.lookups = (int64_t)b.s->R * (b.s->cloud.n - 1);
+      r->info.votes = b.votes;
+      r->info.max_count = b.max_count;
+      r->info.n_candidates = b.n_cand;
+      auto *cc = b.cand.as<unsigned long long>();
+      auto *cn = reinterpret_cast<uint32_t *>(cc + b.cap);
+      int e = finish_lookup(lane, b.m, b.s, p, cc, cn, b.n_cand, b.max_count, r.get());
+      if (e) {
+        rc[li] = e;
+        err[li] = std::string("batch job ") + std::to_string(b.index) + ": " + ppf_last_error();
+        break;
+      }
+      b.cand.release();
+      results[order[k]] = std::move(r);
+    }
+  };
+  if (lanes.size() == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (size_t li = 0; li < lanes.size(); ++li) th.emplace_back(work, li);
+    for (auto &t : th) t.join();
+  }
+  for (size_t li = 0; li < lanes.size(); ++li)
+    if (rc[li]) {
+      set_error(err[li]);
+      results.clear();
+      return rc[li];
+    }
+  return 0;
+}
+
+}  // namespace ppfi
+
+extern "C" {
+
+int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const ppf_scene *scenes,
+                     const ppf_lookup_params *params, ppf_result *results_out,
+                     ppf_batch_info_t *info) {
+  B_ARG(ctx && models && scenes && params && results_out, "ppf_batch_lookup: NULL argument");
+  B_ARG(n_jobs >= 1 && n_jobs <= PPF_MAX_BATCH_JOBS, "ppf_batch_lookup: n_jobs out of range");
+  ppf_ctx root = ctx->root;
+  for (int32_t j = 0; j < n_jobs; ++j) {
+    results_out[j] = nullptr;
+    B_ARG(models[j] && scenes[j], "ppf_batch_lookup: NULL model or scene");
+    B_ARG(models[j]->ctx == root && scenes[j]->ctx == root,
+          "ppf_batch_lookup: objects belong to another context");
+  }
+  PPF_HIP(hipSetDevice(root->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<size_t> order((size_t)n_jobs);
+  std::iota(order.begin(), order.end(), (size_t)0);
+  // longest jobs first in the item queue (work ~ R * n_s * n_m)
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    auto w = [&](size_t j) {
+      return (double)scenes[j]->R * (double)scenes[j]->cloud.n * (double)models[j]->cloud.n;
+    };
+    return w(a) > w(b);
+  });
+  std::vector<BatchJob> q;
+  q.reserve((size_t)n_jobs);
+  for (size_t j : order) {
+    BatchJob b;
+    b.m = models[j];
+    b.s = scenes[j];
+    b.thr = params->vote_count_threshold;
+    b.cap = batch_cap(scenes[j]->R);
+    b.index = (int64_t)j;
+    q.push_back(std::move(b));
+  }
+  float vote_ms = 0.0f;
+  int launches = 0;
+  PPF_TRY(batch_votes(root, q, params->exact_votes != 0, &vote_ms, &launches));
+  std::vector<std::unique_ptr<ppf_result_s>> res;
+  const auto t1 = std::chrono::steady_clock::now();
+  PPF_TRY(batch_finish(root, q, params, PPF_BATCH_LANES, res));
+  const auto t2 = std::chrono::steady_clock::now();
+  int64_t L = 0, V = 0;
+  for (size_t k = 0; k < order.size(); ++k) {
+    res[k]->info.vote_ms = vote_ms;
+    res[k]->info.total_ms = vote_ms + res[k]->info.select_ms + res[k]->info.pose_ms + res[k]->info.cluster_ms;
+    L += res[k]->info.lookups;
+    V += res[k]->info.votes;
+    results_out[order[k]] = res[k].release();
+  }
+  if (info) {
+    info->n_jobs = n_jobs;
+    info->lookups = L;
+    info->votes = V;
+    info->vote_launches = launches;
+    info->lanes = std::min<int>(PPF_BATCH_LANES, n_jobs);
+    info->vote_ms = vote_ms;
+    info->finish_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    info->total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+  }
+  return PPF_OK;
+}
+
+}  // extern "C"

@@ -330,7 +330,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
     if (item >= n_items) break;
     const VoteJob &J = jobs[s_job];
     const int32_t n_s = J.n_s;
-    const uint32_t r = (uint32_t)(item - J.item_begin);
+    const uint32_t r = (uint32_t)(item - J.item_begin) + (uint32_t)J.r_begin;
     const int32_t s_r = (int32_t)r * J.rpdf;
     const float *Fs = J.sframe + (size_t)r * 16;
     const float srx = gld(J.sx + s_r), sry = gld(J.sy + s_r), srz = gld(J.sz + s_r);

@@ -1,0 +1,43 @@
+// ppf_batch.h -- shared pieces of the batched and served forms.
+#pragma once
+
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "ppf_objects.h"
+
+// Lanes (streams with their own scratch and host thread) that finish batched
+// jobs concurrently.  Four matches the hardware queues HIP gives a process by
+// default.
+#define PPF_BATCH_LANES 4
+
+namespace ppfi {
+
+// One (model, scene) pair of a batch, with its candidate buffer and vote
+// statistics.
+struct BatchJob {
+  ppf_model m = nullptr;
+  ppf_scene s = nullptr;
+  float thr = 0.0f;
+  uint32_t cap = 0;      // candidate capacity
+  TBuf cand;             // (code u64 [cap], count u32 [cap])
+  uint32_t n_cand = 0;   // candidates emitted by the last pass
+  uint32_t max_count = 0;
+  int64_t votes = 0;
+  int64_t index = 0;     // position in the caller's list
+};
+
+inline uint32_t batch_cap(int64_t R) {
+  return (uint32_t)std::min<int64_t>(CAND_CAP_MAX,
+                                     std::max<int64_t>(1 << 20, R * (int64_t)CAND_PER_REF));
+}
+
+// Vote every job in one persistent launch (more only for jobs whose candidate
+// buffer overflowed, rerun with their final maximum preset).
+int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vote_ms, int *launches);
+// Finish every job on up to n_lanes lanes; results[k] belongs to jobs[k].
+int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,
+                 std::vector<std::unique_ptr<ppf_result_s>> &results);
+
+}  // namespace ppfi

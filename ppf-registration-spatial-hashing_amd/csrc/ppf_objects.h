@@ -19,7 +19,13 @@ constexpr int DEFAULT_TILE_POINTS = 512;
 constexpr int N_BINS = 31;
 constexpr int VOTE_BLOCK = 1024;
 
+// A context is one HIP stream with its own scratch buffers on one device.
+// The context a caller creates is a root; batched and served work also run
+// on lanes, contexts that share the root's device and device-memory account
+// (budget, bytes) but own their stream, events and scratch, so that several
+// host threads can drive independent stage chains at once.
 struct ppf_ctx_s {
+  ppf_ctx_s *root = this;
   int device = 0;
   hipStream_t stream = nullptr;
   uint64_t budget = 0;
@@ -30,7 +36,28 @@ struct ppf_ctx_s {
   ppfi::DevBuf tmp_a, tmp_b, tmp_c, tmp_d, hits, queue;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
+  std::vector<ppf_ctx_s *> lanes;  // root only
+
+  // Account n bytes against the root's budget (false: over budget).
+  bool reserve(uint64_t n) {
+    ppf_ctx_s *r = root;
+    uint64_t cur = r->bytes.load();
+    do {
+      if (r->budget && cur + n > r->budget) return false;
+    } while (!r->bytes.compare_exchange_weak(cur, cur + n));
+    return true;
+  }
+  void unreserve(uint64_t n) { root->bytes -= n; }
 };
+
+namespace ppfi {
+// Create the stream, events and limits of a context on `device`.
+int ctx_init(ppf_ctx_s *c, int device);
+// Release everything ctx_init and the scratch buffers hold.
+void ctx_release(ppf_ctx_s *c);
+// The root's lanes, created on first use (at most max_lanes).
+int ctx_lanes(ppf_ctx_s *root, int n, std::vector<ppf_ctx_s *> &out);
+}  // namespace ppfi
 
 // Tracked device allocation owned by a model/scene/result.
 struct TBuf {
@@ -47,6 +74,21 @@ struct TBuf {
   TBuf() = default;
   TBuf(const TBuf &) = delete;
   TBuf &operator=(const TBuf &) = delete;
+  TBuf(TBuf &&o) noexcept : ctx(o.ctx), ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  TBuf &operator=(TBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      ctx = o.ctx;
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
 };
 
 // Structure-of-arrays cloud: x y z nx ny nz |n|, each n floats.
@@ -105,7 +147,7 @@ namespace ppfk {
 struct VoteJob {
   const float *sx, *sy, *sz, *snx, *sny, *snz, *snn;
   const float *sframe;
-  int32_t n_s, rpdf, R, pad0;
+  int32_t n_s, rpdf, R, r_begin;  // items of this job: reference rows r_begin, r_begin+1, ...
   const uint32_t *htab;
   uint32_t hshift, hmask;
   const uint32_t *toff;
@@ -161,3 +203,20 @@ int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *o
 int debug_copy(ppf_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n);  // n multiple of 4
 
 }  // namespace ppfk
+
+namespace ppfi {
+// Candidate-buffer sizing shared by the plain and batched paths: (code, count)
+// pairs per reference point before growth on overflow.
+constexpr uint32_t CAND_PER_REF = 2048;
+constexpr uint32_t CAND_CAP_MIN = 1u << 23;  // plain path: at least 8M candidates
+constexpr uint32_t CAND_CAP_MAX = 1u << 28;
+// Vote job of one (model, scene) pair over all its reference points.  meta:
+// [0] candidates emitted, [1] running maximum, [2..3] vote total (u64).
+ppfk::VoteJob vote_job(ppf_model m, ppf_scene s, float thr, unsigned long long *cand_code,
+                       uint32_t *cand_count, uint32_t *meta, uint32_t cap);
+// Stages after voting (threshold/order, poses, clustering, final pose).
+int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *p,
+                  const unsigned long long *ccode, const uint32_t *ccount, uint32_t n_cand,
+                  uint32_t max_count, ppf_result_s *r);
+}  // namespace ppfi
+

@@ -61,6 +61,12 @@ class ResultInfo(C.Structure):
                 ("cluster_tests", C.c_int64)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("n_jobs", C.c_int64), ("lookups", C.c_int64), ("votes", C.c_int64),
+                ("vote_launches", C.c_int32), ("lanes", C.c_int32), ("vote_ms", C.c_double),
+                ("finish_ms", C.c_double), ("total_ms", C.c_double)]
+
+
 _lib = None
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -99,6 +105,8 @@ def lib():
     L.ppf_registration.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp, _vp,
                                    C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.c_int32, _vp,
                                    _f32p, _vp]
+    L.ppf_batch_lookup.argtypes = [_vp, C.c_int32, _vp, _vp, C.POINTER(LookupParams), _vp,
+                                   C.POINTER(BatchInfo)]
     L.ppf_debug_numerics.argtypes = [_vp, C.c_int32, _f32p, _vp, _f32p, C.c_int64]
     L.ppf_debug_primitive.argtypes = [_vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32]
     sizes = (C.c_uint32 * 3)()
@@ -361,6 +369,27 @@ class Model(_Owned):
     def getTransformations(self) -> np.ndarray:
         return self.result.transformations()
 
+
+
+def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_threshold: float,
+                 cpu_clustering: bool = False, use_l1_norm: bool = False,
+                 use_averaged_clusters: bool = False, exact_votes: bool = False,
+                 ctx: Optional[Context] = None):
+    """Batched form: result j of (models[j], scenes[j]), all jobs voted in one
+    persistent launch (ppf_batch_lookup).  Returns (results, info dict)."""
+    if len(models) != len(scenes) or not models:
+        raise ValueError("batch_lookup needs one scene per model and at least one job")
+    ctx = ctx or models[0].ctx
+    n = len(models)
+    mh = (C.c_void_p * n)(*[m.handle.value for m in models])
+    sh = (C.c_void_p * n)(*[s.handle.value for s in scenes])
+    prm = LookupParams(float(vote_count_threshold), int(cpu_clustering), int(use_l1_norm),
+                       int(use_averaged_clusters), int(exact_votes))
+    out = (C.c_void_p * n)()
+    info = BatchInfo()
+    _check(lib().ppf_batch_lookup(ctx.handle, n, mh, sh, C.byref(prm), out, C.byref(info)))
+    res = [LookupResult(_vp(out[j]), bool(cpu_clustering), ctx) for j in range(n)]
+    return res, {k: getattr(info, k) for k, _ in BatchInfo._fields_}
 
 
 def _close_all_contexts():

@@ -36,6 +36,10 @@ def test_argument_errors_are_status_codes():
     assert rc == ppf_amd.PPF_ERR_ARG
     rc = L.ppf_lookup(None, None, None, C.byref(h))
     assert rc == ppf_amd.PPF_ERR_ARG
+    out = (C.c_void_p * 1)()
+    prm = ppf_amd.LookupParams(0.3, 0, 0, 0, 0)
+    rc = L.ppf_batch_lookup(None, 1, None, None, C.byref(prm), out, None)
+    assert rc == ppf_amd.PPF_ERR_ARG
     # destroy on NULL is a no-op
     assert L.ppf_model_destroy(None) == 0 and L.ppf_result_destroy(None) == 0
 
