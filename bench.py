@@ -21,6 +21,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -92,6 +93,145 @@ def cpu_baseline(w, budget_s: float):
                 votes_per_s=tot_v / tot_s)
 
 
+def run_forms(ctx, plain_models, plain_scene, w, log_every=True):
+    """Batched and served forms beside the plain pipeline.
+
+    1. The dataset-like jobs (4 models x the 50,000-point scene) as one
+       ppf_batch_lookup call, against the plain step.
+    2. A request set shaped like the batched/served configurations but sized
+       to finish within seconds: 8 seeded superquadric models (1k-6k points)
+       built once, 48 scenes of 2k-20k points (log-uniform) each holding 1-4
+       of them.  Plain = each request's scene uploaded and looked up target by
+       target; batched = all (scene, model) jobs in one ppf_batch_lookup;
+       served = ppf_server with Poisson arrivals at 50, 200 and 2000 req/s.
+       The seed gives models with at least 0.45 unique keys per point, as in
+       the dataset-like models; flat-faced superquadrics with ~0.1 (huge
+       buckets) keep 10^7-10^8 votes per job and take minutes per job in
+       clustering, in the reference's algorithm as here.
+    """
+    out = {}
+    # ---- 1. dataset-like jobs batched
+    t = time.perf_counter()
+    res, info = ppf_amd.batch_lookup(plain_models, [plain_scene] * len(plain_models),
+                                     w.vote_count_threshold, cpu_clustering=w.cpu_clustering, ctx=ctx)
+    for r in res:
+        r.pose()
+        r.close()
+    dt = time.perf_counter() - t
+    out["dataset_like_batched"] = {"jobs": len(plain_models), "s": dt,
+                                   "registrations_per_s": len(plain_models) / dt,
+                                   "lookups_per_s": info["lookups"] / dt,
+                                   "vote_launches": info["vote_launches"]}
+    log(f"batched dataset-like: {dt:.2f}s")
+
+    # ---- 2. request set
+    rw = synth.batched(seed=11, n_models=8, n_scenes=48, model_range=(1000, 6000),
+                       scene_range=(2000, 20000), models_per_scene=(1, 4))
+    models = [ppf_amd.Model(m, d, rw.vote_count_threshold, ctx=ctx) for m, d in zip(rw.models, rw.d_dists)]
+    reqs = list(zip(rw.scenes, rw.scene_models))
+    n_targets = sum(len(ids) for _, ids in reqs)
+    desc = {"models": len(models), "model_points": [len(m[0]) for m in rw.models],
+            "requests": len(reqs), "targets": n_targets,
+            "scene_points": [len(s[0]) for s, _ in reqs], "ref_point_df": rw.rpdf,
+            "threshold": rw.vote_count_threshold, "clustering": "gpu"}
+
+    def plain_request(sc, ids):
+        t0 = time.perf_counter()
+        scene = ppf_amd.Scene(sc, 0.0, rw.rpdf, ctx=ctx)
+        L = 0
+        for mi in ids:
+            r = models[mi].ppf_lookup(scene)
+            r.pose()
+            L += r.info["lookups"]
+            r.close()
+        scene.close()
+        return time.perf_counter() - t0, L
+
+    plain_request(*reqs[0])  # warm-up
+    service, lookups = [], 0
+    t0 = time.perf_counter()
+    for sc, ids in reqs:
+        dt, L = plain_request(sc, ids)
+        service.append(dt)
+        lookups += L
+    plain_s = time.perf_counter() - t0
+    log(f"plain request set: {plain_s:.2f}s")
+
+    # batched: every (scene, model) job in one call (scenes resident)
+    scenes = [ppf_amd.Scene(sc, 0.0, rw.rpdf, ctx=ctx) for sc, _ in reqs]
+    jm = [models[mi] for _, ids in reqs for mi in ids]
+    js = [scenes[k] for k, (_, ids) in enumerate(reqs) for _ in ids]
+    res, _ = ppf_amd.batch_lookup(jm, js, rw.vote_count_threshold, ctx=ctx)  # warm-up
+    for r in res:
+        r.close()
+    t0 = time.perf_counter()
+    res, binfo = ppf_amd.batch_lookup(jm, js, rw.vote_count_threshold, ctx=ctx)
+    for r in res:
+        r.pose()
+        r.close()
+    batch_s = time.perf_counter() - t0
+    for sc in scenes:
+        sc.close()
+    log(f"batched request set: {batch_s:.2f}s")
+
+    # served: Poisson arrivals; the plain comparison is the same arrivals
+    # queued FIFO in front of the plain per-request service times above
+    served = []
+    srv = ppf_amd.Server(models, round_ms=20.0, ctx=ctx)
+    rng = np.random.default_rng(11)
+    for rate in (50.0, 200.0, 2000.0):
+        arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+        arr -= arr[0]
+        tickets = [None] * len(reqs)
+
+        def submitter():
+            t_start = time.perf_counter()
+            for k, (sc, ids) in enumerate(reqs):
+                d = t_start + arr[k] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[k] = srv.submit(sc, ids, rw.rpdf, rw.vote_count_threshold)
+
+        th = threading.Thread(target=submitter)
+        t0 = time.perf_counter()
+        th.start()
+        lat = []
+        for k in range(len(reqs)):
+            while tickets[k] is None:
+                time.sleep(0.0005)
+            _, st, info, _ = srv.wait(tickets[k], 600.0)
+            if info["rc"] != ppf_amd.PPF_OK:
+                raise RuntimeError(f"served request failed: {info}")
+            lat.append(info["latency_ms"])
+        th.join()
+        wall = time.perf_counter() - t0
+        fin, plat = 0.0, []
+        for a, sv in zip(arr, service):
+            fin = max(fin, a) + sv
+            plat.append((fin - a) * 1e3)
+        served.append({"rate_req_per_s": rate, "registrations_per_s": n_targets / wall,
+                       "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                       "plain_registrations_per_s": n_targets / fin,
+                       "plain_p50_ms": float(np.percentile(plat, 50)),
+                       "plain_p99_ms": float(np.percentile(plat, 99))})
+        log(f"served at {rate} req/s: {wall:.2f}s")
+    st = srv.stats()
+    srv.close()
+    for m in models:
+        m.close()
+    out["request_set"] = {
+        "workload": desc,
+        "plain": {"s": plain_s, "registrations_per_s": n_targets / plain_s, "lookups_per_s": lookups / plain_s},
+        "batched": {"s": batch_s, "registrations_per_s": n_targets / batch_s,
+                    "lookups_per_s": binfo["lookups"] / batch_s,
+                    "speedup_vs_plain": plain_s / batch_s, "vote_launches": binfo["vote_launches"]},
+        "served": served,
+        "server": {"rounds": st["rounds"], "max_requests_in_round": st["max_requests_in_round"],
+                   "peak_inflight_bytes": st["peak_inflight_bytes"]},
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +240,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-votes", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batched and served forms")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     args = ap.parse_args()
@@ -198,6 +339,8 @@ def main():
             "traffic_source": traffic_src,
         },
     }
+    if not args.no_forms:
+        out["forms"] = run_forms(ctx, models, scene, w)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)

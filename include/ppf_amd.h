@@ -30,6 +30,8 @@ extern "C" {
 #define PPF_ERR_NOMEM 3      /* device allocation or HBM budget exhausted */
 #define PPF_ERR_NO_VOTES 4   /* no scene pair matched the model: no pose */
 #define PPF_ERR_STATE 5      /* wrong object state (e.g. result not computed) */
+#define PPF_ERR_TIMEOUT 6    /* ppf_server_wait: the request is not done yet */
+#define PPF_ERR_CANCELLED 7  /* the request was cancelled or the server stopped */
 
 /* The reference encodes votes as [s_r:32 | m_r:26 | alpha:6] (include/model.h:61-63)
  * and indexes model pairs m_r*n_m + m_i; this build keeps pair indices in 32 bits. */
@@ -109,6 +111,8 @@ typedef struct {
   int32_t use_l1_norm;         /* 1: skip the translation test (kernel.cu:740-746) */
   int32_t use_averaged_clusters;
   int32_t exact_votes;         /* 1: evaluate every alpha bin with the full float path */
+  uint32_t candidate_capacity; /* initial (code, count) buffer capacity; 0 = automatic.  A
+                                  smaller buffer costs reruns of the vote pass, never results */
 } ppf_lookup_params;
 
 int ppf_lookup(ppf_model model, ppf_scene scene, const ppf_lookup_params *params,
@@ -184,9 +188,67 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
                      const ppf_lookup_params *params, ppf_result *results_out,
                      ppf_batch_info_t *info);
 
-/* sizeof(ppf_model_info_t), sizeof(ppf_lookup_params), sizeof(ppf_result_info_t):
- * lets a binding check that it was written against the same header. */
-void ppf_abi_sizes(uint32_t sizes_out[3]);
+/* ------------------------------------------------------------------ served
+ * A request loop over resident models (no counterpart in the reference, whose
+ * driver registers one pair at a time, src/cuda/ppf.cu:63-98).  A request is
+ * one scene and a list of target model ids; the server copies it at submit,
+ * admits it when its device-memory estimate fits the budget, uploads the
+ * scene, and votes all admitted requests together in rounds (continuous
+ * batching: requests arriving meanwhile join at the next round).  Finished
+ * votes go through threshold/order, poses and clustering on the server's
+ * lanes while the next rounds vote.  Each target's result is bit-identical to
+ * ppf_lookup of the same (model, scene, params).  Models must outlive the
+ * server.  All functions may be called from any thread. */
+typedef struct ppf_server_s *ppf_server;
+typedef struct {
+  uint64_t hbm_budget_bytes; /* device memory for requests in flight; 0 = 80% of free memory */
+  int32_t max_active;        /* requests in flight at once; 0 = 64 */
+  int32_t lanes;             /* finishing lanes (host threads + streams); 0 = 4 */
+  double round_ms;           /* target duration of one vote round; 0 = 50 */
+} ppf_server_config;
+typedef struct {
+  int32_t status;      /* PPF_OK, or the first failing target's status */
+  int32_t n_targets;
+  int32_t vote_rounds; /* rounds in which this request was voted */
+  int32_t pad;
+  double queue_ms;     /* submit -> admission */
+  double latency_ms;   /* submit -> all targets done */
+  uint64_t device_bytes; /* admission estimate */
+} ppf_request_info_t;
+typedef struct {
+  int64_t submitted, completed, failed, cancelled, rejected;
+  int64_t rounds, rows_voted, queued, active;
+  uint64_t inflight_bytes, peak_inflight_bytes, budget_bytes;
+  int64_t max_requests_in_round; /* most distinct requests voted in one round */
+  double vote_ms;
+} ppf_server_stats_t;
+
+int ppf_server_create(ppf_ctx ctx, int32_t n_models, const ppf_model *models,
+                      const ppf_server_config *cfg, ppf_server *out);
+/* Stops admitting, cancels queued requests, completes those in flight. */
+int ppf_server_destroy(ppf_server server);
+/* points/normals: float[n*3]; model_ids index the server's model list. */
+int ppf_server_submit(ppf_server server, const float *points, const float *normals, int64_t n,
+                      uint32_t ref_point_downsample_factor, int32_t n_targets,
+                      const int32_t *model_ids, const ppf_lookup_params *params, uint64_t *ticket);
+/* Wait up to timeout_s (< 0: no limit).  PPF_ERR_TIMEOUT leaves the request
+ * pending; any other return consumes the ticket.  poses_out
+ * [n_targets][16], status_out [n_targets], results_out [n_targets] (full
+ * results, owned by the caller) and info may each be NULL.  Returns PPF_OK
+ * when the request ran (per-target statuses in status_out), else its error. */
+int ppf_server_wait(ppf_server server, uint64_t ticket, double timeout_s, float *poses_out,
+                    int32_t *status_out, ppf_result *results_out, ppf_request_info_t *info);
+int ppf_server_poll(ppf_server server, uint64_t ticket, int32_t *done);
+/* Cancel a request that is still queued (PPF_ERR_STATE once admitted). */
+int ppf_server_cancel(ppf_server server, uint64_t ticket);
+int ppf_server_stats(ppf_server server, ppf_server_stats_t *stats);
+
+/* Sizes of the ABI structs, in this order: ppf_model_info_t,
+ * ppf_lookup_params, ppf_result_info_t, ppf_batch_info_t, ppf_server_config,
+ * ppf_request_info_t, ppf_server_stats_t.  Lets a binding check that it was
+ * written against the same header. */
+#define PPF_ABI_N_STRUCTS 7
+void ppf_abi_sizes(uint32_t sizes_out[PPF_ABI_N_STRUCTS]);
 
 /* ------------------------------------------------------------- diagnostics
  * Evaluate one numerics primitive on the device for n inputs (used to check

@@ -303,6 +303,23 @@ void greedy_cluster(const std::vector<float> &T, const std::vector<uint32_t> &co
 }  // namespace
 
 namespace ppfi {
+int scene_create(ppf_ctx lane, const float *points, const float *normals, int64_t n, uint32_t rpdf,
+                 ppf_scene *out) {
+  auto *s = new ppf_scene_s();
+  s->ctx = lane->root;
+  s->rpdf = rpdf;
+  s->R = (n + rpdf - 1) / rpdf;
+  int rc = upload_cloud(lane, points, normals, n, s->cloud, &s->frames, rpdf, s->R);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return PPF_OK;
+}
+}  // namespace ppfi
+
+namespace ppfi {
 ppfk::VoteJob vote_job(ppf_model m, ppf_scene s, float thr, unsigned long long *cand_code,
                        uint32_t *cand_count, uint32_t *meta, uint32_t cap) {
   ppfk::VoteJob J{};
@@ -430,10 +447,14 @@ extern "C" {
 
 const char *ppf_last_error(void) { return g_err.c_str(); }
 
-void ppf_abi_sizes(uint32_t sizes_out[3]) {
+void ppf_abi_sizes(uint32_t sizes_out[PPF_ABI_N_STRUCTS]) {
   sizes_out[0] = (uint32_t)sizeof(ppf_model_info_t);
   sizes_out[1] = (uint32_t)sizeof(ppf_lookup_params);
   sizes_out[2] = (uint32_t)sizeof(ppf_result_info_t);
+  sizes_out[3] = (uint32_t)sizeof(ppf_batch_info_t);
+  sizes_out[4] = (uint32_t)sizeof(ppf_server_config);
+  sizes_out[5] = (uint32_t)sizeof(ppf_request_info_t);
+  sizes_out[6] = (uint32_t)sizeof(ppf_server_stats_t);
 }
 
 int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
@@ -579,17 +600,7 @@ int ppf_scene_upload(ppf_ctx ctx, const float *points, const float *normals, int
   ARG_CHECK(n >= 2 && n < ((int64_t)1 << 31), "ppf_scene_upload: need 2 <= n < 2^31 points");
   ARG_CHECK(rpdf >= 1, "ppf_scene_upload: ref_point_downsample_factor must be >= 1");
   TRY(set_device(ctx));
-  auto *s = new ppf_scene_s();
-  s->ctx = ctx;
-  s->rpdf = rpdf;
-  s->R = (n + rpdf - 1) / rpdf;
-  int rc = upload_cloud(ctx, points, normals, n, s->cloud, &s->frames, rpdf, s->R);
-  if (rc) {
-    delete s;
-    return rc;
-  }
-  *out = s;
-  return PPF_OK;
+  return ppfi::scene_create(ctx, points, normals, n, rpdf, out);
 }
 
 int ppf_scene_destroy(ppf_scene scene) {
@@ -636,15 +647,20 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   ppfi::DevBuf cand;
   // candidates per reference point are bounded by the pruning; start at
   // 2048 per reference point (at least 8M), grow on overflow
-  uint32_t cap = (uint32_t)std::min<int64_t>(CAND_CAP_MAX, std::max<int64_t>(CAND_CAP_MIN, s->R * (int64_t)CAND_PER_REF));
+  uint32_t cap = p->candidate_capacity
+                     ? round_cap(p->candidate_capacity)
+                     : round_cap((uint64_t)std::max<int64_t>(CAND_CAP_MIN, s->R * (int64_t)CAND_PER_REF));
   uint32_t meta_h[2] = {0, 0};
   int64_t votes = 0;
   float vote_ms = 0.0f;
   for (int attempt = 0;; ++attempt) {
-    TRY(cand.ensure((size_t)cap * 12 + 64));
-    unsigned long long *cc = cand.as<unsigned long long>();
+    // [meta: 16 B, 256 B reserved][codes: cap x u64][counts: cap x u32].  The
+    // counters come first so that the 64-bit vote total stays 8-byte aligned
+    // for any cap (a misaligned 64-bit atomic faults).
+    TRY(cand.ensure(CAND_HEADER + (size_t)cap * 12));
+    uint32_t *meta = cand.as<uint32_t>();  // [0] candidates emitted, [1] running maximum, [2..3] votes
+    unsigned long long *cc = reinterpret_cast<unsigned long long *>(cand.as<char>() + CAND_HEADER);
     uint32_t *cn = reinterpret_cast<uint32_t *>(cc + cap);
-    uint32_t *meta = cn + cap;  // [0] candidates emitted, [1] running maximum, [2..3] votes
     const uint32_t preset[4] = {0u, meta_h[1], 0u, 0u};
     HIPC(hipMemcpyAsync(meta, preset, 16, hipMemcpyHostToDevice, st));
     ppfk::VoteJob J = ppfi::vote_job(m, s, p->vote_count_threshold, cc, cn, meta, cap);
@@ -661,7 +677,7 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
       set_error("ppf_lookup: kept-vote buffer overflow (" + std::to_string(meta_h[0]) + " candidates)");
       return PPF_ERR_NOMEM;
     }
-    if (attempt >= 1) cap = meta_h[0];
+    if (attempt >= 1) cap = round_cap(meta_h[0]);
   }
   const uint32_t n_cand = meta_h[0];
   r->info.lookups = (int64_t)s->R * (s->cloud.n - 1);
@@ -669,7 +685,8 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   r->info.max_count = meta_h[1];
   r->info.n_candidates = n_cand;
   r->info.vote_ms = vote_ms;
-  const unsigned long long *ccode = cand.as<unsigned long long>();
+  const unsigned long long *ccode =
+      reinterpret_cast<const unsigned long long *>(cand.as<char>() + CAND_HEADER);
   const uint32_t *ccount = reinterpret_cast<const uint32_t *>(ccode + cap);
   TRY(ppfi::finish_lookup(ctx, m, s, p, ccode, ccount, n_cand, meta_h[1], r.get()));
   float tm = 0.0f;

@@ -85,7 +85,7 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
       }
       // the rerun starts from the final maximum, so it emits exactly the
       // kept votes, of which there are at most n_cand
-      b.cap = std::max(b.cap, b.n_cand);
+      b.cap = std::max(b.cap, round_cap(b.n_cand));
       again.push_back(j);
     }
     todo.swap(again);
@@ -186,7 +186,7 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
     b.m = models[j];
     b.s = scenes[j];
     b.thr = params->vote_count_threshold;
-    b.cap = batch_cap(scenes[j]->R);
+    b.cap = params->candidate_capacity ? round_cap(params->candidate_capacity) : batch_cap(scenes[j]->R);
     b.index = (int64_t)j;
     q.push_back(std::move(b));
   }

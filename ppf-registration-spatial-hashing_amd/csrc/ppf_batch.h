@@ -29,8 +29,7 @@ struct BatchJob {
 };
 
 inline uint32_t batch_cap(int64_t R) {
-  return (uint32_t)std::min<int64_t>(CAND_CAP_MAX,
-                                     std::max<int64_t>(1 << 20, R * (int64_t)CAND_PER_REF));
+  return round_cap((uint64_t)std::max<int64_t>(1 << 20, R * (int64_t)CAND_PER_REF));
 }
 
 // Vote every job in one persistent launch (more only for jobs whose candidate

@@ -210,10 +210,19 @@ namespace ppfi {
 constexpr uint32_t CAND_PER_REF = 2048;
 constexpr uint32_t CAND_CAP_MIN = 1u << 23;  // plain path: at least 8M candidates
 constexpr uint32_t CAND_CAP_MAX = 1u << 28;
+constexpr size_t CAND_HEADER = 256;  // counters ahead of the (code, count) arrays
+// Candidate capacities are multiples of 256 (keeps every array aligned).
+inline uint32_t round_cap(uint64_t c) {
+  const uint64_t r = (c + 255u) / 256u * 256u;
+  return (uint32_t)(r < CAND_CAP_MAX ? r : CAND_CAP_MAX);
+}
 // Vote job of one (model, scene) pair over all its reference points.  meta:
 // [0] candidates emitted, [1] running maximum, [2..3] vote total (u64).
 ppfk::VoteJob vote_job(ppf_model m, ppf_scene s, float thr, unsigned long long *cand_code,
                        uint32_t *cand_count, uint32_t *meta, uint32_t cap);
+// Upload a scene through `lane` (its stream and scratch); owned by lane->root.
+int scene_create(ppf_ctx lane, const float *points, const float *normals, int64_t n, uint32_t rpdf,
+                 ppf_scene *out);
 // Stages after voting (threshold/order, poses, clustering, final pose).
 int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *p,
                   const unsigned long long *ccode, const uint32_t *ccount, uint32_t n_cand,

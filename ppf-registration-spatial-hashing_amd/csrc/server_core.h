@@ -1,0 +1,517 @@
+// server_core.h -- request scheduler of the served form, independent of the
+// device.  Pure C++17 (threads, mutexes): the HIP backend (server.hip)
+// implements the Backend interface; a host-only test double drives the same
+// scheduler in tests.
+//
+// Continuous batching: a request (one scene, several target models) is
+// admitted when its device-memory estimate fits the free part of the budget.
+// Its scene is uploaded and each target becomes a job.  The scheduler thread
+// runs vote rounds: one launch over slices of reference rows of every voting
+// job, sized so that a round takes about `round_ms`, each job getting an equal
+// share of the round's work.  Requests that arrive meanwhile join at the next
+// round.  A job whose rows are all voted moves to the lanes: host threads
+// with their own streams that run threshold/order, poses and clustering while
+// the next rounds vote.  When all of a request's jobs are finished its device
+// memory returns to the budget.
+//
+// The reference has no served form: it registers one (scene, model) pair at
+// a time (src/cuda/ppf.cu:63-98).
+#pragma once
+
+#include <array>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ppf_amd.h"
+
+namespace ppfs {
+
+using Clock = std::chrono::steady_clock;
+
+enum class State { Queued, Voting, Finishing, Done, Failed, Cancelled };
+
+struct Job {
+  int32_t target = 0;    // position in the request's target list
+  int32_t model_id = 0;
+  int64_t rows = 0;      // reference rows of the scene
+  int64_t next_row = 0;  // first row not yet voted in the current pass
+  double row_units = 1;  // work estimate of one row (n_s * n_m)
+  int passes = 0;        // vote passes started (more than one after an overflow)
+  int status = PPF_OK;
+  std::string error;
+  std::shared_ptr<void> dev;  // backend state of the job
+};
+
+struct Request {
+  uint64_t ticket = 0;
+  int64_t n_points = 0;
+  uint32_t rpdf = 1;
+  int64_t rows = 0;
+  std::vector<float> points, normals;  // host copies, dropped after upload
+  std::vector<int32_t> model_ids;
+  ppf_lookup_params params{};
+  std::vector<Job> jobs;
+  State state = State::Queued;
+  int status = PPF_OK;
+  std::string error;
+  uint64_t bytes = 0;  // device-memory estimate used for admission
+  int jobs_left = 0;
+  int rounds = 0;
+  Clock::time_point t_submit, t_admit, t_done;
+  std::vector<std::array<float, 16>> poses;
+  std::shared_ptr<void> dev;                // backend state of the request (scene)
+  std::vector<std::shared_ptr<void>> results;  // backend results per target
+};
+
+struct Slice {
+  Request *req;
+  Job *job;
+  int64_t row0, rows;
+};
+
+class Backend {
+ public:
+  virtual ~Backend() = default;
+  // Device bytes the request holds while in flight.
+  virtual uint64_t request_bytes(const Request &r) = 0;
+  // Scheduler thread.  Upload the scene; start a vote pass of job j.
+  virtual int upload(Request &r, std::string &err) = 0;
+  virtual int start_pass(Request &r, Job &j, std::string &err) = 0;
+  // Scheduler thread: one vote launch over the slices; returns when done.
+  virtual int vote_round(std::vector<Slice> &slices, double *ms, std::string &err) = 0;
+  // Scheduler thread, after the last row of a pass: *rerun asks for another
+  // pass (candidate overflow); start_pass is called again for it.
+  virtual int pass_done(Request &r, Job &j, bool *rerun, std::string &err) = 0;
+  // Lane thread `lane`: stages after voting; fills r.poses[j.target] and
+  // r.results[j.target], and the job's outcome (PPF_OK or PPF_ERR_NO_VOTES)
+  // in *job_status.
+  virtual int finish(int lane, Request &r, Job &j, int *job_status, std::string &err) = 0;
+  // Any thread: free the request's device state (results stay).
+  virtual void release(Request &r) = 0;
+};
+
+struct Config {
+  uint64_t budget = 0;          // device bytes for requests in flight
+  int max_active = 64;          // requests in flight at once
+  int lanes = 4;                // finishing lanes
+  double round_ms = 50.0;       // target duration of one vote round
+  int64_t min_round_rows = 1;   // at least this many rows per round (if available)
+  int64_t max_round_rows = 1 << 20;
+};
+
+struct Stats {
+  int64_t submitted = 0, completed = 0, failed = 0, cancelled = 0, rejected = 0;
+  int64_t rounds = 0, rows_voted = 0, queued = 0, active = 0;
+  uint64_t inflight_bytes = 0, peak_inflight_bytes = 0;
+  int64_t max_requests_in_round = 0;  // most distinct requests voted in one round
+  double vote_ms = 0;
+};
+
+class Scheduler {
+ public:
+  Scheduler(Backend *backend, const Config &cfg) : be_(backend), cfg_(cfg) {
+    if (cfg_.lanes < 1) cfg_.lanes = 1;
+    if (cfg_.max_active < 1) cfg_.max_active = 1;
+    sched_ = std::thread([this] { loop(); });
+    for (int l = 0; l < cfg_.lanes; ++l) lanes_.emplace_back([this, l] { lane_loop(l); });
+  }
+
+  ~Scheduler() { shutdown(); }
+
+  // Stop: cancel queued requests, let requests in flight complete, join.
+  void shutdown() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (stopping_) return;
+      stopping_ = true;
+      for (Request *r : queue_) finalize_locked(*r, State::Cancelled, PPF_ERR_CANCELLED, "server stopped");
+      queue_.clear();
+    }
+    sched_cv_.notify_all();
+    if (sched_.joinable()) sched_.join();
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      lanes_stop_ = true;
+    }
+    lane_cv_.notify_all();
+    for (auto &t : lanes_)
+      if (t.joinable()) t.join();
+  }
+
+  // Takes ownership; returns the ticket (> 0).
+  uint64_t submit(std::unique_ptr<Request> r) {
+    std::lock_guard<std::mutex> g(mu_);
+    r->ticket = ++next_ticket_;
+    r->t_submit = Clock::now();
+    r->state = State::Queued;
+    r->jobs_left = (int)r->model_ids.size();
+    r->poses.assign(r->model_ids.size(), std::array<float, 16>{});
+    r->results.assign(r->model_ids.size(), nullptr);
+    r->bytes = be_->request_bytes(*r);
+    Request *p = r.get();
+    reqs_[p->ticket] = std::move(r);
+    ++st_.submitted;
+    if (stopping_) {
+      finalize_locked(*p, State::Cancelled, PPF_ERR_CANCELLED, "server stopped");
+    } else {
+      queue_.push_back(p);
+      sched_cv_.notify_all();
+    }
+    return p->ticket;
+  }
+
+  // Wait for the request; on return (any status but PPF_ERR_TIMEOUT) the
+  // record is removed and handed to *out.
+  int wait(uint64_t ticket, double timeout_s, std::unique_ptr<Request> *out) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = reqs_.find(ticket);
+    if (it == reqs_.end()) return PPF_ERR_ARG;
+    Request *r = it->second.get();
+    auto done = [&] { return is_final(r->state); };
+    if (timeout_s < 0) {
+      done_cv_.wait(lk, done);
+    } else {
+      // a system_clock deadline waits in pthread_cond_timedwait, which thread
+      // checkers understand (steady_clock waits go through clockwait)
+      const auto until = std::chrono::system_clock::now() +
+                         std::chrono::duration_cast<std::chrono::system_clock::duration>(
+                             std::chrono::duration<double>(timeout_s));
+      if (!done_cv_.wait_until(lk, until, done)) return PPF_ERR_TIMEOUT;
+    }
+    *out = std::move(it->second);
+    reqs_.erase(it);
+    return PPF_OK;
+  }
+
+  // 1 if the request has completed (any final state), 0 if not.
+  int poll(uint64_t ticket, int *done) {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = reqs_.find(ticket);
+    if (it == reqs_.end()) return PPF_ERR_ARG;
+    *done = is_final(it->second->state) ? 1 : 0;
+    return PPF_OK;
+  }
+
+  // Cancel a request that has not been admitted yet.
+  int cancel(uint64_t ticket) {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = reqs_.find(ticket);
+    if (it == reqs_.end()) return PPF_ERR_ARG;
+    Request *r = it->second.get();
+    if (r->state != State::Queued) return PPF_ERR_STATE;
+    for (auto q = queue_.begin(); q != queue_.end(); ++q)
+      if (*q == r) {
+        queue_.erase(q);
+        break;
+      }
+    finalize_locked(*r, State::Cancelled, PPF_ERR_CANCELLED, "cancelled");
+    return PPF_OK;
+  }
+
+  Stats stats() {
+    std::lock_guard<std::mutex> g(mu_);
+    Stats s = st_;
+    s.queued = (int64_t)queue_.size();
+    s.active = (int64_t)active_.size();
+    s.inflight_bytes = inflight_;
+    return s;
+  }
+
+ private:
+  static bool is_final(State s) {
+    return s == State::Done || s == State::Failed || s == State::Cancelled;
+  }
+
+  void finalize_locked(Request &r, State s, int status, const std::string &err) {
+    r.state = s;
+    if (status != PPF_OK && r.status == PPF_OK) {
+      r.status = status;
+      r.error = err;
+    }
+    r.t_done = Clock::now();
+    if (s == State::Done) ++st_.completed;
+    if (s == State::Failed) ++st_.failed;
+    if (s == State::Cancelled) ++st_.cancelled;
+    r.points.clear();
+    r.points.shrink_to_fit();
+    r.normals.clear();
+    r.normals.shrink_to_fit();
+    done_cv_.notify_all();
+  }
+
+  // Return the request's memory and drop it from the active set.
+  void retire_locked(Request &r) {
+    inflight_ -= r.bytes;
+    for (auto a = active_.begin(); a != active_.end(); ++a)
+      if (*a == &r) {
+        active_.erase(a);
+        break;
+      }
+    sched_cv_.notify_all();
+  }
+
+  // A request failed before all of its jobs reached the lanes.
+  void fail_inflight_locked(std::unique_lock<std::mutex> &lk, Request &r, int status,
+                            const std::string &err) {
+    for (Job &j : r.jobs)
+      if (j.status == PPF_OK) {
+        j.status = status;
+        j.error = err;
+      }
+    r.status = status;
+    r.error = err;
+    // jobs already with the lanes finish there; the rest are dropped now
+    int queued = 0;
+    for (auto &f : finq_)
+      if (f.first == &r) ++queued;
+    const int with_lanes = queued + busy_jobs_of(r);
+    r.jobs_left = with_lanes;
+    if (with_lanes == 0) {
+      lk.unlock();
+      be_->release(r);
+      lk.lock();
+      retire_locked(r);
+      finalize_locked(r, State::Failed, status, err);
+    }
+  }
+
+  int busy_jobs_of(const Request &r) {
+    int n = 0;
+    for (const Request *b : busy_) n += (b == &r);
+    return n;
+  }
+
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu_);
+    double unit_ms = 0.0;  // measured device ms per work unit (row * n_s * n_m)
+    for (;;) {
+      // ---- admission (FIFO; a request never fitting the budget is rejected)
+      while (!queue_.empty()) {
+        Request *r = queue_.front();
+        if (cfg_.budget && r->bytes > cfg_.budget) {
+          queue_.pop_front();
+          ++st_.rejected;
+          finalize_locked(*r, State::Failed, PPF_ERR_NOMEM,
+                          "request needs " + std::to_string(r->bytes) + " device bytes, budget is " +
+                              std::to_string(cfg_.budget));
+          continue;
+        }
+        if ((cfg_.budget && inflight_ + r->bytes > cfg_.budget) ||
+            (int)active_.size() >= cfg_.max_active)
+          break;
+        queue_.pop_front();
+        inflight_ += r->bytes;
+        st_.peak_inflight_bytes = std::max(st_.peak_inflight_bytes, inflight_);
+        active_.push_back(r);
+        r->state = State::Voting;
+        r->t_admit = Clock::now();
+        lk.unlock();
+        std::string err;
+        int rc = be_->upload(*r, err);
+        for (size_t k = 0; rc == PPF_OK && k < r->jobs.size(); ++k) {
+          rc = be_->start_pass(*r, r->jobs[k], err);
+          r->jobs[k].passes = 1;
+        }
+        lk.lock();
+        if (rc != PPF_OK) fail_inflight_locked(lk, *r, rc, err);
+      }
+      // ---- the voting jobs
+      std::vector<std::pair<Request *, Job *>> voting;
+      for (Request *r : active_)
+        if (r->state == State::Voting && r->status == PPF_OK)
+          for (Job &j : r->jobs)
+            if (j.status == PPF_OK && j.next_row < j.rows) voting.emplace_back(r, &j);
+      if (voting.empty()) {
+        if (stopping_ && queue_.empty() && active_.empty()) break;
+        sched_cv_.wait(lk);
+        continue;
+      }
+      // ---- plan the round: equal work share per job, rows >= 1
+      double budget_units = 0.0;
+      if (unit_ms > 0.0) budget_units = cfg_.round_ms / unit_ms;
+      std::vector<Slice> slices;
+      int64_t total_rows = 0;
+      for (auto &v : voting) {
+        Job &j = *v.second;
+        const int64_t left = j.rows - j.next_row;
+        int64_t rows = left;
+        if (budget_units > 0.0) {
+          const double share = budget_units / (double)voting.size();
+          rows = std::max<int64_t>(1, (int64_t)(share / std::max(1.0, j.row_units)));
+        } else {
+          rows = std::min<int64_t>(left, std::max<int64_t>(1, cfg_.min_round_rows / (int64_t)voting.size()));
+        }
+        rows = std::min(rows, left);
+        slices.push_back({v.first, &j, j.next_row, rows});
+        total_rows += rows;
+      }
+      // fill the device: grow slices toward min_round_rows, cap at max_round_rows
+      for (size_t pass = 0; total_rows < cfg_.min_round_rows && pass < 4; ++pass) {
+        bool grew = false;
+        for (Slice &s : slices) {
+          const int64_t left = s.job->rows - s.row0 - s.rows;
+          const int64_t want = (cfg_.min_round_rows - total_rows + (int64_t)slices.size() - 1) /
+                               (int64_t)slices.size();
+          const int64_t add = std::min(left, want);
+          if (add > 0) {
+            s.rows += add;
+            total_rows += add;
+            grew = true;
+          }
+        }
+        if (!grew) break;
+      }
+      while (total_rows > cfg_.max_round_rows) {
+        bool shrunk = false;
+        for (Slice &s : slices)
+          if (s.rows > 1 && total_rows > cfg_.max_round_rows) {
+            const int64_t cut = std::min(s.rows - 1, total_rows - cfg_.max_round_rows);
+            s.rows -= cut;
+            total_rows -= cut;
+            shrunk = true;
+          }
+        if (!shrunk) break;
+      }
+      double units = 0.0;
+      int64_t distinct = 0;
+      {
+        const Request *last = nullptr;
+        for (Slice &s : slices) {
+          units += (double)s.rows * s.job->row_units;
+          if (s.req != last) ++distinct, last = s.req;
+        }
+      }
+      lk.unlock();
+      double ms = 0.0;
+      std::string err;
+      int rc = be_->vote_round(slices, &ms, err);
+      lk.lock();
+      ++st_.rounds;
+      st_.vote_ms += ms;
+      st_.rows_voted += total_rows;
+      st_.max_requests_in_round = std::max(st_.max_requests_in_round, distinct);
+      if (rc != PPF_OK) {
+        std::vector<Request *> hit;
+        for (Slice &s : slices)
+          if (hit.empty() || hit.back() != s.req) hit.push_back(s.req);
+        for (Request *r : hit)
+          if (r->status == PPF_OK) fail_inflight_locked(lk, *r, rc, err);
+        continue;
+      }
+      if (units > 0.0 && ms > 0.0) {
+        const double u = ms / units;
+        unit_ms = unit_ms > 0.0 ? 0.5 * unit_ms + 0.5 * u : u;
+      }
+      {
+        const Request *last = nullptr;
+        for (Slice &s : slices) {
+          s.job->next_row += s.rows;
+          if (s.req != last) ++s.req->rounds, last = s.req;
+        }
+      }
+      // ---- jobs whose pass ended
+      for (Slice &s : slices) {
+        Request &r = *s.req;
+        Job &j = *s.job;
+        if (j.next_row < j.rows || r.status != PPF_OK) continue;
+        bool rerun = false;
+        lk.unlock();
+        std::string e2;
+        int rc2 = be_->pass_done(r, j, &rerun, e2);
+        if (rc2 == PPF_OK && rerun) {
+          j.next_row = 0;
+          ++j.passes;
+          rc2 = be_->start_pass(r, j, e2);
+        }
+        lk.lock();
+        if (rc2 != PPF_OK) {
+          fail_inflight_locked(lk, r, rc2, e2);
+          continue;
+        }
+        if (!rerun) {
+          finq_.emplace_back(&r, &j);
+          lane_cv_.notify_one();
+        }
+      }
+      // a request whose jobs have all left voting is finishing
+      for (Request *r : active_)
+        if (r->state == State::Voting && r->status == PPF_OK) {
+          bool all = true;
+          for (Job &j : r->jobs) all = all && j.next_row >= j.rows;
+          if (all) r->state = State::Finishing;
+        }
+    }
+  }
+
+  void lane_loop(int lane) {
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      lane_cv_.wait(lk, [&] { return !finq_.empty() || lanes_stop_; });
+      if (finq_.empty()) {
+        if (lanes_stop_) break;
+        continue;
+      }
+      auto f = finq_.front();
+      finq_.pop_front();
+      Request &r = *f.first;
+      Job &j = *f.second;
+      busy_.push_back(&r);
+      lk.unlock();
+      std::string err;
+      int rc = PPF_OK, js = PPF_OK;
+      if (j.status == PPF_OK) rc = be_->finish(lane, r, j, &js, err);
+      lk.lock();
+      if (rc == PPF_OK && j.status == PPF_OK) j.status = js;
+      for (auto b = busy_.begin(); b != busy_.end(); ++b)
+        if (*b == &r) {
+          busy_.erase(b);
+          break;
+        }
+      if (rc != PPF_OK && j.status == PPF_OK) {
+        j.status = rc;
+        j.error = err;
+      }
+      if (--r.jobs_left == 0) {
+        lk.unlock();
+        be_->release(r);
+        lk.lock();
+        retire_locked(r);
+        int worst = r.status;
+        std::string e = r.error;
+        for (Job &x : r.jobs)
+          if (worst == PPF_OK && x.status != PPF_OK && x.status != PPF_ERR_NO_VOTES) {
+            worst = x.status;
+            e = x.error;
+          }
+        finalize_locked(r, worst == PPF_OK ? State::Done : State::Failed, worst, e);
+      }
+    }
+  }
+
+  Backend *be_;
+  Config cfg_;
+  std::mutex mu_;
+  std::condition_variable sched_cv_, lane_cv_, done_cv_;
+  std::map<uint64_t, std::unique_ptr<Request>> reqs_;
+  std::deque<Request *> queue_;
+  std::vector<Request *> active_;
+  std::deque<std::pair<Request *, Job *>> finq_;
+  std::vector<Request *> busy_;
+  uint64_t inflight_ = 0;
+  uint64_t next_ticket_ = 0;
+  bool stopping_ = false, lanes_stop_ = false;
+  Stats st_;
+  std::thread sched_;
+  std::vector<std::thread> lanes_;
+};
+
+}  // namespace ppfs

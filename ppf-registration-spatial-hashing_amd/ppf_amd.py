@@ -49,7 +49,7 @@ class ModelInfo(C.Structure):
 class LookupParams(C.Structure):
     _fields_ = [("vote_count_threshold", C.c_float), ("cpu_clustering", C.c_int32),
                 ("use_l1_norm", C.c_int32), ("use_averaged_clusters", C.c_int32),
-                ("exact_votes", C.c_int32)]
+                ("exact_votes", C.c_int32), ("candidate_capacity", C.c_uint32)]
 
 
 class ResultInfo(C.Structure):
@@ -65,6 +65,24 @@ class BatchInfo(C.Structure):
     _fields_ = [("n_jobs", C.c_int64), ("lookups", C.c_int64), ("votes", C.c_int64),
                 ("vote_launches", C.c_int32), ("lanes", C.c_int32), ("vote_ms", C.c_double),
                 ("finish_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_int32), ("lanes", C.c_int32),
+                ("round_ms", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_targets", C.c_int32), ("vote_rounds", C.c_int32),
+                ("pad", C.c_int32), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+                ("device_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "cancelled", "rejected",
+                                         "rounds", "rows_voted", "queued", "active")] + \
+               [(k, C.c_uint64) for k in ("inflight_bytes", "peak_inflight_bytes", "budget_bytes")] + \
+               [("max_requests_in_round", C.c_int64), ("vote_ms", C.c_double)]
 
 
 _lib = None
@@ -107,11 +125,20 @@ def lib():
                                    _f32p, _vp]
     L.ppf_batch_lookup.argtypes = [_vp, C.c_int32, _vp, _vp, C.POINTER(LookupParams), _vp,
                                    C.POINTER(BatchInfo)]
+    L.ppf_server_create.argtypes = [_vp, C.c_int32, _vp, C.POINTER(ServerConfig), C.POINTER(_vp)]
+    L.ppf_server_destroy.argtypes = [_vp]
+    L.ppf_server_submit.argtypes = [_vp, _f32p, _f32p, C.c_int64, C.c_uint32, C.c_int32, _vp,
+                                    C.POINTER(LookupParams), C.POINTER(C.c_uint64)]
+    L.ppf_server_wait.argtypes = [_vp, C.c_uint64, C.c_double, _vp, _vp, _vp, C.POINTER(RequestInfo)]
+    L.ppf_server_poll.argtypes = [_vp, C.c_uint64, C.POINTER(C.c_int32)]
+    L.ppf_server_cancel.argtypes = [_vp, C.c_uint64]
+    L.ppf_server_stats.argtypes = [_vp, C.POINTER(ServerStats)]
     L.ppf_debug_numerics.argtypes = [_vp, C.c_int32, _f32p, _vp, _f32p, C.c_int64]
     L.ppf_debug_primitive.argtypes = [_vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32]
-    sizes = (C.c_uint32 * 3)()
+    sizes = (C.c_uint32 * 7)()
     L.ppf_abi_sizes(sizes)
-    want = (C.sizeof(ModelInfo), C.sizeof(LookupParams), C.sizeof(ResultInfo))
+    want = tuple(C.sizeof(t) for t in (ModelInfo, LookupParams, ResultInfo, BatchInfo, ServerConfig,
+                                       RequestInfo, ServerStats))
     if tuple(sizes) != want:
         raise PPFError(PPF_ERR_STATE, f"{LIB_PATH} was built from a different include/ppf_amd.h "
                                       f"(struct sizes {tuple(sizes)} != {want}): rebuild it")
@@ -179,7 +206,8 @@ class Context:
         if getattr(self, "handle", None):
             # results before models/scenes (they are independent on the device,
             # but keep a deterministic order)
-            kids = sorted(list(self._children), key=lambda o: 0 if isinstance(o, LookupResult) else 1)
+            order = {"Server": 0, "LookupResult": 1}
+            kids = sorted(list(self._children), key=lambda o: order.get(type(o).__name__, 2))
             for k in kids:
                 k.close()
             lib().ppf_ctx_destroy(self.handle)
@@ -352,10 +380,11 @@ class Model(_Owned):
         w = np.ascontiguousarray(weights, dtype=np.float32)
         _check(lib().ppf_model_set_weights(self.handle, w, len(w)))
 
-    def ppf_lookup(self, scene: Scene, exact_votes: bool = False) -> LookupResult:
+    def ppf_lookup(self, scene: Scene, exact_votes: bool = False,
+                   candidate_capacity: int = 0) -> LookupResult:
         prm = LookupParams(self.vote_count_threshold, int(self.cpu_clustering),
                            int(self.use_l1_norm), int(self.use_averaged_clusters),
-                           int(exact_votes))
+                           int(exact_votes), int(candidate_capacity))
         h = _vp()
         _check(lib().ppf_lookup(self.handle, scene.handle, C.byref(prm), C.byref(h)))
         self.result = LookupResult(h, self.cpu_clustering, self.ctx)
@@ -374,7 +403,7 @@ class Model(_Owned):
 def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_threshold: float,
                  cpu_clustering: bool = False, use_l1_norm: bool = False,
                  use_averaged_clusters: bool = False, exact_votes: bool = False,
-                 ctx: Optional[Context] = None):
+                 ctx: Optional[Context] = None, candidate_capacity: int = 0):
     """Batched form: result j of (models[j], scenes[j]), all jobs voted in one
     persistent launch (ppf_batch_lookup).  Returns (results, info dict)."""
     if len(models) != len(scenes) or not models:
@@ -384,12 +413,84 @@ def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_
     mh = (C.c_void_p * n)(*[m.handle.value for m in models])
     sh = (C.c_void_p * n)(*[s.handle.value for s in scenes])
     prm = LookupParams(float(vote_count_threshold), int(cpu_clustering), int(use_l1_norm),
-                       int(use_averaged_clusters), int(exact_votes))
+                       int(use_averaged_clusters), int(exact_votes), int(candidate_capacity))
     out = (C.c_void_p * n)()
     info = BatchInfo()
     _check(lib().ppf_batch_lookup(ctx.handle, n, mh, sh, C.byref(prm), out, C.byref(info)))
     res = [LookupResult(_vp(out[j]), bool(cpu_clustering), ctx) for j in range(n)]
     return res, {k: getattr(info, k) for k, _ in BatchInfo._fields_}
+
+
+PPF_ERR_TIMEOUT = 6
+PPF_ERR_CANCELLED = 7
+
+
+class Server(_Owned):
+    """Served form: a request loop over resident models (ppf_server_*).
+
+    submit() copies the scene and returns a ticket; wait() returns
+    (poses[n_targets,4,4], statuses[n_targets], info) and, with
+    results=True, the full LookupResult of every target."""
+
+    def __init__(self, models: Sequence["Model"], hbm_budget_bytes: int = 0, max_active: int = 0,
+                 lanes: int = 0, round_ms: float = 0.0, ctx: Optional[Context] = None):
+        ctx = ctx or models[0].ctx
+        self.models = list(models)
+        mh = (C.c_void_p * len(models))(*[m.handle.value for m in models])
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), int(lanes), float(round_ms))
+        h = _vp()
+        _check(lib().ppf_server_create(ctx.handle, len(models), mh, C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._destroy = "ppf_server_destroy"
+        self._register(ctx)
+        self._targets = {}
+
+    def submit(self, cloud, model_ids: Sequence[int], ref_point_downsample_factor: int,
+               vote_count_threshold: float, cpu_clustering: bool = False, use_l1_norm: bool = False,
+               use_averaged_clusters: bool = False, exact_votes: bool = False,
+               candidate_capacity: int = 0) -> int:
+        p, n = _cloud(*cloud)
+        ids = np.ascontiguousarray(model_ids, dtype=np.int32)
+        prm = LookupParams(float(vote_count_threshold), int(cpu_clustering), int(use_l1_norm),
+                           int(use_averaged_clusters), int(exact_votes), int(candidate_capacity))
+        t = C.c_uint64()
+        _check(lib().ppf_server_submit(self.handle, p, n, len(p), int(ref_point_downsample_factor),
+                                       len(ids), ids.ctypes.data, C.byref(prm), C.byref(t)))
+        self._targets[t.value] = (len(ids), bool(cpu_clustering))
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float = -1.0, results: bool = False):
+        nt, cpu = self._targets[ticket]
+        poses = np.zeros(nt * 16, np.float32)
+        st = np.zeros(nt, np.int32)
+        res = (C.c_void_p * nt)() if results else None
+        info = RequestInfo()
+        rc = lib().ppf_server_wait(self.handle, ticket, float(timeout_s), poses.ctypes.data,
+                                   st.ctypes.data, res, C.byref(info))
+        if rc == PPF_ERR_TIMEOUT:
+            raise PPFError(rc, lib().ppf_last_error().decode())
+        del self._targets[ticket]
+        out_res = None
+        if results:
+            out_res = [LookupResult(_vp(res[k]), cpu, self.ctx) if res[k] else None for k in range(nt)]
+        d = {k: getattr(info, k) for k, _ in RequestInfo._fields_}
+        d["rc"] = rc
+        if rc != PPF_OK:
+            d["error"] = lib().ppf_last_error().decode()
+        return poses.reshape(nt, 4, 4), st, d, out_res
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int32()
+        _check(lib().ppf_server_poll(self.handle, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def cancel(self, ticket: int) -> int:
+        return int(lib().ppf_server_cancel(self.handle, ticket))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().ppf_server_stats(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
 
 
 def _close_all_contexts():

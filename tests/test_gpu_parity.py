@@ -260,3 +260,42 @@ def test_device_radix_sort_is_stable(ctx, case):
 def test_device_copy_kernel(ctx):
     a = np.random.default_rng(0).integers(0, 2**32, 1 << 20, dtype=np.uint64).astype(np.uint32)
     assert np.array_equal(ctx.debug_copy(a), a)
+
+
+def test_candidate_overflow_reruns_give_same_result(ctx):
+    """A candidate buffer far too small for the kept votes forces two reruns
+    of the vote pass (the second sized to an odd kept-vote count rounds up);
+    results equal the default-capacity lookup, in the plain, batched and
+    served forms."""
+    w = synth.plain_small(seed=13, n_model=500, n_clutter=800)
+    m = ppf_amd.Model(w.models[0], w.d_dists[0], w.vote_count_threshold, ctx=ctx)
+    s = ppf_amd.Scene(w.scenes[0], 0.0, w.rpdf, ctx=ctx)
+    ref = m.ppf_lookup(s)
+    assert ref.info["n_kept"] > 600  # > 512: the 257-entry buffer (rounded to 512) overflows twice
+    c0, n0 = ref.votes()
+    small = m.ppf_lookup(s, candidate_capacity=257)
+    c1, n1 = small.votes()
+    assert np.array_equal(c0, c1) and np.array_equal(n0, n1)
+    assert small.pose().tobytes() == ref.pose().tobytes()
+    assert small.info["votes"] == ref.info["votes"]
+    res, info = ppf_amd.batch_lookup([m, m], [s, s], w.vote_count_threshold, ctx=ctx,
+                                     candidate_capacity=300)
+    assert info["vote_launches"] >= 2
+    for r in res:
+        c2, n2 = r.votes()
+        assert np.array_equal(c0, c2) and np.array_equal(n0, n2)
+        assert r.pose().tobytes() == ref.pose().tobytes()
+        r.close()
+    srv = ppf_amd.Server([m], round_ms=1.0, ctx=ctx)
+    try:
+        t = srv.submit(w.scenes[0], [0], w.rpdf, w.vote_count_threshold, candidate_capacity=100)
+        poses, st, info, out = srv.wait(t, 120.0, results=True)
+        assert info["rc"] == ppf_amd.PPF_OK
+        c3, n3 = out[0].votes()
+        assert np.array_equal(c0, c3) and np.array_equal(n0, n3)
+        assert poses[0].tobytes() == ref.pose().tobytes()
+        out[0].close()
+    finally:
+        srv.close()
+    for o in (small, ref, m, s):
+        o.close()

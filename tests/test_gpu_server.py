@@ -1,0 +1,110 @@
+"""Served form (ppf_server_*): each request's result is the same alone and in
+a mix of concurrent requests, and equal to the plain Model::ppf_lookup of the
+same (model, scene) pair (src/cuda/model.cu:269-306): vote codes and counts,
+transformations, clustering outputs and final poses bit for bit."""
+import numpy as np
+import pytest
+
+import ppf_amd
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _workload(seed=31):
+    return synth.batched(seed=seed, n_models=4, n_scenes=6, model_range=(300, 1200),
+                         scene_range=(800, 4000), models_per_scene=(1, 3))
+
+
+def _digest(res: ppf_amd.LookupResult, gpu: bool):
+    c, n = res.votes()
+    out = [res.info["status"], res.info["votes"], res.info["max_count"], res.info["n_kept"],
+           res.info["max_idx"], c.tobytes(), n.tobytes(), res.transformations().tobytes(),
+           res.pose().tobytes()]
+    if gpu and res.n_kept:
+        out += [x.tobytes() for x in res.clusters()]
+    return out
+
+
+@pytest.mark.parametrize("cpu_clustering", [False, True])
+def test_alone_equals_mix_equals_plain(ctx, cpu_clustering):
+    w = _workload()
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, cpu_clustering=cpu_clustering, ctx=ctx)
+              for m, d in zip(w.models, w.d_dists)]
+    srv = ppf_amd.Server(models, round_ms=2.0, ctx=ctx)
+    try:
+        reqs = [(s, ids) for s, ids in zip(w.scenes, w.scene_models)]
+        alone = []
+        for s, ids in reqs:
+            t = srv.submit(s, ids, w.rpdf, w.vote_count_threshold, cpu_clustering=cpu_clustering)
+            poses, st, info, res = srv.wait(t, 120.0, results=True)
+            assert info["rc"] == ppf_amd.PPF_OK, info
+            alone.append((poses, st, [_digest(r, not cpu_clustering) for r in res]))
+            for r in res:
+                r.close()
+        tickets = [srv.submit(s, ids, w.rpdf, w.vote_count_threshold, cpu_clustering=cpu_clustering)
+                   for s, ids in reqs]
+        for (s, ids), t, (poses_a, st_a, dig_a) in zip(reqs, tickets, alone):
+            poses, st, info, res = srv.wait(t, 120.0, results=True)
+            assert info["rc"] == ppf_amd.PPF_OK, info
+            assert poses.tobytes() == poses_a.tobytes()
+            assert np.array_equal(st, st_a)
+            assert [_digest(r, not cpu_clustering) for r in res] == dig_a
+            for r in res:
+                r.close()
+        stats = srv.stats()
+        assert stats["completed"] == 2 * len(reqs) and stats["failed"] == 0
+        assert stats["max_requests_in_round"] >= 2  # the mix shared vote rounds
+        assert stats["inflight_bytes"] == 0
+        # alone == plain lookups of the same pairs
+        for (s, ids), (poses_a, st_a, dig_a) in zip(reqs, alone):
+            sc = ppf_amd.Scene(s, 0.0, w.rpdf, ctx=ctx)
+            for k, mi in enumerate(ids):
+                r = models[mi].ppf_lookup(sc)
+                assert _digest(r, not cpu_clustering) == dig_a[k]
+                r.close()
+            sc.close()
+    finally:
+        srv.close()
+        for m in models:
+            m.close()
+
+
+def test_budget_cancel_and_errors(ctx):
+    w = _workload(seed=32)
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    small = synth.plain_small(seed=9, n_model=200, n_clutter=300).scenes[0]
+    big = w.scenes[int(np.argmax([len(s[0]) for s in w.scenes]))]
+    # budget: one single-target small request fits, the big one never does
+    n_small = len(small[0])
+    R_small = (n_small + 4) // 5
+    need_small = n_small * 28 + R_small * 64 + 512 + max(1 << 20, R_small * 2048) * 12 + 512
+    srv = ppf_amd.Server(models, hbm_budget_bytes=need_small + 1024, round_ms=2.0, ctx=ctx)
+    try:
+        tb = srv.submit(big, [0], 5, w.vote_count_threshold)
+        ts = srv.submit(small, [1], 5, w.vote_count_threshold)
+        _, _, info_b, _ = srv.wait(tb, 60.0)
+        assert info_b["rc"] == ppf_amd.PPF_ERR_NOMEM and "budget" in info_b["error"]
+        poses, st, info_s, _ = srv.wait(ts, 60.0)
+        assert info_s["rc"] == ppf_amd.PPF_OK and info_s["vote_rounds"] >= 1
+        assert set(st.tolist()) <= {ppf_amd.PPF_OK, ppf_amd.PPF_ERR_NO_VOTES}
+        # cancel: the second of two small requests waits for budget
+        t1 = srv.submit(small, [2], 5, w.vote_count_threshold)
+        t2 = srv.submit(small, [3], 5, w.vote_count_threshold)
+        rc = srv.cancel(t2)
+        _, _, i1, _ = srv.wait(t1, 60.0)
+        _, _, i2, _ = srv.wait(t2, 60.0)
+        assert i1["rc"] == ppf_amd.PPF_OK
+        if rc == ppf_amd.PPF_OK:
+            assert i2["rc"] == ppf_amd.PPF_ERR_CANCELLED
+        else:  # already admitted when cancel ran
+            assert rc == ppf_amd.PPF_ERR_STATE and i2["rc"] == ppf_amd.PPF_OK
+        with pytest.raises(ppf_amd.PPFError):
+            srv.submit(small, [7], 5, 0.3)  # model id out of range
+        st = srv.stats()
+        assert st["rejected"] == 1 and st["inflight_bytes"] == 0
+        assert st["peak_inflight_bytes"] <= st["budget_bytes"]
+    finally:
+        srv.close()
+        for m in models:
+            m.close()
