@@ -241,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-votes", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and served forms")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="only the batched and served forms (no plain steps, no CPU baseline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     args = ap.parse_args()
@@ -268,6 +270,10 @@ def main():
               for m, d in zip(w.models, w.d_dists)]
     log(f"models built in {time.time() - t0:.2f}s")
     scene = ppf_amd.Scene(w.scenes[0], 0.0, w.rpdf, ctx=ctx)
+
+    if args.forms_only:
+        print(json.dumps({"forms": run_forms(ctx, models, scene, w)}), flush=True)
+        return
 
     def step(record):
         for mdl in models:

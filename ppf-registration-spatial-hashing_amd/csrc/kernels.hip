@@ -1137,6 +1137,16 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
   return 0;
 }
 
+int vote_slots(ppf_ctx ctx, int max_tm, int *slots) {
+  const size_t lds = std::max((size_t)max_tm * N_BINS * 4, (size_t)2 * GT_SIZE * 4);
+  auto kern = k_vote<false>;
+  PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  PPF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, VOTE_BLOCK, lds));
+  *slots = ctx->n_cu * std::max(1, per_cu);
+  return 0;
+}
+
 int select_and_sort(ppf_ctx ctx, const unsigned long long *cand_code, const uint32_t *cand_count,
                     uint32_t n_cand, uint32_t max_count, float thr, int64_t n_s, int64_t n_m,
                     unsigned long long *out_codes, uint32_t *out_counts, int64_t *P_out) {

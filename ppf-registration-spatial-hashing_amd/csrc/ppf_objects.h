@@ -190,6 +190,8 @@ int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mfr
                   const CloudDev &m, int32_t TM, uint32_t *rec);
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
               int64_t max_ns, bool exact, float *vote_ms);
+// Blocks of a vote launch that run at once (reference rows voted in parallel).
+int vote_slots(ppf_ctx ctx, int max_tm, int *slots);
 int select_and_sort(ppf_ctx ctx, const unsigned long long *cand_code, const uint32_t *cand_count,
                     uint32_t n_cand, uint32_t max_count, float thr, int64_t n_s, int64_t n_m,
                     unsigned long long *out_codes, uint32_t *out_counts, int64_t *P_out);

@@ -44,6 +44,9 @@ class HipBackend : public ppfs::Backend {
   ppf_ctx sched = nullptr;          // uploads and vote rounds
   std::vector<ppf_ctx> lanes;       // finishing
   std::vector<ppf_model> models;
+  int slots = 1;
+
+  int parallel_slots() override { return slots; }
 
   uint64_t request_bytes(const ppfs::Request &r) override {
     // scene: SoA (7 floats per point) + frames of the reference rows; each
@@ -237,6 +240,8 @@ int ppf_server_create(ppf_ctx ctx, int32_t n_models, const ppf_model *models,
   c.lanes = cfg && cfg->lanes > 0 ? cfg->lanes : 4;
   c.max_active = cfg && cfg->max_active > 0 ? cfg->max_active : 64;
   c.round_ms = cfg && cfg->round_ms > 0 ? cfg->round_ms : 50.0;
+  c.tail_rows = 8.0;
+  c.aging = 1.0;
   if (cfg && cfg->hbm_budget_bytes) {
     c.budget = cfg->hbm_budget_bytes;
   } else {
@@ -262,6 +267,16 @@ int ppf_server_create(ppf_ctx ctx, int32_t n_models, const ppf_model *models,
   s->be->sched = s->own[0];
   s->be->lanes.assign(s->own.begin() + 1, s->own.end());
   s->be->models.assign(models, models + n_models);
+  int max_tm = 32;
+  for (int32_t k = 0; k < n_models; ++k) max_tm = std::max(max_tm, (int)models[k]->TM);
+  {
+    int rc = ppfk::vote_slots(root, max_tm, &s->be->slots);
+    if (rc) {
+      s->be.reset();
+      free_lanes(s->own);
+      return rc;
+    }
+  }
   s->cfg = c;
   s->sched.reset(new ppfs::Scheduler(s->be.get(), c));
   *out = s.release();

@@ -18,7 +18,9 @@
 // a time (src/cuda/ppf.cu:63-98).
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
@@ -97,6 +99,8 @@ class Backend {
   virtual int finish(int lane, Request &r, Job &j, int *job_status, std::string &err) = 0;
   // Any thread: free the request's device state (results stay).
   virtual void release(Request &r) = 0;
+  // Reference rows the device votes at once (blocks of the vote launch).
+  virtual int parallel_slots() { return 1; }
 };
 
 struct Config {
@@ -104,6 +108,8 @@ struct Config {
   int max_active = 64;          // requests in flight at once
   int lanes = 4;                // finishing lanes
   double round_ms = 50.0;       // target duration of one vote round
+  double tail_rows = 8.0;       // a round lasts at least this many rows per slot
+  double aging = 1.0;           // ms of priority a request gains per ms in flight
   int64_t min_round_rows = 1;   // at least this many rows per round (if available)
   int64_t max_round_rows = 1 << 20;
 };
@@ -335,60 +341,65 @@ class Scheduler {
         sched_cv_.wait(lk);
         continue;
       }
-      // ---- plan the round: equal work share per job, rows >= 1
-      double budget_units = 0.0;
-      if (unit_ms > 0.0) budget_units = cfg_.round_ms / unit_ms;
+      // ---- plan the round.  Jobs in order of estimated remaining device
+      // time minus their age (shortest remaining first, so short requests
+      // finish early; age keeps long ones from starving).  The round's work
+      // budget is round_ms, but at least tail_rows rows per slot of the
+      // costliest row: a round ends with its slowest block, so rounds much
+      // longer than one row keep the device busy.
+      const auto now = Clock::now();
+      const double slots = (double)std::max(1, be_->parallel_slots());
+      double max_row_units = 0.0;
+      for (auto &v : voting) max_row_units = std::max(max_row_units, v.second->row_units);
+      std::vector<std::pair<double, size_t>> order;
+      for (size_t k = 0; k < voting.size(); ++k) {
+        const Job &j = *voting[k].second;
+        const double rem = (double)(j.rows - j.next_row) * j.row_units;
+        const double age = std::chrono::duration<double, std::milli>(now - voting[k].first->t_admit).count();
+        order.emplace_back(unit_ms > 0.0 ? rem * unit_ms - cfg_.aging * age : rem, k);
+      }
+      std::stable_sort(order.begin(), order.end(),
+                       [](const std::pair<double, size_t> &x, const std::pair<double, size_t> &y) {
+                         return x.first < y.first;
+                       });
       std::vector<Slice> slices;
       int64_t total_rows = 0;
-      for (auto &v : voting) {
-        Job &j = *v.second;
-        const int64_t left = j.rows - j.next_row;
-        int64_t rows = left;
-        if (budget_units > 0.0) {
-          const double share = budget_units / (double)voting.size();
-          rows = std::max<int64_t>(1, (int64_t)(share / std::max(1.0, j.row_units)));
-        } else {
-          rows = std::min<int64_t>(left, std::max<int64_t>(1, cfg_.min_round_rows / (int64_t)voting.size()));
+      if (unit_ms > 0.0) {
+        double budget = std::max(cfg_.round_ms / unit_ms, cfg_.tail_rows * slots * max_row_units);
+        for (auto &o : order) {
+          if (budget <= 0.0 && total_rows >= cfg_.min_round_rows) break;
+          Request *r = voting[o.second].first;
+          Job &j = *voting[o.second].second;
+          const int64_t left = j.rows - j.next_row;
+          int64_t rows = (int64_t)std::ceil(std::max(0.0, budget) / std::max(1.0, j.row_units));
+          rows = std::min(left, std::max<int64_t>(rows, 1));
+          slices.push_back({r, &j, j.next_row, rows});
+          total_rows += rows;
+          budget -= (double)rows * j.row_units;
         }
-        rows = std::min(rows, left);
-        slices.push_back({v.first, &j, j.next_row, rows});
-        total_rows += rows;
-      }
-      // fill the device: grow slices toward min_round_rows, cap at max_round_rows
-      for (size_t pass = 0; total_rows < cfg_.min_round_rows && pass < 4; ++pass) {
-        bool grew = false;
-        for (Slice &s : slices) {
-          const int64_t left = s.job->rows - s.row0 - s.rows;
-          const int64_t want = (cfg_.min_round_rows - total_rows + (int64_t)slices.size() - 1) /
-                               (int64_t)slices.size();
-          const int64_t add = std::min(left, want);
-          if (add > 0) {
-            s.rows += add;
-            total_rows += add;
-            grew = true;
-          }
+      } else {
+        // first round: no cost estimate yet
+        const int64_t each = std::max<int64_t>(1, cfg_.min_round_rows / (int64_t)voting.size());
+        for (auto &o : order) {
+          Job &j = *voting[o.second].second;
+          const int64_t rows = std::min(j.rows - j.next_row, each);
+          slices.push_back({voting[o.second].first, &j, j.next_row, rows});
+          total_rows += rows;
         }
-        if (!grew) break;
       }
-      while (total_rows > cfg_.max_round_rows) {
-        bool shrunk = false;
-        for (Slice &s : slices)
-          if (s.rows > 1 && total_rows > cfg_.max_round_rows) {
-            const int64_t cut = std::min(s.rows - 1, total_rows - cfg_.max_round_rows);
-            s.rows -= cut;
-            total_rows -= cut;
-            shrunk = true;
-          }
-        if (!shrunk) break;
-      }
+      // costliest rows first in the launch, so the round's tail holds cheap rows
+      std::stable_sort(slices.begin(), slices.end(), [](const Slice &x, const Slice &y) {
+        return x.job->row_units > y.job->row_units;
+      });
       double units = 0.0;
       int64_t distinct = 0;
       {
-        const Request *last = nullptr;
+        std::vector<const Request *> seen;
         for (Slice &s : slices) {
           units += (double)s.rows * s.job->row_units;
-          if (s.req != last) ++distinct, last = s.req;
+          if (std::find(seen.begin(), seen.end(), s.req) == seen.end()) seen.push_back(s.req);
         }
+        distinct = (int64_t)seen.size();
       }
       lk.unlock();
       double ms = 0.0;
@@ -402,7 +413,7 @@ class Scheduler {
       if (rc != PPF_OK) {
         std::vector<Request *> hit;
         for (Slice &s : slices)
-          if (hit.empty() || hit.back() != s.req) hit.push_back(s.req);
+          if (std::find(hit.begin(), hit.end(), s.req) == hit.end()) hit.push_back(s.req);
         for (Request *r : hit)
           if (r->status == PPF_OK) fail_inflight_locked(lk, *r, rc, err);
         continue;
@@ -412,10 +423,13 @@ class Scheduler {
         unit_ms = unit_ms > 0.0 ? 0.5 * unit_ms + 0.5 * u : u;
       }
       {
-        const Request *last = nullptr;
+        std::vector<Request *> seen;
         for (Slice &s : slices) {
           s.job->next_row += s.rows;
-          if (s.req != last) ++s.req->rounds, last = s.req;
+          if (std::find(seen.begin(), seen.end(), s.req) == seen.end()) {
+            seen.push_back(s.req);
+            ++s.req->rounds;
+          }
         }
       }
       // ---- jobs whose pass ended

@@ -301,7 +301,8 @@ static void test_lanes_overlap() {
   c.round_ms = 1.0;
   Scheduler s(&be, c);
   std::vector<uint64_t> t;
-  for (int k = 0; k < 8; ++k) t.push_back(s.submit(make_req(200, 1, {0, 1, 2, 3})));
+  // one row per job: every job leaves voting in the first round
+  for (int k = 0; k < 8; ++k) t.push_back(s.submit(make_req(8, 8, {0, 1, 2, 3})));
   for (uint64_t k : t) {
     std::unique_ptr<Request> r;
     CHECK(s.wait(k, 60.0, &r) == PPF_OK && r->status == PPF_OK);
