@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "ppf_cluster.h"
 #include "ppf_geom.h"
 #include "ppf_objects.h"
 
@@ -711,53 +712,87 @@ __global__ void k_cluster_chunks(const uint32_t *__restrict__ ucount, int64_t U,
   nchunk[u] = (ucount[u] + 511u) / 512u;
 }
 
+// Order of the poses of a cell for forming clustering chunks: a 4-D Morton
+// code of the normalized quaternion (8 bits per component), so that the 128
+// poses of one wave have close rotations.  The order of the members walked
+// in each neighbour cell is unchanged (pose order); only which poses share a
+// wave depends on this key.
+__device__ __forceinline__ uint32_t spread4(uint32_t v) {  // 8 bits -> every 4th bit
+  v &= 0xffu;
+  v = (v | (v << 12)) & 0x000f000fu;
+  v = (v | (v << 6)) & 0x03030303u;
+  v = (v | (v << 3)) & 0x11111111u;
+  return v;
+}
+
+__global__ void k_cluster_order_keys(const float *__restrict__ quats, const uint64_t *__restrict__ words,
+                                     int64_t P, uint64_t *__restrict__ okeys,
+                                     uint32_t *__restrict__ ovals) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  const uint64_t wd = words[k];  // sorted (cell hash, pose)
+  const uint32_t i = (uint32_t)wd;
+  float q[4], n2 = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    q[c] = quats[4 * (size_t)i + c];
+    n2 += q[c] * q[c];
+  }
+  const float inv = n2 > 0.0f && n2 < 1e30f ? 1.0f / sqrtf(n2) : 0.0f;
+  uint32_t m = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float x = q[c] * inv;
+    x = x == x ? fminf(fmaxf(x, -1.0f), 1.0f) : 0.0f;
+    const uint32_t b = (uint32_t)((x + 1.0f) * 127.5f);
+    m |= spread4(b) << c;
+  }
+  okeys[k] = (wd & 0xffffffff00000000ull) | (uint64_t)m;
+  ovals[k] = i;
+}
+
+// Member data in cell order: quaternion, and translation + weight.
+__global__ void k_cluster_gather(const uint32_t *__restrict__ members, int64_t P,
+                                 const float *__restrict__ trans, const float *__restrict__ quats,
+                                 const float *__restrict__ w, float4 *__restrict__ cq,
+                                 float4 *__restrict__ ct) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  const uint32_t j = members[k];
+  cq[k] = make_float4(quats[4 * (size_t)j], quats[4 * (size_t)j + 1], quats[4 * (size_t)j + 2],
+                      quats[4 * (size_t)j + 3]);
+  ct[k] = make_float4(trans[3 * (size_t)j], trans[3 * (size_t)j + 1], trans[3 * (size_t)j + 2], w[j]);
+}
+
 // Poses that share a voxel cell share the 26 neighbour hashes, so one block
-// takes up to 512 poses of one cell (two per thread) and streams every
-// neighbour bucket through LDS once; each thread walks the bucket in order,
-// which keeps the summation order of rot_clustering_kernel
-// (kernel.cu:724-756).  Chunks whose poses do not all lie in one cell (an FNV
-// collision between cells) walk their own neighbour lists per pose.
+// takes up to 512 poses of one cell (128 per wave, two per thread, in the
+// rotation order above) and streams every neighbour bucket through LDS
+// once.  Each thread walks the bucket in member order, which keeps the
+// summation order of rot_clustering_kernel (kernel.cu:724-756).  Chunks
+// whose poses do not all lie in one cell (an FNV collision between cells)
+// walk their own neighbour lists per pose.
 //
 // Translation test: sqrtf is correctly rounded and monotone, so
 // sqrtf(s) < d_dist holds exactly when s <= s_max, the largest float whose
 // square root rounds below d_dist (computed on the host).  s is the same
 // ((dx*dx + dy*dy) + dz*dz) sum as norm(); no square root is taken.
+//
+// Wave pre-test: a wave bounds its poses by a rotation cap (centre c, angle
+// rho, largest quaternion norm) and a translation ball (centre, radius).
+// The rotation test can pass only when the float dot q_i.q_j exceeds
+// 1 - ROT_THRESH^2/8 > 0.978, i.e. when the angle between the quaternions is
+// below acos(0.978 / (|q_i||q_j|)); by the triangle inequality on the unit
+// 3-sphere a member whose angle to c exceeds that plus rho fails for every
+// pose of the wave.  Likewise a member farther than d_dist + radius from the
+// ball's centre fails the translation test.  Both bounds carry margins far
+// above float rounding (5e-3 rad, 1e-5 relative), and a member is skipped
+// only when a bound proves that it fails for all poses of the wave; skipped
+// members would add nothing, so every sum keeps its order and bits.
 constexpr int CL_BLOCK = 256;
 constexpr int CL_PER = 2;
 constexpr int CL_CHUNK = CL_BLOCK * CL_PER;
-
-struct ClPose {
-  float tx, ty, tz, qx, qy, qz, qw;
-  float vc, ox, oy, oz;
-};
-
-__device__ __forceinline__ void cluster_test(ClPose &P, const float4 jq, const float4 jt,
-                                             float s_max, int use_l1, int use_avg) {
-  const float rot_thresh_sq = ppfm::ROT_THRESH * ppfm::ROT_THRESH;
-  const float d4 = P.qx * jq.x + P.qy * jq.y + P.qz * jq.z + P.qw * jq.w;
-  const float qd = fabsf(8.0f * (1.0f - d4));
-  if (qd < rot_thresh_sq) {
-    if (!use_l1) {
-      const float dx = P.tx - jt.x, dy = P.ty - jt.y, dz = P.tz - jt.z;
-      const float ss = dx * dx + dy * dy + dz * dz;
-      if (!(ss <= s_max)) return;
-    }
-    const float wj = jt.w;
-    if (use_avg) {
-      P.ox = P.vc * P.ox;
-      P.oy = P.vc * P.oy;
-      P.oz = P.vc * P.oz;
-      P.ox = P.ox + wj * jt.x;
-      P.oy = P.oy + wj * jt.y;
-      P.oz = P.oz + wj * jt.z;
-      const float sc = 1.0f / (P.vc + wj);
-      P.ox = sc * P.ox;
-      P.oy = sc * P.oy;
-      P.oz = sc * P.oz;
-    }
-    P.vc += wj;
-  }
-}
+using ppfc::ClPose;
+using ppfc::ClWave;
 
 __device__ __forceinline__ int64_t cell_lower_bound(const uint32_t *__restrict__ ukeys, int64_t U,
                                                     uint32_t h) {
@@ -787,19 +822,56 @@ __device__ __forceinline__ void load_pose(ClPose &P, const float *__restrict__ t
   P.oz = P.tz;
 }
 
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+__device__ __forceinline__ float readfirst_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// Bounds of the 128 poses of one wave (centre: lane 0's first pose).
+__device__ ClWave cluster_wave_bounds(const ClPose *P, const bool *have) {
+  ClPose c = P[0];
+  c.tx = readfirst_f32(c.tx);
+  c.ty = readfirst_f32(c.ty);
+  c.tz = readfirst_f32(c.tz);
+  c.qx = readfirst_f32(c.qx);
+  c.qy = readfirst_f32(c.qy);
+  c.qz = readfirst_f32(c.qz);
+  c.qw = readfirst_f32(c.qw);
+  const bool h0 = __builtin_amdgcn_readfirstlane(have[0] ? 1 : 0) != 0;
+  ClWave W;
+  ppfc::cl_wave_center(c, h0, W);
+  float rho = 0.0f, nmax = 0.0f, rad = 0.0f;
+#pragma unroll
+  for (int r = 0; r < CL_PER; ++r)
+    if (have[r]) ppfc::cl_wave_add(W, P[r], rho, nmax, rad);
+  ppfc::cl_wave_finish(W, wave_max_f32(rho), wave_max_f32(nmax), wave_max_f32(rad));
+  return W;
+}
+
+__device__ __forceinline__ void cl_test(ClPose &P, const float4 q, const float4 t, float s_max,
+                                        int use_l1, int use_avg) {
+  ppfc::cluster_test(P, q.x, q.y, q.z, q.w, t.x, t.y, t.z, t.w, s_max, use_l1, use_avg);
+}
+
 __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
-    const float *__restrict__ trans, const float *__restrict__ quats, const float *__restrict__ w,
-    const uint32_t *__restrict__ adj, const uint32_t *__restrict__ ukeys,
-    const uint32_t *__restrict__ ufirst, const uint32_t *__restrict__ ucount, int64_t U,
-    const uint32_t *__restrict__ chunk_start, const uint32_t *__restrict__ members, float s_max,
-    int use_l1, int use_avg, float *__restrict__ vc_out, float *__restrict__ trans_out,
+    const float *__restrict__ trans, const float *__restrict__ quats, const float4 *__restrict__ cq,
+    const float4 *__restrict__ ct, const uint32_t *__restrict__ adj,
+    const uint32_t *__restrict__ ukeys, const uint32_t *__restrict__ ufirst,
+    const uint32_t *__restrict__ ucount, int64_t U, const uint32_t *__restrict__ chunk_start,
+    const uint32_t *__restrict__ centers, float s_max, float d_dist, int use_l1, int use_avg,
+    float *__restrict__ vc_out, float *__restrict__ trans_out,
     unsigned long long *__restrict__ n_tests) {
   __shared__ float4 s_q[CL_BLOCK];
   __shared__ float4 s_t[CL_BLOCK];
   __shared__ int s_u, s_c, s_uniform;
   __shared__ uint32_t s_adj[27];
   __shared__ int64_t s_bucket;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
   if (tid == 0) {
     // which (cell, chunk) this block owns
     const uint32_t b = blockIdx.x;
@@ -819,15 +891,17 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   const int u = s_u;
   const uint32_t f = ufirst[u], n = ucount[u];
   const uint32_t base = (uint32_t)s_c * CL_CHUNK;
-  const uint32_t i0 = members[f + base];
+  const uint32_t i0 = centers[f + base];
   bool have[CL_PER];
   uint32_t idx[CL_PER];
   ClPose P[CL_PER];
 #pragma unroll
   for (int r = 0; r < CL_PER; ++r) {
-    const uint32_t k = base + (uint32_t)(r * CL_BLOCK + tid);
+    // wave wv holds chunk positions [128 wv, 128 wv + 128): consecutive in
+    // the rotation order
+    const uint32_t k = base + (uint32_t)(wv * (WAVE * CL_PER) + r * WAVE + lane);
     have[r] = k < n;
-    idx[r] = have[r] ? members[f + k] : i0;
+    idx[r] = have[r] ? centers[f + k] : i0;
     if (have[r]) {
       for (int q = 0; q < 27; ++q)
         if (adj[27 * (size_t)idx[r] + q] != adj[27 * (size_t)i0 + q]) s_uniform = 0;
@@ -838,6 +912,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   __syncthreads();
   unsigned long long tests = 0;
   if (s_uniform) {
+    const ClWave W = cluster_wave_bounds(P, have);
     for (int blk = 0; blk < 27; ++blk) {
       const uint32_t h = s_adj[blk];
       if (h == 0u) continue;
@@ -852,26 +927,25 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
       for (uint32_t c0 = 0; c0 < bn; c0 += CL_BLOCK) {
         const uint32_t cn = min((uint32_t)CL_BLOCK, bn - c0);
         if ((uint32_t)tid < cn) {
-          const uint32_t j = members[bf + c0 + tid];
-          s_q[tid] = make_float4(quats[4 * (size_t)j], quats[4 * (size_t)j + 1],
-                                 quats[4 * (size_t)j + 2], quats[4 * (size_t)j + 3]);
-          s_t[tid] = make_float4(trans[3 * (size_t)j], trans[3 * (size_t)j + 1],
-                                 trans[3 * (size_t)j + 2], w[j]);
+          s_q[tid] = cq[bf + c0 + tid];
+          s_t[tid] = ct[bf + c0 + tid];
         }
         __syncthreads();
-        uint32_t jj = 0;
-        for (; jj + 2 <= cn; jj += 2) {
-          const float4 q0 = s_q[jj], t0 = s_t[jj], q1 = s_q[jj + 1], t1 = s_t[jj + 1];
-#pragma unroll
-          for (int r = 0; r < CL_PER; ++r) {
-            cluster_test(P[r], q0, t0, s_max, use_l1, use_avg);
-            cluster_test(P[r], q1, t1, s_max, use_l1, use_avg);
+        for (uint32_t g0 = 0; g0 < cn; g0 += WAVE) {
+          const uint32_t gj = g0 + (uint32_t)lane;
+          bool live = false;
+          if (gj < cn) {
+            const float4 mq = s_q[gj], mt = s_t[gj];
+            live = !ppfc::cluster_skip(W, mq.x, mq.y, mq.z, mq.w, mt.x, mt.y, mt.z, d_dist, use_l1);
           }
-        }
-        if (jj < cn) {
-          const float4 q0 = s_q[jj], t0 = s_t[jj];
+          uint64_t mask = __ballot(live);
+          while (mask) {
+            const uint32_t k = g0 + (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            const float4 q0 = s_q[k], t0 = s_t[k];
 #pragma unroll
-          for (int r = 0; r < CL_PER; ++r) cluster_test(P[r], q0, t0, s_max, use_l1, use_avg);
+            for (int r = 0; r < CL_PER; ++r) cl_test(P[r], q0, t0, s_max, use_l1, use_avg);
+          }
         }
         __syncthreads();
       }
@@ -887,14 +961,8 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
         if (bu < 0) continue;
         const uint32_t bf = ufirst[bu], bn = ucount[bu];
         tests += bn;
-        for (uint32_t jj = 0; jj < bn; ++jj) {
-          const uint32_t j = members[bf + jj];
-          const float4 jq = make_float4(quats[4 * (size_t)j], quats[4 * (size_t)j + 1],
-                                        quats[4 * (size_t)j + 2], quats[4 * (size_t)j + 3]);
-          const float4 jt = make_float4(trans[3 * (size_t)j], trans[3 * (size_t)j + 1],
-                                        trans[3 * (size_t)j + 2], w[j]);
-          cluster_test(P[r], jq, jt, s_max, use_l1, use_avg);
-        }
+        for (uint32_t jj = 0; jj < bn; ++jj)
+          cl_test(P[r], cq[bf + jj], ct[bf + jj], s_max, use_l1, use_avg);
       }
     }
   }
@@ -1212,11 +1280,17 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   *max_idx = -1;
   *pair_tests = 0;
   if (P <= 0) return 0;
-  // trans (3P) w (P) words (P u64) adj (27P) ukeys/ufirst/ucount (3P) members (P) argmax (8)
-  // test counter (8)
-  const size_t bytes = (size_t)P * (12 + 4 + 8 + 108 + 12 + 4) + 128;
+  // cq, ct (2 x float4 P) words, okeys (2 x u64 P) trans (3P) w (P) adj (27P)
+  // ukeys/ufirst/ucount (3P) members, centers (2P) argmax (8) test counter (8)
+  const size_t bytes = (size_t)P * (32 + 16 + 12 + 4 + 108 + 12 + 8) + 128;
   PPF_TRY(ctx->tmp_b.ensure(bytes));
   char *p = ctx->tmp_b.as<char>();
+  float4 *cq = reinterpret_cast<float4 *>(p);
+  p += (size_t)P * 16;
+  float4 *ct = reinterpret_cast<float4 *>(p);
+  p += (size_t)P * 16;
+  uint64_t *okeys = reinterpret_cast<uint64_t *>(p);
+  p += (size_t)P * 8;
   uint64_t *words = reinterpret_cast<uint64_t *>(p);
   p += (size_t)P * 8;
   float *trans = reinterpret_cast<float *>(p);
@@ -1233,6 +1307,8 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   p += (size_t)P * 4;
   uint32_t *members = reinterpret_cast<uint32_t *>(p);
   p += (size_t)P * 4;
+  uint32_t *centers = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
   long long *am = reinterpret_cast<long long *>(((uintptr_t)p + 7) & ~(uintptr_t)7);
   unsigned long long *ntest = reinterpret_cast<unsigned long long *>(am + 1);
   PPF_HIP(hipMemsetAsync(ntest, 0, 8, ctx->stream));
@@ -1243,6 +1319,15 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   int64_t U = 0;
   PPF_TRY(unique_count(ctx, words, P, &U));
   PPF_TRY(unique_emit(ctx, words, P, U, ukeys, ufirst, ucount, members));
+  // chunk order of each cell's poses (rotation Morton code); member data in
+  // cell order for contiguous tile loads
+  hipLaunchKernelGGL(k_cluster_order_keys, dim3(nblk(P, 256)), dim3(256), 0, ctx->stream, quats, words,
+                     P, okeys, centers);
+  PPF_HIP(hipGetLastError());
+  PPF_TRY(radix_sort_u64(okeys, centers, P, 0, 64, ctx->sort, ctx->stream));
+  hipLaunchKernelGGL(k_cluster_gather, dim3(nblk(P, 256)), dim3(256), 0, ctx->stream, members, P, trans,
+                     quats, w, cq, ct);
+  PPF_HIP(hipGetLastError());
   // chunk table: chunk_start[u] = exclusive scan of ceil(count/256)
   PPF_TRY(ctx->tmp_a.ensure((size_t)(U + 1) * 8 + 64));
   uint32_t *nch = ctx->tmp_a.as<uint32_t>();
@@ -1254,9 +1339,9 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   PPF_HIP(hipMemcpyAsync(&n_items, ctot, 4, hipMemcpyDeviceToHost, ctx->stream));
   PPF_HIP(hipStreamSynchronize(ctx->stream));
   if (n_items > 0)
-    hipLaunchKernelGGL(k_cluster_tiled, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, w,
-                       adj, ukeys, ufirst, ucount, U, cstart, members, sqrt_below_max(d_dist),
-                       use_l1 ? 1 : 0, use_avg ? 1 : 0, vc, trans_out, ntest);
+    hipLaunchKernelGGL(k_cluster_tiled, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, cq,
+                       ct, adj, ukeys, ufirst, ucount, U, cstart, centers, sqrt_below_max(d_dist),
+                       d_dist, use_l1 ? 1 : 0, use_avg ? 1 : 0, vc, trans_out, ntest);
   hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
   PPF_HIP(hipGetLastError());
   long long mi_t[2] = {-1, 0};

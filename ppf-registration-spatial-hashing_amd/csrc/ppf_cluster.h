@@ -1,0 +1,134 @@
+// ppf_cluster.h -- the pose-pair test of GPU spatial-hash clustering and the
+// conservative group bounds that let a wave skip members that fail the test
+// for all of its poses.  Host/device, so that the bounds can be checked on
+// the host against the exact test.
+#pragma once
+
+#include "ppf_math.h"
+
+namespace ppfc {
+
+using namespace ppfm;
+
+// The rotation test below can pass only when the float dot product of the two
+// quaternions exceeds 1 - ROT_THRESH^2/8 = 0.97807.
+constexpr float CL_ROT_DOT_MIN = 0.978f;
+constexpr float CL_ANGLE_MARGIN = 5e-3f;  // rad, far above float rounding of the angles
+constexpr float CL_DIST_MARGIN = 1e-5f;   // relative
+
+struct ClPose {
+  float tx, ty, tz, qx, qy, qz, qw;
+  float vc, ox, oy, oz;  // running vote count and (averaged) translation
+};
+
+// One member j of a neighbour cell against pose P: rot_clustering_kernel's
+// loop body (kernel.cu:735-756).  The rotation test |8(1 - q.q_j)| <
+// ROT_THRESH^2 as written there; the translation test sqrtf(s) < d_dist as
+// s <= s_max (sqrtf is correctly rounded and monotone, s_max is the largest
+// float whose square root rounds below d_dist).  jt.w is the member's weight.
+PPF_HD void cluster_test(ClPose &P, float jqx, float jqy, float jqz, float jqw, float jtx, float jty,
+                         float jtz, float wj, float s_max, int use_l1, int use_avg) {
+  const float rot_thresh_sq = ROT_THRESH * ROT_THRESH;
+  const float d4 = P.qx * jqx + P.qy * jqy + P.qz * jqz + P.qw * jqw;
+  const float qd = fabsf(8.0f * (1.0f - d4));
+  if (qd < rot_thresh_sq) {
+    if (!use_l1) {
+      const float dx = P.tx - jtx, dy = P.ty - jty, dz = P.tz - jtz;
+      const float ss = dx * dx + dy * dy + dz * dz;
+      if (!(ss <= s_max)) return;
+    }
+    if (use_avg) {
+      P.ox = P.vc * P.ox;
+      P.oy = P.vc * P.oy;
+      P.oz = P.vc * P.oz;
+      P.ox = P.ox + wj * jtx;
+      P.oy = P.oy + wj * jty;
+      P.oz = P.oz + wj * jtz;
+      const float sc = 1.0f / (P.vc + wj);
+      P.ox = sc * P.ox;
+      P.oy = sc * P.oy;
+      P.oz = sc * P.oz;
+    }
+    P.vc += wj;
+  }
+}
+
+PPF_HD float clamp_unit(float x) { return fminf(fmaxf(x, -1.0f), 1.0f); }
+
+// Bounds of a group of poses: a rotation cap (normalized centre c, angle rho,
+// largest quaternion norm) and a translation ball (centre, radius).  A member
+// whose quaternion angle to c exceeds acos(0.978 / (nmax |q_j|)) + rho cannot
+// pass the rotation test with any pose of the group (a pass needs the cosine
+// of the angle between q_i and q_j above that bound; triangle inequality on
+// the unit 3-sphere); a member farther than d_dist + rad from the ball's
+// centre cannot pass the translation test.  ok = false disables skipping.
+struct ClWave {
+  float cqx, cqy, cqz, cqw;
+  float ctx, cty, ctz;
+  float rho, nmax, rad, cmag;
+  bool ok;
+};
+
+// Centre from the group's first pose.
+PPF_HD void cl_wave_center(const ClPose &c, bool have, ClWave &W) {
+  const float cn2 = c.qx * c.qx + c.qy * c.qy + c.qz * c.qz + c.qw * c.qw;
+  bool ok = have && cn2 > 0.25f && cn2 < 4.0f;
+  const float v[7] = {c.qx, c.qy, c.qz, c.qw, c.tx, c.ty, c.tz};
+  for (int k = 0; k < 7; ++k) ok = ok && (v[k] == v[k]) && fabsf(v[k]) < 1e30f;
+  const float ci = ok ? 1.0f / sqrtf(cn2) : 0.0f;
+  W.cqx = c.qx * ci;
+  W.cqy = c.qy * ci;
+  W.cqz = c.qz * ci;
+  W.cqw = c.qw * ci;
+  W.ctx = c.tx;
+  W.cty = c.ty;
+  W.ctz = c.tz;
+  W.rho = W.nmax = W.rad = W.cmag = 0.0f;
+  W.ok = ok;
+}
+
+// Fold one pose of the group into (rho, nmax, rad).  A pose with NaN
+// components never passes a test, so fmaxf dropping NaN keeps the bounds
+// valid.
+PPF_HD void cl_wave_add(const ClWave &W, const ClPose &P, float &rho, float &nmax, float &rad) {
+  const float n = sqrtf(P.qx * P.qx + P.qy * P.qy + P.qz * P.qz + P.qw * P.qw);
+  if (n > 0.0f) {
+    const float d = (W.cqx * P.qx + W.cqy * P.qy + W.cqz * P.qz + W.cqw * P.qw) / n;
+    rho = fmaxf(rho, acosf_own(clamp_unit(d)));
+  }
+  nmax = fmaxf(nmax, n);
+  const float dx = P.tx - W.ctx, dy = P.ty - W.cty, dz = P.tz - W.ctz;
+  rad = fmaxf(rad, sqrtf(dx * dx + dy * dy + dz * dz));
+}
+
+PPF_HD void cl_wave_finish(ClWave &W, float rho, float nmax, float rad) {
+  W.rho = rho + CL_ANGLE_MARGIN;
+  W.nmax = nmax * (1.0f + CL_DIST_MARGIN);
+  W.rad = rad;
+  W.cmag = sqrtf(W.ctx * W.ctx + W.cty * W.cty + W.ctz * W.ctz);
+}
+
+// true: the member fails the test against every pose of the group.
+PPF_HD bool cluster_skip(const ClWave &W, float jqx, float jqy, float jqz, float jqw, float jtx,
+                         float jty, float jtz, float d_dist, int use_l1) {
+  if (!W.ok) return false;
+  const float n = sqrtf(jqx * jqx + jqy * jqy + jqz * jqz + jqw * jqw);
+  if (n > 0.0f && n < 1e15f) {
+    const float kappa = CL_ROT_DOT_MIN / (W.nmax * n) - 1e-5f;
+    if (kappa >= 1.0f) return true;
+    if (kappa > -1.0f) {
+      const float cj = (W.cqx * jqx + W.cqy * jqy + W.cqz * jqz + W.cqw * jqw) / n;
+      const float th = acosf_own(clamp_unit(cj));
+      if (th > acosf_own(kappa) + W.rho + CL_ANGLE_MARGIN) return true;
+    }
+  }
+  if (!use_l1) {
+    const float dx = jtx - W.ctx, dy = jty - W.cty, dz = jtz - W.ctz;
+    const float D = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float jm = sqrtf(jtx * jtx + jty * jty + jtz * jtz);
+    if (D > d_dist + W.rad + CL_DIST_MARGIN * (W.cmag + jm + d_dist + W.rad)) return true;
+  }
+  return false;
+}
+
+}  // namespace ppfc
