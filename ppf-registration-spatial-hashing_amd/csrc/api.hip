@@ -348,6 +348,7 @@ ppfk::VoteJob vote_job(ppf_model m, ppf_scene s, float thr, unsigned long long *
   J.n_m = (int32_t)m->cloud.n;
   J.TM = m->TM;
   J.T = m->T;
+  J.acc_pitch = m->acc_pitch;
   J.d_dist = m->d_dist;
   J.thr = thr;
   J.max_count = meta + 1;
@@ -520,10 +521,16 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   TRY(m->hcount.alloc(ctx, (size_t)U * 4));
   TRY(ppfk::unique_emit(ctx, words.as<uint64_t>(), N, U, m->hkeys.as<uint32_t>(),
                         m->hfirst.as<uint32_t>(), m->hcount.as<uint32_t>(), m->pair.as<uint32_t>()));
-  // vote order: (key, tile, phi_u), pair indices in vpair
+  // vote order and per-tile sub-ranges
   TRY(m->vpair.alloc(ctx, (size_t)N * 4));
-  TRY(ppfk::vote_order(ctx, words.as<uint64_t>(), m->vpair.as<uint32_t>(), N, m->frames.as<float>(),
-                       m->cloud, m->TM));
+  TRY(m->toff.alloc(ctx, (size_t)U * (m->T + 1) * 4));
+  TRY(ppfk::vote_order(ctx, words.as<uint64_t>(), m->vpair.as<uint32_t>(), N, m->cloud, m->TM, m->T,
+                       m->hfirst.as<uint32_t>(), m->hcount.as<uint32_t>(), U, m->toff.as<uint32_t>()));
+  {
+    int32_t pw = 64;
+    while (pw < m->TM) pw <<= 1;
+    m->acc_pitch = pw + 1;
+  }
   HIPC(hipStreamSynchronize(st));
   words.release();
   // key -> unique index table
@@ -533,10 +540,7 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   m->hshift = 32 - lg;
   TRY(m->htab.alloc(ctx, (size_t)m->hcap * 8));
   TRY(ppfk::build_htable(ctx, m->hkeys.as<uint32_t>(), U, m->htab.as<uint32_t>(), m->hcap, m->hshift));
-  // per-tile sub-ranges and packed vote records
-  TRY(m->toff.alloc(ctx, (size_t)U * (m->T + 1) * 4));
-  TRY(ppfk::build_tile_offsets(ctx, m->hfirst.as<uint32_t>(), m->hcount.as<uint32_t>(),
-                               m->vpair.as<uint32_t>(), U, (int32_t)n, m->TM, m->T, m->toff.as<uint32_t>()));
+  // packed vote records
   TRY(m->rec.alloc(ctx, (size_t)N * 4));
   TRY(ppfk::build_records(ctx, m->vpair.as<uint32_t>(), N, m->frames.as<float>(), m->cloud, m->TM,
                           m->rec.as<uint32_t>()));

@@ -248,59 +248,105 @@ __global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const fl
   rec[j] = (q << 12) | slow | (m_r % (uint32_t)TM);
 }
 
-// Sort words of the vote-record order: (key, tile, phi_u).  Within a bucket
-// and tile, entries ordered by angle instead of by pair index make the lanes
-// of one wave hit different model points, so their LDS accumulator updates do
-// not collide on one address.
+// Vote-record order.  Records are sorted by (key, tile, m_r mod 64), stably
+// (pair order within), then within each (bucket, tile) segment by (rank in
+// its class, class): the first rows of 64 hold one record of every class, and
+// a class only repeats in a row once fewer than 64 classes have records left.
+// With the accumulator at acc[bin * pitch + m], pitch = 2^k + 1 with 2^k a
+// multiple of 64, an update's LDS bank is (m + bin) mod 64, so lanes with
+// distinct classes and one bin, or with one model point and distinct bins,
+// never share a bank.  Any order gives the same votes; this one keeps the
+// 64 atomic adds of a wave instruction in distinct banks as far as possible.
 __global__ void k_vote_order_words(uint64_t *__restrict__ words, uint32_t *__restrict__ vals,
-                                   int64_t N, const float *__restrict__ mframe,
-                                   const float *__restrict__ soa, int32_t n_m, int32_t TM) {
+                                   int64_t N, int32_t n_m, int32_t TM) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   const uint64_t w = words[j];
   const uint32_t p = (uint32_t)w;
-  const uint32_t m_r = p / (uint32_t)n_m, m_i = p - m_r * (uint32_t)n_m;
-  const float *F = mframe + (int64_t)m_r * 16;
-  float uy, uz;
-  ppfg::frame_apply_yz(F + 4, F + 8, soa[m_i], soa[n_m + m_i], soa[2 * n_m + m_i], uy, uz);
-  const double phi = atan2((double)uz, (double)uy);
-  const uint32_t q = (uint32_t)(int32_t)llrint(phi * (1048576.0 / 6.283185307179586)) & 0xfffffu;
+  const uint32_t m_r = p / (uint32_t)n_m;
   const uint32_t tile = m_r / (uint32_t)TM;
-  words[j] = (w & 0xffffffff00000000ull) | ((uint64_t)tile << 20) | (uint64_t)q;
+  words[j] = (w & 0xffffffff00000000ull) | ((uint64_t)tile << 25) | ((uint64_t)(m_r & 63u) << 19);
   vals[j] = p;
+}
+
+// words sorted by (key, tile, class) -> out: (key, tile, rank within the
+// class run, class) words for the second sort
+__global__ void k_vote_rank_words2(const uint64_t *__restrict__ words, uint64_t *__restrict__ out,
+                                   const uint32_t *__restrict__ hfirst, int64_t U,
+                                   const uint32_t *__restrict__ toff, int32_t T, int64_t N) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint64_t w = words[j];
+  const uint32_t tc = (uint32_t)(w >> 19) & 0x1fffu;  // tile << 6 | class
+  int64_t lo = 0, hi = U - 1;  // bucket: last u with hfirst[u] <= j
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if ((int64_t)hfirst[mid] <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  // first record of this (tile, class) in the bucket's tile segment
+  uint32_t a = toff[lo * (T + 1) + (tc >> 6)], b = (uint32_t)j;
+  while (a < b) {
+    const uint32_t mid = (a + b) >> 1;
+    if (((uint32_t)(words[mid] >> 19) & 0x1fffu) < tc)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  const uint32_t rank = min((uint32_t)j - a, (1u << 19) - 1u);
+  out[j] = (w & 0xffffffff00000000ull) | ((uint64_t)(tc >> 6) << 25) | ((uint64_t)rank << 6) |
+           (uint64_t)(tc & 63u);
 }
 
 // --------------------------------------------------------------- voting
 // Fast alpha bin: with theta = phi_v - phi_u the exact angle between the y/z
 // parts of u and v, the float pipeline of alpha_bin() yields
 // floor((theta + pi_f)/D) except within ~1.2e-6 rad of a bin edge (rounding of
-// cross/dot, atan2f error <= 3.2 ulp, rounding of alpha + pi).  phi_u carries
-// <= 3e-6 rad of quantization, phi_v the float atan2 error (<= 8e-7 rad), the
-// float position below <= 2e-6 bin, and nominal edges k*2pi/30 differ from the
-// true ones by < 3e-7 rad.  Together < 4e-5 bin; whenever the position is
-// farther than FAST_MARGIN_BINS (1.2e-4 bin = 2.5e-5 rad) from a nominal edge
-// the bin is decided, otherwise (and for degenerate u or v) the exact float
-// pipeline runs.  The wrap point theta = +-pi (position 0 or 30) is always
-// within the margin, so bin 30 only comes from the exact path.
-constexpr float BIN_SCALE = 30.0f / 4294967296.0f;
+// cross/dot, atan2f error <= 3.2 ulp, rounding of alpha + pi).  The fast path
+// works in 24-bit turns: a = (v24 - phi24) mod 2^24 with v24 = phi_v + pi and
+// phi24 = phi_u, then a * 30 * 2^8 = (bin << 32) | frac exactly (the 24 x 24
+// bit multiplies v_mul_u32_u24 / v_mul_hi_u32_u24).  phi_u carries <= 3e-6
+// rad of quantization (20 bits), phi_v the float atan2 error (<= 8e-7 rad),
+// the truncation to 24 bits < 3.7e-7 rad, and nominal edges k*2pi/30 differ
+// from the true ones by < 3e-7 rad: < 5.5e-6 rad = 2.7e-5 bin in all.
+// Whenever frac is farther than FAST_MARGIN_BINS (1.2e-4 bin = 2.5e-5 rad)
+// from an edge the bin is decided; otherwise (and for degenerate u or v) the
+// exact float pipeline runs.  The wrap point theta = +-pi (position 0 or 30)
+// is always within the margin, so bin 30 only comes from the exact path.
 constexpr float FAST_MARGIN_BINS = 1.2e-4f;
+// frac (as a 32-bit fraction) is decided when it lies in [OFF, 2^32 - OFF]
+// with OFF = floor(1.2e-4 * 2^32) + 1, i.e. when frac - OFF < RANGE (mod 2^32)
+constexpr uint32_t FAST_OFF = 515397u;
+constexpr uint32_t FAST_RANGE = (uint32_t)(4294967296ull - 2ull * (FAST_OFF - 1u) - 1ull);
+static_assert(FAST_OFF - 1u == (uint32_t)(1.2e-4 * 4294967296.0), "fast-bin margin");
 constexpr int VOTE_UNROLL = 4;
 constexpr int GT_LOG2 = 12;
 constexpr int GT_SIZE = 1 << GT_LOG2;  // hit-grouping table slots (2 x 16 KB of LDS)
+
+// Per-block scalars of the vote kernel, kept in dynamic LDS after the
+// accumulator region.
+struct VoteShared {
+  unsigned long long item;
+  int job;
+  uint32_t nh, next, ovf, ngroups;
+  uint32_t wmax[VOTE_BLOCK / WAVE];
+  uint32_t wsum[VOTE_BLOCK / WAVE][2];
+  float thr;
+};
 
 template <bool EXACT>
 __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
                                                      uint64_t n_items,
                                                      unsigned long long *__restrict__ queue,
                                                      uint4 *__restrict__ hit_scratch,
-                                                     uint32_t hit_cap, uint32_t stride_u4) {
+                                                     uint32_t hit_cap, uint32_t stride_u4,
+                                                     uint32_t misc_words) {
+  // dynamic LDS only, so that the accumulator starts at LDS address 0:
+  // [accumulator tile | spare word | ...][VoteShared at misc_words]
   extern __shared__ uint32_t acc[];
-  __shared__ unsigned long long s_item;
-  __shared__ int s_job;
-  __shared__ uint32_t s_nh, s_next, s_ovf, s_ngroups;
-  __shared__ uint32_t s_wmax[VOTE_BLOCK / WAVE];
-  __shared__ uint32_t s_wsum[VOTE_BLOCK / WAVE][2];
-  __shared__ float s_thr;
+  VoteShared &sh = *reinterpret_cast<VoteShared *>(acc + misc_words);
   const int tid = threadIdx.x, wv = tid / WAVE, lane = tid % WAVE;
   const uint64_t lt = lanemask_lt();
   // per-block scratch: hits, hits in bucket order, slot of each hit, groups
@@ -312,7 +358,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
   for (;;) {
     if (tid == 0) {
       const unsigned long long it = atomicAdd(queue, 1ull);
-      s_item = it;
+      sh.item = it;
       if (it < n_items) {
         int lo = 0, hi = n_jobs - 1;
         while (lo < hi) {
@@ -322,14 +368,14 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
           else
             hi = mid - 1;
         }
-        s_job = lo;
+        sh.job = lo;
       }
-      s_nh = 0;
+      sh.nh = 0;
     }
     __syncthreads();
-    const unsigned long long item = s_item;
+    const unsigned long long item = sh.item;
     if (item >= n_items) break;
-    const VoteJob &J = jobs[s_job];
+    const VoteJob &J = jobs[sh.job];
     const int32_t n_s = J.n_s;
     const uint32_t r = (uint32_t)(item - J.item_begin) + (uint32_t)J.r_begin;
     const int32_t s_r = (int32_t)r * J.rpdf;
@@ -371,7 +417,8 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
             } else {
               const float phi = ppfm::atan2f_own(vz, vy);
               const double turns = (double)phi * (4294967296.0 / 6.283185307179586);
-              v32 = (uint32_t)(long long)llrint(turns) + 0x80000000u;
+              // 24-bit turns of phi_v + pi
+              v32 = ((uint32_t)(long long)llrint(turns) + 0x80000000u) >> 8;
             }
             hr = make_uint4(u | flag, v32, ppfm::fbits(vy), ppfm::fbits(vz));
           }
@@ -379,12 +426,12 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       }
       const uint64_t m = __ballot(hit);
       uint32_t base = 0;
-      if (lane == 0 && m) base = atomicAdd(&s_nh, (uint32_t)__popcll(m));
+      if (lane == 0 && m) base = atomicAdd(&sh.nh, (uint32_t)__popcll(m));
       base = __shfl(base, 0, WAVE);
       if (hit) hits[base + (uint32_t)__popcll(m & lt)] = hr;
     }
     __syncthreads();
-    const uint32_t nh = s_nh;
+    const uint32_t nh = sh.nh;
 
     // ---- phase 1b: group hits by model bucket.  Partners of one reference
     // point hit few distinct buckets (tens of hits per bucket is common), so
@@ -396,7 +443,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       tkey[i] = 0u;
       tcnt[i] = 0u;
     }
-    if (tid == 0) s_ovf = 0u;
+    if (tid == 0) sh.ovf = 0u;
     __syncthreads();
     for (uint32_t h = tid; h < nh; h += VOTE_BLOCK) {
       const uint32_t u = gld(reinterpret_cast<const uint32_t *>(hits + h)) & 0x7fffffffu;
@@ -412,13 +459,13 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
         slot = (slot + 1u) & (GT_SIZE - 1);
       }
       if (slot == 0xffffffffu)
-        s_ovf = 1u;
+        sh.ovf = 1u;
       else
         atomicAdd(&tcnt[slot], 1u);
       slotid[h] = slot;
     }
     __syncthreads();
-    const bool identity = s_ovf != 0u;  // table full: one group per hit
+    const bool identity = sh.ovf != 0u;  // table full: one group per hit
     if (!identity) {
       // exclusive scans of the slot counts (group starts) and of the used
       // slots (group ids); each thread owns GT_SIZE / VOTE_BLOCK slots
@@ -432,17 +479,17 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       }
       const uint32_t ci = wave_incl_scan_u32(cs), ui = wave_incl_scan_u32(us);
       if (lane == WAVE - 1) {
-        s_wsum[wv][0] = ci;
-        s_wsum[wv][1] = ui;
+        sh.wsum[wv][0] = ci;
+        sh.wsum[wv][1] = ui;
       }
       __syncthreads();
       uint32_t cb = ci - cs, ub = ui - us, utot = 0;
       for (int k = 0; k < VOTE_BLOCK / WAVE; ++k) {
         if (k < wv) {
-          cb += s_wsum[k][0];
-          ub += s_wsum[k][1];
+          cb += sh.wsum[k][0];
+          ub += sh.wsum[k][1];
         }
-        utot += s_wsum[k][1];
+        utot += sh.wsum[k][1];
       }
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
@@ -453,34 +500,40 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
         tcnt[tid * PER + q] = cb;  // cursor
         cb += c[q];
       }
-      if (tid == 0) s_ngroups = utot;
+      if (tid == 0) sh.ngroups = utot;
       __syncthreads();
       for (uint32_t h = tid; h < nh; h += VOTE_BLOCK) {
         const uint32_t pos = atomicAdd(&tcnt[slotid[h]], 1u);
         sorted[pos] = gld4(hits + h);
       }
     } else if (tid == 0) {
-      s_ngroups = nh;
+      sh.ngroups = nh;
     }
     __syncthreads();
-    const uint32_t ngroups = s_ngroups;
+    const uint32_t ngroups = sh.ngroups;
     const uint4 *ghits = identity ? hits : sorted;
 
     // ---- phase 2: per model-point tile, accumulate then emit
     unsigned long long wave_votes = 0;
     const int32_t T = J.T, TM = J.TM, n_m = J.n_m;
+    // bin-major accumulator acc[bin * pitch + m], pitch = 2^k + 1 with 2^k a
+    // multiple of 64: the LDS bank of an update is (m + bin) mod 64.  LDS byte
+    // addresses: accumulator base, and 64 spare words (one per lane) past it.
+    const uint32_t pitch = (uint32_t)J.acc_pitch, pitch4 = pitch * 4u;
+    const uint32_t acc_base = (uint32_t)(uintptr_t)acc;
+    const int32_t nb = N_BINS * (int32_t)pitch;
+    const uint32_t spare4 = acc_base + ((uint32_t)nb + (uint32_t)lane) * 4u;
+    const uint32_t one = 1u;
     const uint32_t *__restrict__ toff = J.toff;
     const uint32_t *__restrict__ rec = J.rec;
     for (int32_t t = 0; t < T; ++t) {
       const int32_t tile_base = t * TM;
-      const int32_t tm = min(TM, n_m - tile_base);
-      const int32_t nb = tm * N_BINS;
       for (int32_t i = tid; i < nb; i += VOTE_BLOCK) acc[i] = 0u;
-      if (tid == 0) s_next = 0;
+      if (tid == 0) sh.next = 0;
       __syncthreads();
       for (;;) {
         uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(&s_next, 1u);
+        if (lane == 0) g = atomicAdd(&sh.next, 1u);
         g = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(g, 0, WAVE));
         if (g >= ngroups) break;
         uint32_t gs = g, gc = 1;
@@ -495,36 +548,62 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
         if (b >= e) continue;
         wave_votes += (unsigned long long)(e - b) * gc;
         for (uint32_t s0 = 0; s0 < gc; s0 += WAVE) {
-          const uint32_t sc = min((uint32_t)WAVE, gc - s0);
+          const uint32_t sc =
+              (uint32_t)__builtin_amdgcn_readfirstlane((int)min((uint32_t)WAVE, gc - s0));
           uint4 my = make_uint4(0u, 0u, 0u, 0u);
           if ((uint32_t)lane < sc) my = gld4(ghits + gs + s0 + lane);
           for (uint32_t j0 = b + lane; j0 < e; j0 += VOTE_UNROLL * WAVE) {
-            uint32_t rc[VOTE_UNROLL];
+            // per record: phi_u in 24-bit turns, byte address of its
+            // accumulator column, and the fast-path range (0: never fast)
+            uint32_t ph[VOTE_UNROLL], row4[VOTE_UNROLL], rg[VOTE_UNROLL];
+            uint64_t vmask[VOTE_UNROLL];
 #pragma unroll
             for (int q = 0; q < VOTE_UNROLL; ++q) {
               const uint32_t j = j0 + q * WAVE;
-              rc[q] = j < e ? gld(rec + j) : 0u;
+              const uint32_t r = j < e ? gld(rec + j) : REC_DUMMY_BIT;
+              const bool valid = (r & REC_DUMMY_BIT) == 0u;
+              ph[q] = (r >> 8) & 0xfffff0u;
+              row4[q] = acc_base + ((r & REC_M_MASK) << 2);
+              rg[q] = (valid && !(r & REC_SLOW_BIT)) ? FAST_RANGE : 0u;
+              vmask[q] = __ballot(valid);
             }
             for (uint32_t k = 0; k < sc; ++k) {
               const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)my.x, (int)k);
-              const uint32_t v32 = (uint32_t)__builtin_amdgcn_readlane((int)my.y, (int)k);
-              const bool slow_all = EXACT || (hx >> 31) != 0u;
+              const uint32_t v24 = (uint32_t)__builtin_amdgcn_readlane((int)my.y, (int)k);
+              uint64_t bad = 0;
+              if (EXACT || (hx >> 31) != 0u) {
+                // degenerate v (or exact mode): every vote of this hit exact
 #pragma unroll
-              for (int q = 0; q < VOTE_UNROLL; ++q) {
-                const uint32_t j = j0 + q * WAVE;
-                if (j < e) {
-                  uint32_t bin = 0, mloc = rc[q] & REC_M_MASK;
-                  bool ok = false;
-                  if (!slow_all && !(rc[q] & REC_SLOW_BIT)) {
-                    const uint32_t a32 = v32 - (rc[q] & REC_PHI_MASK);
-                    const float x = (float)a32 * BIN_SCALE;
-                    bin = (uint32_t)x;
-                    const float fr = x - (float)bin;
-                    ok = fabsf(fr - 0.5f) < 0.5f - FAST_MARGIN_BINS;
+                for (int q = 0; q < VOTE_UNROLL; ++q) bad |= vmask[q];
+              } else {
+#pragma unroll
+                for (int q = 0; q < VOTE_UNROLL; ++q) {
+                  // a = 24-bit turns of theta + pi; a * 30 * 2^8 = (bin << 32) | frac
+                  const uint32_t a = v24 - ph[q];
+                  const bool ok = (a & 0xffffffu) * (30u << 8) - FAST_OFF < rg[q];
+                  const uint64_t okm = __ballot(ok);
+                  bad |= ~okm & vmask[q];
+                  // undecided lanes add to the spare word past the accumulator
+                  uint32_t bin, addr;
+                  asm("v_mul_hi_u32_u24 %0, 0x1e00, %1" : "=v"(bin) : "v"(a));
+                  addr = row4[q] + __umul24(bin, pitch4);
+                  asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(addr) : "v"(spare4), "v"(addr), "s"(okm));
+                  asm volatile("ds_add_u32 %0, %1" : : "v"(addr), "v"(one) : "memory");
+                }
+              }
+              if (bad) {
+                // rare: the exact float pipeline for the undecided votes
+                const float vy = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.z, (int)k));
+                const float vz = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.w, (int)k));
+#pragma unroll
+                for (int q = 0; q < VOTE_UNROLL; ++q) {
+                  bool need = ((vmask[q] >> lane) & 1u) != 0u;
+                  if (need && !EXACT && (hx >> 31) == 0u) {
+                    const uint32_t a = (v24 - ph[q]) & 0xffffffu;
+                    need = !(a * (30u << 8) - FAST_OFF < rg[q]);
                   }
-                  if (!ok) {
-                    const float vy = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.z, (int)k));
-                    const float vz = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.w, (int)k));
+                  if (need) {
+                    const uint32_t j = j0 + q * WAVE;
                     const uint32_t pr = gld(J.pair + j);
                     const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
                     const float *Fm = J.mframe + (size_t)m_r * 16;
@@ -537,10 +616,10 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
                     float uy, uz;
                     ppfg::frame_apply_yz(r1, r2, gld(J.mx + m_i), gld(J.my + m_i), gld(J.mz + m_i),
                                          uy, uz);
-                    bin = ppfg::alpha_bin(uy, uz, vy, vz);
-                    mloc = m_r - (uint32_t)tile_base;
+                    const uint32_t bin = ppfg::alpha_bin(uy, uz, vy, vz);
+                    const uint32_t mloc = m_r - (uint32_t)tile_base;
+                    atomicAdd(&acc[bin * pitch + mloc], 1u);
                   }
-                  atomicAdd(&acc[__umul24(mloc, (uint32_t)N_BINS) + bin], 1u);
                 }
               }
             }
@@ -552,17 +631,17 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       uint32_t lm = 0;
       for (int32_t i = tid; i < nb; i += VOTE_BLOCK) lm = max(lm, acc[i]);
       lm = wave_max_u32(lm);
-      if (lane == 0) s_wmax[wv] = lm;
+      if (lane == 0) sh.wmax[wv] = lm;
       __syncthreads();
       if (tid == 0) {
         uint32_t m = 0;
-        for (int k = 0; k < VOTE_BLOCK / WAVE; ++k) m = max(m, s_wmax[k]);
+        for (int k = 0; k < VOTE_BLOCK / WAVE; ++k) m = max(m, sh.wmax[k]);
         uint32_t g = atomicMax(J.max_count, m);
         g = max(g, m);
-        s_thr = J.thr * (float)g;
+        sh.thr = J.thr * (float)g;
       }
       __syncthreads();
-      const float thr_lb = s_thr;
+      const float thr_lb = sh.thr;
       const int32_t nb_r = (nb + VOTE_BLOCK - 1) / VOTE_BLOCK * VOTE_BLOCK;
       for (int32_t i = tid; i < nb_r; i += VOTE_BLOCK) {
         const uint32_t c = i < nb ? acc[i] : 0u;
@@ -574,8 +653,8 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
         if (emit) {
           const uint32_t pos = base + (uint32_t)__popcll(m & lt);
           if (pos < J.cand_cap) {
-            const uint32_t mr = (uint32_t)tile_base + (uint32_t)(i / N_BINS);
-            const uint32_t a = (uint32_t)(i % N_BINS);
+            const uint32_t a = (uint32_t)i / pitch;
+            const uint32_t mr = (uint32_t)tile_base + ((uint32_t)i - a * pitch);
             J.cand_code[pos] = ((unsigned long long)(uint32_t)s_r << 32) | (unsigned long long)((mr << 6) | a);
             J.cand_count[pos] = c;
           }
@@ -1152,13 +1231,28 @@ int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcou
   return 0;
 }
 
-int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const float *mframe,
-               const CloudDev &m, int32_t TM) {
+int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const CloudDev &m,
+               int32_t TM, int32_t T, const uint32_t *hfirst, const uint32_t *hcount, int64_t U,
+               uint32_t *toff) {
   if (N <= 0) return 0;
   hipLaunchKernelGGL(k_vote_order_words, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, vals, N,
-                     mframe, m.soa.as<float>(), (int32_t)m.n, TM);
+                     (int32_t)m.n, TM);
   PPF_HIP(hipGetLastError());
-  return radix_sort_u64(words, vals, N, 0, 64, ctx->sort, ctx->stream);
+  PPF_TRY(radix_sort_u64(words, vals, N, 19, 64, ctx->sort, ctx->stream));
+  if (U <= 0) return 0;
+  // tile offsets (the same after the second sort: it reorders within segments)
+  hipLaunchKernelGGL(k_tile_offsets, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, hfirst, hcount, vals,
+                     U, (int32_t)m.n, TM, T, toff);
+  PPF_HIP(hipGetLastError());
+  // ranks read other records' words, so write the new words to a copy
+  PPF_TRY(ctx->tmp_a.ensure((size_t)N * 8));
+  uint64_t *w2 = ctx->tmp_a.as<uint64_t>();
+  PPF_HIP(hipMemcpyAsync(w2, words, (size_t)N * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_vote_rank_words2, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, w2, hfirst,
+                     U, toff, T, N);
+  PPF_HIP(hipGetLastError());
+  PPF_TRY(radix_sort_u64(w2, vals, N, 0, 64, ctx->sort, ctx->stream));
+  return 0;
 }
 
 int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
@@ -1170,11 +1264,22 @@ int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mfr
   return 0;
 }
 
+// Words of dynamic LDS before VoteShared: the largest accumulator (31 rows of
+// pitch 2^k + 1, 2^k >= 64) plus 64 spare words, or the hit-grouping table,
+// rounded to 8 bytes.
+uint32_t vote_misc_words(int max_tm) {
+  uint32_t pw = 64;
+  while ((int)pw < max_tm) pw <<= 1;
+  const uint32_t w = std::max((pw + 1u) * N_BINS + (uint32_t)WAVE, (uint32_t)2 * GT_SIZE);
+  return (w + 1u) & ~1u;
+}
+
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
               int64_t max_ns, bool exact, float *vote_ms) {
   *vote_ms = 0.0f;
   if (n_items == 0 || n_jobs <= 0) return 0;
-  const size_t lds = std::max((size_t)max_tm * N_BINS * 4, (size_t)2 * GT_SIZE * 4);
+  const uint32_t misc_words = vote_misc_words(max_tm);
+  const size_t lds = (size_t)misc_words * 4 + sizeof(VoteShared);
   if (lds > ctx->max_lds) {
     set_error("run_votes: accumulator tile exceeds the LDS limit");
     return PPF_ERR_ARG;
@@ -1197,7 +1302,8 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
   PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream,
                      ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items,
-                     ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4);
+                     ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4,
+                     misc_words);
   PPF_HIP(hipGetLastError());
   PPF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   PPF_HIP(hipEventSynchronize(ctx->ev1));
@@ -1206,7 +1312,8 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
 }
 
 int vote_slots(ppf_ctx ctx, int max_tm, int *slots) {
-  const size_t lds = std::max((size_t)max_tm * N_BINS * 4, (size_t)2 * GT_SIZE * 4);
+  const uint32_t misc_words = vote_misc_words(max_tm);
+  const size_t lds = (size_t)misc_words * 4 + sizeof(VoteShared);
   auto kern = k_vote<false>;
   PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;

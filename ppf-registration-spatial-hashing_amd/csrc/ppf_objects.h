@@ -8,13 +8,14 @@
 #include "../../include/ppf_amd.h"
 #include "ppf_internal.h"
 
-// Vote records: bits 0..10 model point within its tile, bit 11 "evaluate with
-// the full float path", bits 12..31 the angle phi_u = atan2(u.z, u.y) of the
-// model pair in units of 2*pi/2^20.
-constexpr uint32_t REC_M_MASK = 0x7ffu;
+// Vote records: bits 0..8 model point within its tile, bit 10 "no record",
+// bit 11 "evaluate with the full float path", bits 12..31 the angle
+// phi_u = atan2(u.z, u.y) of the model pair in units of 2*pi/2^20.
+constexpr uint32_t REC_M_MASK = 0x1ffu;
+constexpr uint32_t REC_DUMMY_BIT = 0x400u;  // no record (lanes past a segment's end)
 constexpr uint32_t REC_SLOW_BIT = 0x800u;
 constexpr uint32_t REC_PHI_MASK = 0xfffff000u;
-constexpr int MAX_TILE_POINTS = 2048;
+constexpr int MAX_TILE_POINTS = 512;
 constexpr int DEFAULT_TILE_POINTS = 512;
 constexpr int N_BINS = 31;
 constexpr int VOTE_BLOCK = 1024;
@@ -116,9 +117,10 @@ struct ppf_model_s {
   TBuf hcount;   // [U]
   TBuf hfirst;   // [U]
   TBuf pair;     // [n*n] pair index m_r*n + m_i, sorted by (key, pair): hashkeyToDataMap
-  TBuf vpair;    // [n*n] pair index in vote order (key, tile, phi_u)
+  int32_t acc_pitch = 0;  // accumulator row pitch in words: 2^k + 1 with 2^k >= TM, >= 64
+  TBuf vpair;    // [n*n] pair index in vote order (key, tile, rank in class, class)
   TBuf rec;      // [n*n] packed vote records, vote order
-  TBuf toff;     // [U][T+1] absolute start of each tile's sub-range
+  TBuf toff;     // [U][T+1] start of each tile's sub-range
   TBuf htab;     // [hcap][2] (key, u) open addressing, key 0 = empty
   TBuf weights;  // [n]
 };
@@ -155,7 +157,7 @@ struct VoteJob {
   const uint32_t *pair;
   const float *mframe;
   const float *mx, *my, *mz;
-  int32_t n_m, TM, T, pad1;
+  int32_t n_m, TM, T, acc_pitch;
   float d_dist, thr;
   uint32_t *max_count;
   unsigned long long *cand_code;
@@ -184,8 +186,11 @@ int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcou
                        uint32_t *toff);
 // words (key << 32 | pair, sorted by key) -> vote order (key, tile, phi_u);
 // vals receives the pair index of each entry in that order.
-int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const float *mframe,
-               const CloudDev &m, int32_t TM);
+// Vote order of the model pairs (vals receives the pair indices, toff the
+// [U][T+1] start of every bucket's tile sub-range); words are sorted by key.
+int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const CloudDev &m,
+               int32_t TM, int32_t T, const uint32_t *hfirst, const uint32_t *hcount, int64_t U,
+               uint32_t *toff);
 int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
                   const CloudDev &m, int32_t TM, uint32_t *rec);
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
