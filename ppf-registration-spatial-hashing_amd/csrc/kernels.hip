@@ -242,10 +242,10 @@ __global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const fl
     slow = REC_SLOW_BIT;
   } else {
     const double phi = atan2((double)uz, (double)uy);  // (-pi, pi]
-    const double turns = phi * (1048576.0 / 6.283185307179586);
-    q = (uint32_t)(int32_t)llrint(turns) & 0xfffffu;
+    const double turns = phi * (4194304.0 / 6.283185307179586);
+    q = (uint32_t)(int32_t)llrint(turns) & 0x3fffffu;
   }
-  rec[j] = (q << 12) | slow | (m_r % (uint32_t)TM);
+  rec[j] = (q << REC_PHI_SHIFT) | slow | (m_r % (uint32_t)TM);
 }
 
 // Vote-record order.  Records are sorted by (key, tile, m_r mod 64), stably
@@ -303,24 +303,26 @@ __global__ void k_vote_rank_words2(const uint64_t *__restrict__ words, uint64_t 
 // --------------------------------------------------------------- voting
 // Fast alpha bin: with theta = phi_v - phi_u the exact angle between the y/z
 // parts of u and v, the float pipeline of alpha_bin() yields
-// floor((theta + pi_f)/D) except within ~1.2e-6 rad of a bin edge (rounding of
-// cross/dot, atan2f error <= 3.2 ulp, rounding of alpha + pi).  The fast path
-// works in 24-bit turns: a = (v24 - phi24) mod 2^24 with v24 = phi_v + pi and
-// phi24 = phi_u, then a * 30 * 2^8 = (bin << 32) | frac exactly (the 24 x 24
-// bit multiplies v_mul_u32_u24 / v_mul_hi_u32_u24).  phi_u carries <= 3e-6
-// rad of quantization (20 bits), phi_v the float atan2 error (<= 8e-7 rad),
-// the truncation to 24 bits < 3.7e-7 rad, and nominal edges k*2pi/30 differ
-// from the true ones by < 3e-7 rad: < 5.5e-6 rad = 2.7e-5 bin in all.
-// Whenever frac is farther than FAST_MARGIN_BINS (1.2e-4 bin = 2.5e-5 rad)
-// from an edge the bin is decided; otherwise (and for degenerate u or v) the
-// exact float pipeline runs.  The wrap point theta = +-pi (position 0 or 30)
-// is always within the margin, so bin 30 only comes from the exact path.
-constexpr float FAST_MARGIN_BINS = 1.2e-4f;
+// floor((theta + pi_f)/D) except within 1.13e-6 rad of a bin edge (rounding
+// of cross/dot 1.2e-7, atan2f error <= 3.2 ulp = 7.7e-7, rounding of
+// alpha + pi 2.4e-7).  The fast path works in 24-bit turns: a = (v24 - phi24)
+// mod 2^24 with v24 = phi_v + pi and phi24 = phi_u, then
+// a * 30 * 2^8 = (bin << 32) | frac exactly (the 24 x 24 bit multiplies
+// v_mul_u32_u24 / v_mul_hi_u32_u24).  phi_u carries <= 7.5e-7 rad of
+// quantization (22 bits), phi_v the float atan2 error (<= 7.7e-7 rad), the
+// truncation to 24 bits < 3.7e-7 rad, and nominal edges k*2pi/30 differ from
+// the float pipeline's k*D - pi_f by < 2.6e-7 rad: < 3.3e-6 rad = 1.6e-5 bin
+// in all.  Whenever frac is farther than FAST_MARGIN_BINS (4e-5 bin =
+// 8.4e-6 rad) from an edge the bin is decided; otherwise (and for degenerate u
+// or v) the exact float pipeline runs.  The wrap point theta = +-pi (position
+// 0 or 30) is always within the margin, so bin 30 only comes from the exact
+// path.
+constexpr float FAST_MARGIN_BINS = 4e-5f;
 // frac (as a 32-bit fraction) is decided when it lies in [OFF, 2^32 - OFF]
-// with OFF = floor(1.2e-4 * 2^32) + 1, i.e. when frac - OFF < RANGE (mod 2^32)
-constexpr uint32_t FAST_OFF = 515397u;
+// with OFF = floor(4e-5 * 2^32) + 1, i.e. when frac - OFF < RANGE (mod 2^32)
+constexpr uint32_t FAST_OFF = 171799u;
 constexpr uint32_t FAST_RANGE = (uint32_t)(4294967296ull - 2ull * (FAST_OFF - 1u) - 1ull);
-static_assert(FAST_OFF - 1u == (uint32_t)(1.2e-4 * 4294967296.0), "fast-bin margin");
+static_assert(FAST_OFF - 1u == (uint32_t)(4e-5 * 4294967296.0), "fast-bin margin");
 constexpr int VOTE_UNROLL = 4;
 constexpr int GT_LOG2 = 12;
 constexpr int GT_SIZE = 1 << GT_LOG2;  // hit-grouping table slots (2 x 16 KB of LDS)
@@ -560,9 +562,9 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
 #pragma unroll
             for (int q = 0; q < VOTE_UNROLL; ++q) {
               const uint32_t j = j0 + q * WAVE;
-              const uint32_t r = j < e ? gld(rec + j) : REC_DUMMY_BIT;
-              const bool valid = (r & REC_DUMMY_BIT) == 0u;
-              ph[q] = (r >> 8) & 0xfffff0u;
+              const bool valid = j < e;
+              const uint32_t r = valid ? gld(rec + j) : 0u;
+              ph[q] = (r >> (REC_PHI_SHIFT - 2)) & 0xfffffcu;  // phi_u in 24-bit turns
               row4[q] = acc_base + ((r & REC_M_MASK) << 2);
               rg[q] = (valid && !(r & REC_SLOW_BIT)) ? FAST_RANGE : 0u;
               vmask[q] = __ballot(valid);

@@ -8,13 +8,12 @@
 #include "../../include/ppf_amd.h"
 #include "ppf_internal.h"
 
-// Vote records: bits 0..8 model point within its tile, bit 10 "no record",
-// bit 11 "evaluate with the full float path", bits 12..31 the angle
-// phi_u = atan2(u.z, u.y) of the model pair in units of 2*pi/2^20.
+// Vote records: bits 0..8 model point within its tile, bit 9 "evaluate with
+// the full float path", bits 10..31 the angle phi_u = atan2(u.z, u.y) of the
+// model pair in units of 2*pi/2^22.
 constexpr uint32_t REC_M_MASK = 0x1ffu;
-constexpr uint32_t REC_DUMMY_BIT = 0x400u;  // no record (lanes past a segment's end)
-constexpr uint32_t REC_SLOW_BIT = 0x800u;
-constexpr uint32_t REC_PHI_MASK = 0xfffff000u;
+constexpr uint32_t REC_SLOW_BIT = 0x200u;
+constexpr uint32_t REC_PHI_SHIFT = 10;
 constexpr int MAX_TILE_POINTS = 512;
 constexpr int DEFAULT_TILE_POINTS = 512;
 constexpr int N_BINS = 31;
