@@ -165,6 +165,38 @@ int ppf_registration(ppf_ctx ctx, int32_t n_scenes, const float *const *scene_po
                      int32_t use_averaged_clusters, const float *const *model_weights,
                      float *poses_out, int32_t *status_out);
 
+/* ------------------------------------------------------------ pre-processing
+ * Voxel-grid downsampling as the alignment driver applies it before
+ * registration (pcl::VoxelGrid with leaf (l, l, l), src/alignment.cpp:78-87,
+ * 259-288: the scene with scene_leaf_size, each model with its d_dist).
+ * Points with non-finite x, y or z are dropped; each occupied voxel becomes
+ * the average of its points (positions and normals), voxels in ascending
+ * voxel index; a leaf too small for the cloud's extent (more than INT32_MAX
+ * voxels) returns the input unchanged, as PCL does.  points_out/normals_out
+ * (float[n*3] suffices) may be NULL to get only the count. */
+int ppf_voxel_downsample(ppf_ctx ctx, const float *points, const float *normals, int64_t n, float leaf,
+                         float *points_out, float *normals_out, int64_t *n_out);
+/* max - min per axis over the finite points (pcl::getMinMax3D,
+ * src/alignment.cpp:246-257); the driver's d_dist is tau_d * the largest. */
+int ppf_bbox_extent(const float *points, int64_t n, float extent_out[3]);
+
+/* ------------------------------------------------ files and validation (host)
+ * PLY point clouds with normals as the driver loads them
+ * (pcl::io::loadPLYFile<PointNormal>, src/alignment.cpp:207-244): the vertex
+ * element's x y z and nx ny nz (or normal_x ...), ascii or binary, any
+ * numeric property types; other properties and elements are skipped and
+ * missing normals read as 0.  Call with NULL buffers to get *n_out first. */
+int ppf_ply_read(const char *path, float *points, float *normals, int64_t capacity, int64_t *n_out);
+/* Writes x y z (and nx ny nz when normals is not NULL) as float properties. */
+int ppf_ply_write(const char *path, const float *points, const float *normals, int64_t n, int32_t binary);
+/* 16 whitespace-separated numbers, row-major (include/impl/util.hpp:95-104). */
+int ppf_xf_read(const char *path, float pose16[16]);
+/* ht_dist (src/cuda/linalg.cu:9-20): out2[0] = |t_a - t_b|, out2[1] = angle
+ * of rot_a^-1 rot_b in radians; the driver's validation accepts a pose when
+ * out2[0] < validation_translation_threshold * diameter and out2[1] <
+ * validation_rotation_threshold (src/alignment.cpp:317-323). */
+int ppf_ht_dist(const float a16[16], const float b16[16], float out2[2]);
+
 /* ----------------------------------------------------------------- batched
  * Many (model, scene) pairs in one call: the batched form of the loop in
  * ppf_registration (src/cuda/ppf.cu:63-98) over device-resident models.

@@ -913,6 +913,78 @@ int oracle_vote_throughput(const float *m_pts, const float *m_nrm, int64_t n_m,
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }
 
+// pcl::VoxelGrid<PointNormal>::applyFilter with leaf (l, l, l) and all fields
+// averaged, as the reference's voxelGridDownsample (alignment.cpp:78-87)
+// calls it: inverse leaf in float, min_b/max_b = floor(min/max * inv) over
+// finite points, INT32_MAX voxel limit (input returned), index
+// ijk . (1, dx, dx*dy), sort by index, float sums / count per voxel.  PCL's
+// std::sort leaves the order inside a voxel unspecified; this restatement
+// uses point order (stable sort).
+int oracle_voxel_downsample(const float *pts, const float *nrm, int64_t n, float leaf, float *out_pts,
+                            float *out_nrm, int64_t *n_out) {
+  *n_out = 0;
+  if (n <= 0) return 0;
+  const float inv = 1.0f / leaf;
+  bool any = false;
+  float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  for (int64_t i = 0; i < n; ++i) {
+    const float *p = pts + 3 * i;
+    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+    for (int d = 0; d < 3; ++d) {
+      if (!any || p[d] < mn[d]) mn[d] = p[d];
+      if (!any || p[d] > mx[d]) mx[d] = p[d];
+    }
+    any = true;
+  }
+  if (!any) return 0;
+  int64_t min_b[3], max_b[3];
+  for (int d = 0; d < 3; ++d) {
+    min_b[d] = (int32_t)std::floor(mn[d] * inv);
+    max_b[d] = (int32_t)std::floor(mx[d] * inv);
+  }
+  const int64_t dx = max_b[0] - min_b[0] + 1, dy = max_b[1] - min_b[1] + 1, dz = max_b[2] - min_b[2] + 1;
+  if (dx * dy * dz > (int64_t)INT32_MAX) {
+    std::memcpy(out_pts, pts, (size_t)n * 12);
+    std::memcpy(out_nrm, nrm, (size_t)n * 12);
+    *n_out = n;
+    return 0;
+  }
+  std::vector<std::pair<int64_t, int64_t>> idx;  // (voxel index, point)
+  for (int64_t i = 0; i < n; ++i) {
+    const float *p = pts + 3 * i;
+    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+    int64_t ijk[3];
+    for (int d = 0; d < 3; ++d) ijk[d] = (int32_t)std::floor(p[d] * inv) - min_b[d];
+    idx.push_back({ijk[0] + ijk[1] * dx + ijk[2] * dx * dy, i});
+  }
+  std::stable_sort(idx.begin(), idx.end(),
+                   [](const std::pair<int64_t, int64_t> &a, const std::pair<int64_t, int64_t> &b) {
+                     return a.first < b.first;
+                   });
+  int64_t v = 0;
+  for (size_t a = 0; a < idx.size();) {
+    size_t b = a;
+    float c[6] = {0, 0, 0, 0, 0, 0};
+    while (b < idx.size() && idx[b].first == idx[a].first) {
+      const int64_t i = idx[b].second;
+      for (int d = 0; d < 3; ++d) {
+        c[d] += pts[3 * i + d];
+        c[3 + d] += nrm[3 * i + d];
+      }
+      ++b;
+    }
+    const float cnt = (float)(b - a);
+    for (int d = 0; d < 3; ++d) {
+      out_pts[3 * v + d] = c[d] / cnt;
+      out_nrm[3 * v + d] = c[3 + d] / cnt;
+    }
+    ++v;
+    a = b;
+  }
+  *n_out = v;
+  return 0;
+}
+
 void *oracle_model_create(const float *m_pts, const float *m_nrm, int64_t n_m, float d_dist) {
   if (n_m < 1 || n_m >= 65536) return nullptr;
   return new ModelDesc(build_model(m_pts, m_nrm, n_m, d_dist));

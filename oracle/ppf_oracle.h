@@ -88,6 +88,8 @@ int oracle_vote_throughput(const float *m_pts, const float *m_nrm, int64_t n_m,
                            double *seconds);
 
 int oracle_num_threads(void);
+int oracle_voxel_downsample(const float *pts, const float *nrm, int64_t n, float leaf, float *out_pts,
+                            float *out_nrm, int64_t *n_out);
 
 /* ---- model handle for repeated timing calls (the hash array is built once) */
 void *oracle_model_create(const float *m_pts, const float *m_nrm, int64_t n_m, float d_dist);

@@ -63,6 +63,8 @@ def lib():
                                              C.c_float, _i64p, C.c_int64, _i64p,
                                              C.POINTER(C.c_double)]
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_voxel_downsample.argtypes = [_f32p, _f32p, C.c_int64, C.c_float, _f32p, _f32p,
+                                              C.POINTER(C.c_int64)]
         L.oracle_numerics.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_int64]
         L.oracle_model_create.argtypes = [_f32p, _f32p, C.c_int64, C.c_float]
         L.oracle_model_create.restype = C.c_void_p
@@ -254,3 +256,14 @@ class OracleModel:
 
 def num_threads() -> int:
     return int(lib().oracle_num_threads())
+
+
+def voxel_downsample(points, normals, leaf: float):
+    """pcl::VoxelGrid restatement (alignment.cpp:78-87): (points, normals)."""
+    p = _c(np.asarray(points, np.float32).reshape(-1, 3), np.float32)
+    n = _c(np.asarray(normals, np.float32).reshape(-1, 3), np.float32)
+    op = np.zeros_like(p)
+    on = np.zeros_like(n)
+    k = C.c_int64()
+    assert lib().oracle_voxel_downsample(p, n, len(p), float(leaf), op, on, C.byref(k)) == 0
+    return op[:k.value].copy(), on[:k.value].copy()

@@ -123,6 +123,13 @@ def lib():
     L.ppf_registration.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp, _vp,
                                    C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.c_int32, _vp,
                                    _f32p, _vp]
+    L.ppf_voxel_downsample.argtypes = [_vp, _f32p, _f32p, C.c_int64, C.c_float, _vp, _vp,
+                                       C.POINTER(C.c_int64)]
+    L.ppf_bbox_extent.argtypes = [_f32p, C.c_int64, _f32p]
+    L.ppf_ply_read.argtypes = [C.c_char_p, _vp, _vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.ppf_ply_write.argtypes = [C.c_char_p, _f32p, _vp, C.c_int64, C.c_int32]
+    L.ppf_xf_read.argtypes = [C.c_char_p, _f32p]
+    L.ppf_ht_dist.argtypes = [_f32p, _f32p, _f32p]
     L.ppf_batch_lookup.argtypes = [_vp, C.c_int32, _vp, _vp, C.POINTER(LookupParams), _vp,
                                    C.POINTER(BatchInfo)]
     L.ppf_server_create.argtypes = [_vp, C.c_int32, _vp, C.POINTER(ServerConfig), C.POINTER(_vp)]
@@ -398,6 +405,62 @@ class Model(_Owned):
     def getTransformations(self) -> np.ndarray:
         return self.result.transformations()
 
+
+
+def voxel_downsample(cloud, leaf: float, ctx: Optional[Context] = None):
+    """pcl::VoxelGrid downsampling with leaf (leaf, leaf, leaf) on the device
+    (the reference driver's voxelGridDownsample, src/alignment.cpp:78-87):
+    returns (points, normals) of the occupied voxels' averages."""
+    ctx = ctx or default_context()
+    p, n = _cloud(*cloud)
+    op = np.zeros(max(len(p), 1) * 3, np.float32)
+    on = np.zeros(max(len(p), 1) * 3, np.float32)
+    k = C.c_int64()
+    _check(lib().ppf_voxel_downsample(ctx.handle, p, n, len(p), float(leaf), op.ctypes.data,
+                                      on.ctypes.data, C.byref(k)))
+    return op[:3 * k.value].reshape(-1, 3).copy(), on[:3 * k.value].reshape(-1, 3).copy()
+
+
+def bbox_extent(points) -> np.ndarray:
+    """max - min per axis over finite points (pcl::getMinMax3D, alignment.cpp:246-257)."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(3, np.float32)
+    _check(lib().ppf_bbox_extent(p, len(p), out))
+    return out
+
+
+def read_ply(path: str):
+    """(points[n,3], normals[n,3]) of a PLY file (pcl::io::loadPLYFile into
+    PointNormal, src/alignment.cpp:207-244)."""
+    b = os.fsencode(path)
+    n = C.c_int64()
+    _check(lib().ppf_ply_read(b, None, None, 0, C.byref(n)))
+    p = np.zeros((n.value, 3), np.float32)
+    nn = np.zeros((n.value, 3), np.float32)
+    _check(lib().ppf_ply_read(b, p.ctypes.data, nn.ctypes.data, n.value, C.byref(n)))
+    return p, nn
+
+
+def write_ply(path: str, points, normals=None, binary: bool = True):
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    nn = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+    _check(lib().ppf_ply_write(os.fsencode(path), p, _ptr(nn), len(p), int(binary)))
+
+
+def read_xf(path: str) -> np.ndarray:
+    """4x4 ground-truth pose of a .xf file (include/impl/util.hpp:95-104)."""
+    out = np.zeros(16, np.float32)
+    _check(lib().ppf_xf_read(os.fsencode(path), out))
+    return out.reshape(4, 4)
+
+
+def ht_dist(a, b):
+    """(translation distance, rotation angle in rad) between two poses
+    (src/cuda/linalg.cu:9-20)."""
+    out = np.zeros(2, np.float32)
+    _check(lib().ppf_ht_dist(np.ascontiguousarray(a, np.float32).reshape(16),
+                             np.ascontiguousarray(b, np.float32).reshape(16), out))
+    return float(out[0]), float(out[1])
 
 
 def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_threshold: float,
