@@ -82,6 +82,15 @@ int ppf_model_info(ppf_model model, ppf_model_info_t *info);
 int ppf_model_hash_array(ppf_model model, uint32_t *hashkeys, uint32_t *counts, uint32_t *first,
                          uint32_t *key2ppf);
 
+/* Save a built model (its device arrays, with a versioned header and a
+ * checksum) and load it on a context, for warm starts without rebuilding the
+ * hash array (the reference only dumps raw device vectors,
+ * include/impl/util.hpp:55-91).  Loading checks the version, the header's
+ * consistency, that the model's tiles fit this device's LDS, and the
+ * checksum. */
+int ppf_model_save(ppf_model model, const char *path);
+int ppf_model_load(ppf_ctx ctx, const char *path, ppf_model *out);
+
 /* Model::SetModelPointVoteWeights (src/cuda/model.cu:91-93): w[m_r] scales the
  * vote count of every kept vote of model point m_r before GPU clustering
  * (vote_weight_kernel, src/cuda/kernel.cu:766-782).  Default all 1. */

@@ -3,6 +3,7 @@
 // pose clustering of the reference's --cpu_clustering path.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <utility>
@@ -443,6 +444,138 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
   return 0;
 }
 }  // namespace ppfi
+
+// ---------------------------------------------------------- model files
+// Little-endian file: a 128-byte header, then the model's arrays in a fixed
+// order, with a 64-bit FNV-1a checksum of those bytes in the header.  The
+// arrays are this build's device layout (version 1), so a loaded model is
+// used as is, without rebuilding its hash array.
+namespace {
+
+constexpr char MODEL_MAGIC[8] = {'P', 'P', 'F', 'M', 'O', 'D', 'E', 'L'};
+constexpr uint32_t MODEL_VERSION = 1;
+
+struct ModelFileHeader {
+  char magic[8];
+  uint32_t version, header_bytes;
+  int64_t n_points;
+  float d_dist;
+  int32_t n_angle, TM, T, acc_pitch;
+  int32_t pad0;
+  int64_t U, N;
+  uint32_t hcap, hshift;
+  uint64_t payload_bytes, checksum;
+  uint8_t reserved[128 - 88];
+};
+static_assert(sizeof(ModelFileHeader) == 128, "model file header");
+
+uint64_t fnv1a64(uint64_t h, const void *data, size_t n) {
+  const unsigned char *p = static_cast<const unsigned char *>(data);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// (buffer, bytes) of every saved array, in file order
+std::vector<std::pair<TBuf *, size_t>> model_arrays(ppf_model m) {
+  const size_t n = (size_t)m->cloud.n, N = n * n, U = (size_t)m->U;
+  return {{&m->cloud.soa, n * 7 * 4}, {&m->frames, n * 64},           {&m->hkeys, U * 4},
+          {&m->hcount, U * 4},        {&m->hfirst, U * 4},            {&m->pair, N * 4},
+          {&m->vpair, N * 4},         {&m->rec, N * 4},               {&m->toff, U * (m->T + 1) * 4},
+          {&m->htab, (size_t)m->hcap * 8}, {&m->weights, n * 4}};
+}
+
+}  // namespace
+
+extern "C" {
+
+int ppf_model_save(ppf_model m, const char *path) {
+  ARG_CHECK(m && path, "ppf_model_save: NULL argument");
+  TRY(set_device(m->ctx));
+  HIPC(hipStreamSynchronize(m->ctx->stream));
+  ModelFileHeader h{};
+  std::memcpy(h.magic, MODEL_MAGIC, 8);
+  h.version = MODEL_VERSION;
+  h.header_bytes = sizeof(ModelFileHeader);
+  h.n_points = m->cloud.n;
+  h.d_dist = m->d_dist;
+  h.n_angle = 30;
+  h.TM = m->TM;
+  h.T = m->T;
+  h.acc_pitch = m->acc_pitch;
+  h.U = m->U;
+  h.N = m->cloud.n * m->cloud.n;
+  h.hcap = m->hcap;
+  h.hshift = m->hshift;
+  std::vector<std::vector<char>> host;
+  uint64_t ck = 14695981039346656037ull, bytes = 0;
+  for (auto &a : model_arrays(m)) {
+    host.emplace_back(a.second);
+    if (a.second) HIPC(hipMemcpy(host.back().data(), a.first->ptr, a.second, hipMemcpyDeviceToHost));
+    ck = fnv1a64(ck, host.back().data(), a.second);
+    bytes += a.second;
+  }
+  h.payload_bytes = bytes;
+  h.checksum = ck;
+  FILE *f = std::fopen(path, "wb");
+  ARG_CHECK(f, std::string("ppf_model_save: cannot open ") + path);
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  for (auto &b : host) ok = ok && (b.empty() || std::fwrite(b.data(), 1, b.size(), f) == b.size());
+  ok = (std::fclose(f) == 0) && ok;
+  ARG_CHECK(ok, std::string("ppf_model_save: write failed: ") + path);
+  return PPF_OK;
+}
+
+int ppf_model_load(ppf_ctx ctx, const char *path, ppf_model *out) {
+  ARG_CHECK(ctx && path && out, "ppf_model_load: NULL argument");
+  *out = nullptr;
+  FILE *f = std::fopen(path, "rb");
+  ARG_CHECK(f, std::string("ppf_model_load: cannot open ") + path);
+  std::unique_ptr<FILE, int (*)(FILE *)> fc(f, std::fclose);
+  ModelFileHeader h{};
+  ARG_CHECK(std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, MODEL_MAGIC, 8) == 0,
+            "ppf_model_load: not a model file");
+  ARG_CHECK(h.version == MODEL_VERSION && h.header_bytes == sizeof(ModelFileHeader),
+            "ppf_model_load: unsupported model file version " + std::to_string(h.version));
+  ARG_CHECK(h.n_angle == 30, "ppf_model_load: model quantized with another angle count");
+  ARG_CHECK(h.n_points >= 2 && h.n_points <= PPF_MAX_MODEL_POINTS && h.N == h.n_points * h.n_points &&
+                h.U >= 1 && h.U <= h.N && h.T >= 1 && h.TM >= 1 && h.TM <= MAX_TILE_POINTS &&
+                (int64_t)h.T * h.TM >= h.n_points && h.hcap >= 2 * h.U && (h.hcap & (h.hcap - 1)) == 0 &&
+                h.acc_pitch > h.TM && h.d_dist > 0.0f && h.hshift >= 1 && h.hshift <= 31 &&
+                (1ull << (32 - h.hshift)) == h.hcap,
+            "ppf_model_load: inconsistent model file header");
+  const size_t lds_points = (ctx->max_lds - 1024) / (N_BINS * 4) / 32 * 32;
+  ARG_CHECK((size_t)h.TM <= lds_points, "ppf_model_load: model tiles exceed this device's LDS");
+  TRY(set_device(ctx));
+  std::unique_ptr<ppf_model_s> m(new ppf_model_s());
+  m->ctx = ctx->root;
+  m->cloud.n = h.n_points;
+  m->d_dist = h.d_dist;
+  m->TM = h.TM;
+  m->T = h.T;
+  m->acc_pitch = h.acc_pitch;
+  m->U = h.U;
+  m->hcap = h.hcap;
+  m->hshift = h.hshift;
+  uint64_t ck = 14695981039346656037ull, bytes = 0;
+  std::vector<char> buf;
+  for (auto &a : model_arrays(m.get())) {
+    buf.resize(a.second);
+    ARG_CHECK(a.second == 0 || std::fread(buf.data(), 1, a.second, f) == a.second,
+              "ppf_model_load: file ends early");
+    ck = fnv1a64(ck, buf.data(), a.second);
+    bytes += a.second;
+    TRY(a.first->alloc(ctx, a.second));
+    HIPC(hipMemcpy(a.first->ptr, buf.data(), a.second, hipMemcpyHostToDevice));
+  }
+  ARG_CHECK(bytes == h.payload_bytes && ck == h.checksum, "ppf_model_load: checksum mismatch");
+  *out = m.release();
+  return PPF_OK;
+}
+
+}  // extern "C"
 
 extern "C" {
 

@@ -123,6 +123,8 @@ def lib():
     L.ppf_registration.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp, _vp,
                                    C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.c_int32, _vp,
                                    _f32p, _vp]
+    L.ppf_model_save.argtypes = [_vp, C.c_char_p]
+    L.ppf_model_load.argtypes = [_vp, C.c_char_p, C.POINTER(_vp)]
     L.ppf_voxel_downsample.argtypes = [_vp, _f32p, _f32p, C.c_int64, C.c_float, _vp, _vp,
                                        C.POINTER(C.c_int64)]
     L.ppf_bbox_extent.argtypes = [_f32p, C.c_int64, _f32p]
@@ -365,6 +367,32 @@ class Model(_Owned):
         self._destroy = "ppf_model_destroy"
         self._register(ctx)
         self.result: Optional[LookupResult] = None
+
+    def save(self, path: str):
+        """Write the built model (device arrays, versioned, checksummed)."""
+        _check(lib().ppf_model_save(self.handle, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path: str, vote_count_threshold: float = 0.4, cpu_clustering: bool = False,
+             use_l1_norm: bool = False, use_averaged_clusters: bool = False,
+             ctx: Optional[Context] = None) -> "Model":
+        """A model saved by save(), without rebuilding its hash array."""
+        ctx = ctx or default_context()
+        h = _vp()
+        _check(lib().ppf_model_load(ctx.handle, os.fsencode(path), C.byref(h)))
+        self = cls.__new__(cls)
+        self.handle = h
+        self._destroy = "ppf_model_destroy"
+        self._register(ctx)
+        self.vote_count_threshold = float(vote_count_threshold)
+        self.cpu_clustering = bool(cpu_clustering)
+        self.use_l1_norm = bool(use_l1_norm)
+        self.use_averaged_clusters = bool(use_averaged_clusters)
+        self.result = None
+        inf = self.info()
+        self.n = int(inf["n_points"])
+        self.d_dist = float(inf["d_dist"])
+        return self
 
     def info(self) -> dict:
         i = ModelInfo()
