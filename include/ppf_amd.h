@@ -122,6 +122,10 @@ typedef struct {
   int32_t exact_votes;         /* 1: evaluate every alpha bin with the full float path */
   uint32_t candidate_capacity; /* initial (code, count) buffer capacity; 0 = automatic.  A
                                   smaller buffer costs reruns of the vote pass, never results */
+  int32_t per_reference_argmax; /* 1: keep only each reference point's best (model point,
+                                   alpha) bin -- larger count, then smaller code -- before the
+                                   threshold: the score pass that include/kernel.h:78-82
+                                   declares (ppf_score_kernel) and never defines */
 } ppf_lookup_params;
 
 int ppf_lookup(ppf_model model, ppf_scene scene, const ppf_lookup_params *params,

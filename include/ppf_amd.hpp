@@ -113,7 +113,7 @@ class Model {
   Model(Context &ctx, const PointCloud *cloud, float d_dist, float vote_count_threshold,
         bool cpu_clustering, bool use_l1_norm, bool use_averaged_clusters)
       : prm_{vote_count_threshold, cpu_clustering ? 1 : 0, use_l1_norm ? 1 : 0,
-             use_averaged_clusters ? 1 : 0, 0, 0} {
+             use_averaged_clusters ? 1 : 0, 0, 0, 0} {
     std::vector<float> p, n;
     detail::split(*cloud, p, n);
     check(ppf_model_build(ctx.get(), p.data(), n.data(), (int64_t)cloud->size(), d_dist, &h_));

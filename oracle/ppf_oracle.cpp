@@ -35,6 +35,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <vector>
 
@@ -490,6 +491,24 @@ void threshold_votes(std::vector<VoteEntry> &e, float vote_count_threshold, uint
   e.resize(keep);
 }
 
+// Per-reference-point argmax, the score pass that include/kernel.h:78-82
+// declares (ppf_score_kernel) and the reference never defines: of each scene
+// reference point's (m_r, alpha) bins keep the one with the largest count,
+// the smallest code among equal counts.  Output in code order.
+void per_ref_argmax(std::vector<VoteEntry> &e) {
+  std::map<uint32_t, VoteEntry> best;
+  for (const VoteEntry &x : e) {
+    const uint32_t s_r = (uint32_t)(x.code >> 32);
+    auto it = best.find(s_r);
+    if (it == best.end())
+      best.emplace(s_r, x);
+    else if (x.count > it->second.count || (x.count == it->second.count && x.code < it->second.code))
+      it->second = x;
+  }
+  e.clear();
+  for (const auto &kv : best) e.push_back(kv.second);
+}
+
 std::vector<int64_t> default_refs(int64_t n_s, int rpdf) {
   std::vector<int64_t> r;
   for (int64_t i = 0; i < n_s; i += rpdf) r.push_back(i);
@@ -781,8 +800,9 @@ int oracle_vote(const float *m_pts, const float *m_nrm, int64_t n_m, const float
       ref_list ? std::vector<int64_t>(ref_list, ref_list + n_ref) : default_refs(n_s, rpdf);
   std::vector<VoteEntry> e;
   int64_t L, H, V;
-  vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, mode, refs, e, &L, &H, &V);
+  vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, mode & 1, refs, e, &L, &H, &V);
   int64_t U = (int64_t)e.size();
+  if (mode & 4) per_ref_argmax(e);
   uint32_t maxc = 0;
   threshold_votes(e, vote_count_threshold, &maxc);
   stats[0] = L;
@@ -839,7 +859,7 @@ int oracle_cluster_greedy(const float *T, const uint32_t *counts, int64_t P, flo
 int oracle_register(const float *m_pts, const float *m_nrm, int64_t n_m, const float *s_pts,
                     const float *s_nrm, int64_t n_s, float d_dist, int rpdf,
                     float vote_count_threshold, int cpu_clustering, int use_l1_norm,
-                    int use_averaged_clusters, float *pose_out, int64_t *stats) {
+                    int use_averaged_clusters, int per_ref, float *pose_out, int64_t *stats) {
   for (int i = 0; i < 16; ++i) pose_out[i] = 0;
   if (n_m < 1 || n_s < 1 || rpdf < 1 || n_m >= 65536) return 1;
   ModelDesc md = build_model(m_pts, m_nrm, n_m, d_dist);
@@ -848,6 +868,7 @@ int oracle_register(const float *m_pts, const float *m_nrm, int64_t n_m, const f
   int64_t L, H, V;
   vote_histogram(md, m_pts, m_nrm, s_pts, s_nrm, n_s, d_dist, 1, refs, e, &L, &H, &V);
   int64_t U = (int64_t)e.size();
+  if (per_ref) per_ref_argmax(e);
   uint32_t maxc = 0;
   threshold_votes(e, vote_count_threshold, &maxc);
   int64_t P = (int64_t)e.size();

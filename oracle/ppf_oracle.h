@@ -41,7 +41,9 @@ uint32_t oracle_alpha_literal(const float *m_r, const float *n_r_m, const float 
                               const float *s_r, const float *n_r_s, const float *s_i);
 
 /* ---- voting + vote reduction (model.cu:95-171).
- * mode 0 = literal per-vote trans_model_scene, 1 = hoisted frames.
+ * mode bit 0: 0 = literal per-vote trans_model_scene, 1 = hoisted frames;
+ * mode bit 2 (| 4): keep only each reference point's best bin (largest count,
+ * then smallest code) before the threshold.
  * ref_list/n_ref: optional subset of scene reference points to process (NULL =
  * all s_r with s_r % rpdf == 0).  Outputs the thresholded, ordered (code,count)
  * list; capacity cap.  stats[0..5] = lookups, hits, votes, unique codes,
@@ -77,7 +79,7 @@ int oracle_cluster_greedy(const float *T, const uint32_t *counts, int64_t P, flo
 int oracle_register(const float *m_pts, const float *m_nrm, int64_t n_m, const float *s_pts,
                     const float *s_nrm, int64_t n_s, float d_dist, int rpdf,
                     float vote_count_threshold, int cpu_clustering, int use_l1_norm,
-                    int use_averaged_clusters, float *pose_out, int64_t *stats);
+                    int use_averaged_clusters, int per_ref, float *pose_out, int64_t *stats);
 
 /* ---- timing helper for the CPU baseline: votes for the given reference
  * points with the hoisted form, OpenMP over reference points; returns

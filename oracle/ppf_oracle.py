@@ -58,7 +58,7 @@ def lib():
                                             C.POINTER(C.c_int64)]
         L.oracle_register.argtypes = [_f32p, _f32p, C.c_int64, _f32p, _f32p, C.c_int64,
                                       C.c_float, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int,
-                                      _f32p, _i64p]
+                                      C.c_int, _f32p, _i64p]
         L.oracle_vote_throughput.argtypes = [_f32p, _f32p, C.c_int64, _f32p, _f32p, C.c_int64,
                                              C.c_float, _i64p, C.c_int64, _i64p,
                                              C.POINTER(C.c_double)]
@@ -127,8 +127,11 @@ def alpha_literal(m_r, n_r_m, m_i, s_r, n_r_s, s_i) -> int:
     return int(lib().oracle_alpha_literal(*a))
 
 
-def vote(model, scene, d_dist, rpdf, thr, mode=1, refs=None, cap=1 << 22):
-    """Thresholded, ordered (codes, counts) and stats dict."""
+def vote(model, scene, d_dist, rpdf, thr, mode=1, refs=None, cap=1 << 22, per_ref=False):
+    """Thresholded, ordered (codes, counts) and stats dict.  per_ref: only
+    each reference point's best bin (largest count, then smallest code)."""
+    if per_ref:
+        mode |= 4
     mp, mn = model
     sp, sn = scene
     codes = np.zeros(cap, np.uint64)
@@ -190,13 +193,14 @@ def cluster_greedy(T, counts, trans_thresh, rot_thresh):
     return out[:16 * k].reshape(k, 4, 4), votes[:k], int(nc.value)
 
 
-def register(model, scene, d_dist, rpdf, thr, cpu_clustering, use_l1=False, use_avg=False):
+def register(model, scene, d_dist, rpdf, thr, cpu_clustering, use_l1=False, use_avg=False,
+             per_ref=False):
     pose = np.zeros(16, np.float32)
     stats = np.zeros(8, np.int64)
     rc = lib().oracle_register(_c(model[0], np.float32), _c(model[1], np.float32), len(model[0]),
                                _c(scene[0], np.float32), _c(scene[1], np.float32), len(scene[0]),
                                d_dist, rpdf, thr, int(cpu_clustering), int(use_l1), int(use_avg),
-                               pose, stats)
+                               int(per_ref), pose, stats)
     st = dict(rc=rc, lookups=int(stats[0]), hits=int(stats[1]), votes=int(stats[2]),
               unique=int(stats[3]), max_count=int(stats[4]), kept=int(stats[5]),
               cluster=int(stats[6]))

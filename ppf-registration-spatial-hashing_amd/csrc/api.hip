@@ -359,6 +359,7 @@ ppfk::VoteJob vote_job(ppf_model m, ppf_scene s, float thr, unsigned long long *
   J.cand_cap = cap;
   J.votes = reinterpret_cast<unsigned long long *>(meta + 2);
   J.item_begin = 0;
+  J.per_ref = 0;
   return J;
 }
 }  // namespace ppfi
@@ -801,6 +802,7 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
     const uint32_t preset[4] = {0u, meta_h[1], 0u, 0u};
     HIPC(hipMemcpyAsync(meta, preset, 16, hipMemcpyHostToDevice, st));
     ppfk::VoteJob J = ppfi::vote_job(m, s, p->vote_count_threshold, cc, cn, meta, cap);
+    J.per_ref = p->per_reference_argmax != 0;
     float ms = 0.0f;
     TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)s->R, m->TM, s->cloud.n, p->exact_votes != 0, &ms));
     vote_ms += ms;

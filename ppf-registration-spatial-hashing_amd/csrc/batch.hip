@@ -57,6 +57,7 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
       auto *cc = b.cand.as<unsigned long long>();
       auto *cn = reinterpret_cast<uint32_t *>(cc + b.cap);
       ppfk::VoteJob J = vote_job(b.m, b.s, b.thr, cc, cn, md + 4 * j, b.cap);
+      J.per_ref = b.per_ref;
       J.item_begin = items;
       items += (uint64_t)b.s->R;
       max_tm = std::max(max_tm, (int)b.m->TM);
@@ -186,6 +187,7 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
     b.m = models[j];
     b.s = scenes[j];
     b.thr = params->vote_count_threshold;
+    b.per_ref = params->per_reference_argmax != 0;
     b.cap = params->candidate_capacity ? round_cap(params->candidate_capacity) : batch_cap(scenes[j]->R);
     b.index = (int64_t)j;
     q.push_back(std::move(b));

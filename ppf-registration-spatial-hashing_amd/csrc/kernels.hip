@@ -336,7 +336,18 @@ struct VoteShared {
   uint32_t wmax[VOTE_BLOCK / WAVE];
   uint32_t wsum[VOTE_BLOCK / WAVE][2];
   float thr;
+  uint32_t wbest[VOTE_BLOCK / WAVE];  // per-wave best (count, code) of a tile
+  unsigned long long wcode[VOTE_BLOCK / WAVE];
+  uint32_t best;                       // the reference row's best so far
+  unsigned long long best_code;
 };
+
+// (count, code) order of the per-reference-row maximum: larger count, then
+// smaller code (the first maximum in code order)
+__device__ __forceinline__ bool better_vote(uint32_t c, unsigned long long k, uint32_t bc,
+                                            unsigned long long bk) {
+  return c > bc || (c == bc && c > 0u && k < bk);
+}
 
 template <bool EXACT>
 __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
@@ -644,6 +655,55 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       }
       __syncthreads();
       const float thr_lb = sh.thr;
+      if (J.per_ref) {
+        // per-reference-point argmax: the tile's first maximum in code order,
+        // kept if it beats the row's best of the earlier tiles
+        uint32_t bc = 0;
+        unsigned long long bk = ~0ull;
+        for (int32_t i = tid; i < nb; i += VOTE_BLOCK) {
+          const uint32_t c = acc[i];
+          const uint32_t a = (uint32_t)i / pitch;
+          const uint32_t mr = (uint32_t)tile_base + ((uint32_t)i - a * pitch);
+          const unsigned long long k = ((unsigned long long)(uint32_t)s_r << 32) | (mr << 6) | a;
+          if (better_vote(c, k, bc, bk)) {
+            bc = c;
+            bk = k;
+          }
+        }
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+          const uint32_t c2 = (uint32_t)__shfl_xor((int)bc, o, WAVE);
+          const unsigned long long k2 = __shfl_xor(bk, o, WAVE);
+          if (better_vote(c2, k2, bc, bk)) {
+            bc = c2;
+            bk = k2;
+          }
+        }
+        if (lane == 0) {
+          sh.wbest[wv] = bc;
+          sh.wcode[wv] = bk;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          if (t == 0) {
+            sh.best = 0u;
+            sh.best_code = ~0ull;
+          }
+          for (int k = 0; k < VOTE_BLOCK / WAVE; ++k)
+            if (better_vote(sh.wbest[k], sh.wcode[k], sh.best, sh.best_code)) {
+              sh.best = sh.wbest[k];
+              sh.best_code = sh.wcode[k];
+            }
+          if (t == T - 1 && sh.best > 0u && (float)sh.best > thr_lb) {
+            const uint32_t pos = atomicAdd(J.cand_n, 1u);
+            if (pos < J.cand_cap) {
+              J.cand_code[pos] = sh.best_code;
+              J.cand_count[pos] = sh.best;
+            }
+          }
+        }
+        __syncthreads();
+        continue;
+      }
       const int32_t nb_r = (nb + VOTE_BLOCK - 1) / VOTE_BLOCK * VOTE_BLOCK;
       for (int32_t i = tid; i < nb_r; i += VOTE_BLOCK) {
         const uint32_t c = i < nb ? acc[i] : 0u;

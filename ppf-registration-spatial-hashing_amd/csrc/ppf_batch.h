@@ -20,6 +20,7 @@ struct BatchJob {
   ppf_model m = nullptr;
   ppf_scene s = nullptr;
   float thr = 0.0f;
+  int32_t per_ref = 0;   // per-reference-point argmax
   uint32_t cap = 0;      // candidate capacity
   TBuf cand;             // (code u64 [cap], count u32 [cap])
   uint32_t n_cand = 0;   // candidates emitted by the last pass

@@ -162,7 +162,8 @@ struct VoteJob {
   unsigned long long *cand_code;
   uint32_t *cand_count;
   uint32_t *cand_n;
-  uint32_t cand_cap, pad2;
+  uint32_t cand_cap;
+  int32_t per_ref;  // 1: emit only each reference row's first maximum
   unsigned long long *votes;  // total votes of the job (statistics)
   uint64_t item_begin;
 };

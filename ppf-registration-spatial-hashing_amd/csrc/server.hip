@@ -120,6 +120,7 @@ class HipBackend : public ppfs::Backend {
         auto *cn = reinterpret_cast<uint32_t *>(cc + js->cap);
         ppfk::VoteJob J = vote_job(m, sc->scene, s.req->params.vote_count_threshold, cc, cn,
                                    js->meta.as<uint32_t>(), js->cap);
+        J.per_ref = s.req->params.per_reference_argmax != 0;
         J.r_begin = (int32_t)s.row0;
         J.item_begin = items;
         items += (uint64_t)s.rows;

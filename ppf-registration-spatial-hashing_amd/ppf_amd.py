@@ -49,7 +49,8 @@ class ModelInfo(C.Structure):
 class LookupParams(C.Structure):
     _fields_ = [("vote_count_threshold", C.c_float), ("cpu_clustering", C.c_int32),
                 ("use_l1_norm", C.c_int32), ("use_averaged_clusters", C.c_int32),
-                ("exact_votes", C.c_int32), ("candidate_capacity", C.c_uint32)]
+                ("exact_votes", C.c_int32), ("candidate_capacity", C.c_uint32),
+                ("per_reference_argmax", C.c_int32)]
 
 
 class ResultInfo(C.Structure):
@@ -416,10 +417,10 @@ class Model(_Owned):
         _check(lib().ppf_model_set_weights(self.handle, w, len(w)))
 
     def ppf_lookup(self, scene: Scene, exact_votes: bool = False,
-                   candidate_capacity: int = 0) -> LookupResult:
+                   candidate_capacity: int = 0, per_reference_argmax: bool = False) -> LookupResult:
         prm = LookupParams(self.vote_count_threshold, int(self.cpu_clustering),
                            int(self.use_l1_norm), int(self.use_averaged_clusters),
-                           int(exact_votes), int(candidate_capacity))
+                           int(exact_votes), int(candidate_capacity), int(per_reference_argmax))
         h = _vp()
         _check(lib().ppf_lookup(self.handle, scene.handle, C.byref(prm), C.byref(h)))
         self.result = LookupResult(h, self.cpu_clustering, self.ctx)
@@ -494,7 +495,8 @@ def ht_dist(a, b):
 def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_threshold: float,
                  cpu_clustering: bool = False, use_l1_norm: bool = False,
                  use_averaged_clusters: bool = False, exact_votes: bool = False,
-                 ctx: Optional[Context] = None, candidate_capacity: int = 0):
+                 ctx: Optional[Context] = None, candidate_capacity: int = 0,
+                 per_reference_argmax: bool = False):
     """Batched form: result j of (models[j], scenes[j]), all jobs voted in one
     persistent launch (ppf_batch_lookup).  Returns (results, info dict)."""
     if len(models) != len(scenes) or not models:
@@ -504,7 +506,8 @@ def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_
     mh = (C.c_void_p * n)(*[m.handle.value for m in models])
     sh = (C.c_void_p * n)(*[s.handle.value for s in scenes])
     prm = LookupParams(float(vote_count_threshold), int(cpu_clustering), int(use_l1_norm),
-                       int(use_averaged_clusters), int(exact_votes), int(candidate_capacity))
+                       int(use_averaged_clusters), int(exact_votes), int(candidate_capacity),
+                       int(per_reference_argmax))
     out = (C.c_void_p * n)()
     info = BatchInfo()
     _check(lib().ppf_batch_lookup(ctx.handle, n, mh, sh, C.byref(prm), out, C.byref(info)))
@@ -539,11 +542,12 @@ class Server(_Owned):
     def submit(self, cloud, model_ids: Sequence[int], ref_point_downsample_factor: int,
                vote_count_threshold: float, cpu_clustering: bool = False, use_l1_norm: bool = False,
                use_averaged_clusters: bool = False, exact_votes: bool = False,
-               candidate_capacity: int = 0) -> int:
+               candidate_capacity: int = 0, per_reference_argmax: bool = False) -> int:
         p, n = _cloud(*cloud)
         ids = np.ascontiguousarray(model_ids, dtype=np.int32)
         prm = LookupParams(float(vote_count_threshold), int(cpu_clustering), int(use_l1_norm),
-                           int(use_averaged_clusters), int(exact_votes), int(candidate_capacity))
+                           int(use_averaged_clusters), int(exact_votes), int(candidate_capacity),
+                           int(per_reference_argmax))
         t = C.c_uint64()
         _check(lib().ppf_server_submit(self.handle, p, n, len(p), int(ref_point_downsample_factor),
                                        len(ids), ids.ctypes.data, C.byref(prm), C.byref(t)))

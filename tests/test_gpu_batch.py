@@ -85,6 +85,30 @@ def test_batch_exact_votes_equal_fast(ctx):
         o.close()
 
 
+def test_batch_per_reference_argmax_equals_plain(ctx):
+    w = _small_batch(seed=23)
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    scenes = [ppf_amd.Scene(s, 0.0, w.rpdf, ctx=ctx) for s in w.scenes]
+    jm = [models[mi] for si, ids in enumerate(w.scene_models) for mi in ids]
+    js = [scenes[si] for si, ids in enumerate(w.scene_models) for _ in ids]
+    res, _ = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, per_reference_argmax=True, ctx=ctx)
+    for m, s, r in zip(jm, js, res):
+        plain = m.ppf_lookup(s, per_reference_argmax=True)
+        _same(r, plain, True)
+        full = m.ppf_lookup(s)
+        # one kept bin per reference row at most, each one of the full list's
+        # bins with the same count
+        c, n = r.votes()
+        fc, fn = full.votes()
+        assert len(np.unique(c >> np.uint64(32))) == len(c)
+        assert len(c) <= len(fc)
+        assert set(zip(c.tolist(), n.tolist())) <= set(zip(fc.tolist(), fn.tolist()))
+        for o in (plain, full, r):
+            o.close()
+    for o in models + scenes:
+        o.close()
+
+
 def test_batch_rejects_foreign_objects(ctx):
     other = ppf_amd.Context(0)
     try:

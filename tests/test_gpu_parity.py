@@ -91,6 +91,35 @@ def test_votes_match_oracle(ctx, medium, exact):
     assert np.array_equal(codes, oc) and np.array_equal(counts, on)
 
 
+@pytest.fixture(scope="module")
+def multi_tile():
+    w = synth.plain_small(seed=5, n_model=1300, n_clutter=900)
+    return w.models[0], w.scenes[0], w.d_dists[0]
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_per_reference_argmax_matches_oracle(ctx, multi_tile, exact):
+    """Score pass (include/kernel.h:78-82): one bin per reference row, the
+    maximum over all model-point tiles (count desc, code asc), then the usual
+    threshold, poses and clustering."""
+    m, s, d = multi_tile
+    model = ppf_amd.Model(m, d, 0.3, ctx=ctx)
+    assert model.info()["n_tiles"] > 1
+    scene = ppf_amd.Scene(s, d, 5, ctx=ctx)
+    res = model.ppf_lookup(scene, exact_votes=exact, per_reference_argmax=True)
+    codes, counts = res.votes()
+    oc, on, st = O.vote(m, s, d, 5, 0.3, per_ref=True)
+    assert len(oc) > 10
+    assert res.info["max_count"] == st["max_count"]
+    assert np.array_equal(codes, oc) and np.array_equal(counts, on)
+    assert len(np.unique(codes >> np.uint64(32))) == len(codes)
+    opose, _ = O.register(m, s, d, 5, 0.3, False, per_ref=True)
+    assert np.array_equal(_bits(res.pose()), _bits(opose))
+    res.close()
+    model.close()
+    scene.close()
+
+
 def test_tiny_golden_case(ctx):
     g = np.load(os.path.join(GOLDEN, "tiny_case.npz"))
     m = (g["model_points"], g["model_normals"])

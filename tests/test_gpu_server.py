@@ -108,3 +108,30 @@ def test_budget_cancel_and_errors(ctx):
         srv.close()
         for m in models:
             m.close()
+
+
+def test_mixed_per_reference_argmax_requests(ctx):
+    """Requests with and without the per-reference score pass share vote
+    rounds; each one's result equals the plain lookup with its own flag."""
+    w = _workload(seed=33)
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    srv = ppf_amd.Server(models, round_ms=2.0, ctx=ctx)
+    try:
+        reqs = [(s, ids, k % 2 == 0) for k, (s, ids) in enumerate(zip(w.scenes, w.scene_models))]
+        tickets = [srv.submit(s, ids, w.rpdf, w.vote_count_threshold, per_reference_argmax=pr)
+                   for s, ids, pr in reqs]
+        for (s, ids, pr), t in zip(reqs, tickets):
+            _, _, info, res = srv.wait(t, 120.0, results=True)
+            assert info["rc"] == ppf_amd.PPF_OK, info
+            sc = ppf_amd.Scene(s, 0.0, w.rpdf, ctx=ctx)
+            for mi, r in zip(ids, res):
+                plain = models[mi].ppf_lookup(sc, per_reference_argmax=pr)
+                assert _digest(r, True) == _digest(plain, True)
+                plain.close()
+                r.close()
+            sc.close()
+        assert srv.stats()["failed"] == 0
+    finally:
+        srv.close()
+        for m in models:
+            m.close()

@@ -120,6 +120,30 @@ def test_vote_ref_subset_is_a_restriction():
     assert st_sub["lookups"] == len(refs) * (len(s[0]) - 1)
 
 
+@pytest.mark.parametrize("seed", [6, 7])
+def test_per_ref_argmax_is_grouped_numpy_argmax(seed):
+    """The per-reference-point score pass against numpy: from the full vote
+    list (threshold 0), each s_r's bin with the largest count, smallest code on
+    ties (lexsort), then the count threshold against the global maximum and the
+    (count desc, code asc) order of the thresholded list."""
+    m, s, d, _ = tiny_workload(seed=seed, n_model=70, n_scene=160)
+    c_all, n_all, st_all = O.vote(m, s, d, 2, 0.0)
+    order = np.lexsort((c_all, -n_all.astype(np.int64), c_all >> np.uint64(32)))
+    sr = (c_all >> np.uint64(32))[order]
+    first = np.concatenate([[True], sr[1:] != sr[:-1]])
+    bc, bn = c_all[order][first], n_all[order][first]
+    assert len(np.unique(c_all >> np.uint64(32))) == len(bc)
+    for thr in (0.0, 0.3, 0.7):
+        keep = bn.astype(np.float32) > np.float32(thr) * np.float32(bn.max())
+        kc, kn = bc[keep], bn[keep]
+        o = np.lexsort((kc, -kn.astype(np.int64)))
+        c, n, st = O.vote(m, s, d, 2, thr, per_ref=True)
+        assert np.array_equal(c, kc[o]) and np.array_equal(n, kn[o])
+        assert st["max_count"] == st_all["max_count"] == int(bn.max())
+    # the per-reference list is what the full list keeps at one bin per row
+    assert set(zip(bc.tolist(), bn.tolist())) <= set(zip(c_all.tolist(), n_all.tolist()))
+
+
 def test_pose_invariants():
     m, s, d, _ = tiny_workload(seed=5, n_model=60, n_scene=100)
     codes, counts, _ = O.vote(m, s, d, 2, 0.0)
