@@ -456,6 +456,14 @@ namespace {
 constexpr char MODEL_MAGIC[8] = {'P', 'P', 'F', 'M', 'O', 'D', 'E', 'L'};
 constexpr uint32_t MODEL_VERSION = 1;
 
+// accumulator pitch of a tile of TM points: 2^k >= max(TM, 64); files written
+// with 2^k + 1 (same records, more LDS bank conflicts) load as they are
+bool pitch_ok(int32_t TM, int32_t pitch) {
+  int32_t pw = 64;
+  while (pw < TM) pw <<= 1;
+  return pitch == pw || pitch == pw + 1;
+}
+
 struct ModelFileHeader {
   char magic[8];
   uint32_t version, header_bytes;
@@ -544,7 +552,7 @@ int ppf_model_load(ppf_ctx ctx, const char *path, ppf_model *out) {
   ARG_CHECK(h.n_points >= 2 && h.n_points <= PPF_MAX_MODEL_POINTS && h.N == h.n_points * h.n_points &&
                 h.U >= 1 && h.U <= h.N && h.T >= 1 && h.TM >= 1 && h.TM <= MAX_TILE_POINTS &&
                 (int64_t)h.T * h.TM >= h.n_points && h.hcap >= 2 * h.U && (h.hcap & (h.hcap - 1)) == 0 &&
-                h.acc_pitch > h.TM && h.d_dist > 0.0f && h.hshift >= 1 && h.hshift <= 31 &&
+                pitch_ok(h.TM, h.acc_pitch) && h.d_dist > 0.0f && h.hshift >= 1 && h.hshift <= 31 &&
                 (1ull << (32 - h.hshift)) == h.hcap,
             "ppf_model_load: inconsistent model file header");
   const size_t lds_points = (ctx->max_lds - 1024) / (N_BINS * 4) / 32 * 32;
@@ -663,7 +671,7 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   {
     int32_t pw = 64;
     while (pw < m->TM) pw <<= 1;
-    m->acc_pitch = pw + 1;
+    m->acc_pitch = pw;
   }
   HIPC(hipStreamSynchronize(st));
   words.release();

@@ -252,11 +252,11 @@ __global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const fl
 // (pair order within), then within each (bucket, tile) segment by (rank in
 // its class, class): the first rows of 64 hold one record of every class, and
 // a class only repeats in a row once fewer than 64 classes have records left.
-// With the accumulator at acc[bin * pitch + m], pitch = 2^k + 1 with 2^k a
-// multiple of 64, an update's LDS bank is (m + bin) mod 64, so lanes with
-// distinct classes and one bin, or with one model point and distinct bins,
-// never share a bank.  Any order gives the same votes; this one keeps the
-// 64 atomic adds of a wave instruction in distinct banks as far as possible.
+// With the accumulator at acc[bin * pitch + m], pitch = 2^k a multiple of 64,
+// an update's LDS bank is m mod 64 whatever its bin, so lanes with distinct
+// classes never share a bank (the bins of a wave's lanes differ from record
+// to record).  Any order gives the same votes; this one keeps the 64 atomic
+// adds of a wave instruction in distinct banks as far as possible.
 __global__ void k_vote_order_words(uint64_t *__restrict__ words, uint32_t *__restrict__ vals,
                                    int64_t N, int32_t n_m, int32_t TM) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -529,9 +529,9 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
     // ---- phase 2: per model-point tile, accumulate then emit
     unsigned long long wave_votes = 0;
     const int32_t T = J.T, TM = J.TM, n_m = J.n_m;
-    // bin-major accumulator acc[bin * pitch + m], pitch = 2^k + 1 with 2^k a
-    // multiple of 64: the LDS bank of an update is (m + bin) mod 64.  LDS byte
-    // addresses: accumulator base, and 64 spare words (one per lane) past it.
+    // bin-major accumulator acc[bin * pitch + m], pitch = 2^k a multiple of
+    // 64: the LDS bank of an update is m mod 64.  LDS byte addresses:
+    // accumulator base, and 64 spare words (one per lane) past it.
     const uint32_t pitch = (uint32_t)J.acc_pitch, pitch4 = pitch * 4u;
     const uint32_t acc_base = (uint32_t)(uintptr_t)acc;
     const int32_t nb = N_BINS * (int32_t)pitch;
@@ -1327,8 +1327,8 @@ int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mfr
 }
 
 // Words of dynamic LDS before VoteShared: the largest accumulator (31 rows of
-// pitch 2^k + 1, 2^k >= 64) plus 64 spare words, or the hit-grouping table,
-// rounded to 8 bytes.
+// pitch 2^k >= 64, or 2^k + 1 for model files written with that pitch) plus
+// 64 spare words, or the hit-grouping table, rounded to 8 bytes.
 uint32_t vote_misc_words(int max_tm) {
   uint32_t pw = 64;
   while ((int)pw < max_tm) pw <<= 1;

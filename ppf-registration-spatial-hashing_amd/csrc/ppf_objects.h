@@ -116,7 +116,7 @@ struct ppf_model_s {
   TBuf hcount;   // [U]
   TBuf hfirst;   // [U]
   TBuf pair;     // [n*n] pair index m_r*n + m_i, sorted by (key, pair): hashkeyToDataMap
-  int32_t acc_pitch = 0;  // accumulator row pitch in words: 2^k + 1 with 2^k >= TM, >= 64
+  int32_t acc_pitch = 0;  // accumulator row pitch in words: 2^k >= TM, >= 64
   TBuf vpair;    // [n*n] pair index in vote order (key, tile, rank in class, class)
   TBuf rec;      // [n*n] packed vote records, vote order
   TBuf toff;     // [U][T+1] start of each tile's sub-range

@@ -233,6 +233,40 @@ def test_model_weights_scale_cluster_votes(ctx, medium):
     assert res.info["max_idx"] == ref["max_idx"]
 
 
+def _line_model(seed=4):
+    """Ellipsoid/box points plus a row of points on the x axis with normals
+    (1, 0, 0): the pairs of that row have u = (dx, 0, 0) in the reference
+    point's frame, so their records always take the float path."""
+    m = synth.ellipsoid_box_model(80, seed)
+    lp = np.zeros((14, 3), np.float32)
+    lp[:, 0] = np.linspace(-0.3, 0.35, 14, dtype=np.float32)
+    ln = np.tile(np.array([1, 0, 0], np.float32), (14, 1))
+    return synth.as_cloud(np.concatenate([m[0], lp]), np.concatenate([m[1], ln]))
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_votes_with_float_path_records_match_oracle(ctx, exact):
+    m = _line_model()
+    rng = np.random.default_rng(8)
+    R = synth.random_rotation(rng)
+    sp, sn = synth.transform(m[0].astype(np.float64), m[1].astype(np.float64), R, np.array([0.2, -0.1, 0.3]))
+    cp = rng.uniform(-0.6, 0.6, (60, 3))
+    cn = rng.normal(size=(60, 3))
+    cn /= np.linalg.norm(cn, axis=1, keepdims=True)
+    s = synth.as_cloud(np.concatenate([sp, cp]), np.concatenate([sn, cn]))
+    d = float(np.float32(0.05))
+    model = ppf_amd.Model(m, d, 0.0, ctx=ctx)
+    res = model.ppf_lookup(ppf_amd.Scene(s, d, 1, ctx=ctx), exact_votes=exact)
+    codes, counts = res.votes()
+    oc, on, st = O.vote(m, s, d, 1, 0.0)
+    assert st["votes"] > 0
+    assert np.array_equal(codes, oc) and np.array_equal(counts, on)
+    # the line's reference points have votes (their row pairs are float-path records)
+    line_rows = set(range(80, 94))
+    m_r = (codes & np.uint64(0xffffffff)) >> np.uint64(6)
+    assert len(line_rows & set(m_r.tolist())) > 0
+
+
 def test_degenerate_inputs(ctx):
     m = synth.ellipsoid_box_model(50, 1)
     far = (m[0] * 0 + np.float32(1000.0), m[1])
