@@ -973,9 +973,9 @@ __device__ __forceinline__ float readfirst_f32(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
 
-// Bounds of the 128 poses of one wave (centre: lane 0's first pose).
-__device__ ClWave cluster_wave_bounds(const ClPose *P, const bool *have) {
-  ClPose c = P[0];
+// Bounds of the 64 poses a wave holds in one slot (centre: lane 0's pose).
+__device__ ClWave cluster_slot_bounds(const ClPose &p, bool have, float d_dist) {
+  ClPose c = p;
   c.tx = readfirst_f32(c.tx);
   c.ty = readfirst_f32(c.ty);
   c.tz = readfirst_f32(c.tz);
@@ -983,14 +983,12 @@ __device__ ClWave cluster_wave_bounds(const ClPose *P, const bool *have) {
   c.qy = readfirst_f32(c.qy);
   c.qz = readfirst_f32(c.qz);
   c.qw = readfirst_f32(c.qw);
-  const bool h0 = __builtin_amdgcn_readfirstlane(have[0] ? 1 : 0) != 0;
+  const bool h0 = __builtin_amdgcn_readfirstlane(have ? 1 : 0) != 0;
   ClWave W;
   ppfc::cl_wave_center(c, h0, W);
   float rho = 0.0f, nmax = 0.0f, rad = 0.0f;
-#pragma unroll
-  for (int r = 0; r < CL_PER; ++r)
-    if (have[r]) ppfc::cl_wave_add(W, P[r], rho, nmax, rad);
-  ppfc::cl_wave_finish(W, wave_max_f32(rho), wave_max_f32(nmax), wave_max_f32(rad));
+  if (have) ppfc::cl_wave_add(W, p, rho, nmax, rad);
+  ppfc::cl_wave_finish(W, wave_max_f32(rho), wave_max_f32(nmax), wave_max_f32(rad), d_dist);
   return W;
 }
 
@@ -1053,7 +1051,11 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   __syncthreads();
   unsigned long long tests = 0;
   if (s_uniform) {
-    const ClWave W = cluster_wave_bounds(P, have);
+    // one bound per slot: 64 consecutive poses of the rotation order each
+    static_assert(CL_PER == 2, "two pose slots per lane");
+    const ClWave W0 = cluster_slot_bounds(P[0], have[0], d_dist);
+    const ClWave W1 = cluster_slot_bounds(P[1], have[1], d_dist);
+    const bool any0 = __ballot(have[0]) != 0ull, any1 = __ballot(have[1]) != 0ull;
     for (int blk = 0; blk < 27; ++blk) {
       const uint32_t h = s_adj[blk];
       if (h == 0u) continue;
@@ -1074,18 +1076,20 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
         __syncthreads();
         for (uint32_t g0 = 0; g0 < cn; g0 += WAVE) {
           const uint32_t gj = g0 + (uint32_t)lane;
-          bool live = false;
+          bool live0 = false, live1 = false;
           if (gj < cn) {
             const float4 mq = s_q[gj], mt = s_t[gj];
-            live = !ppfc::cluster_skip(W, mq.x, mq.y, mq.z, mq.w, mt.x, mt.y, mt.z, d_dist, use_l1);
+            live0 = any0 && !ppfc::cluster_skip(W0, mq.x, mq.y, mq.z, mq.w, mt.x, mt.y, mt.z, use_l1);
+            live1 = any1 && !ppfc::cluster_skip(W1, mq.x, mq.y, mq.z, mq.w, mt.x, mt.y, mt.z, use_l1);
           }
-          uint64_t mask = __ballot(live);
+          const uint64_t m0 = __ballot(live0), m1 = __ballot(live1);
+          uint64_t mask = m0 | m1;
           while (mask) {
-            const uint32_t k = g0 + (uint32_t)__builtin_ctzll(mask);
+            const uint32_t b = (uint32_t)__builtin_ctzll(mask);
             mask &= mask - 1;
-            const float4 q0 = s_q[k], t0 = s_t[k];
-#pragma unroll
-            for (int r = 0; r < CL_PER; ++r) cl_test(P[r], q0, t0, s_max, use_l1, use_avg);
+            const float4 q0 = s_q[g0 + b], t0 = s_t[g0 + b];
+            if ((m0 >> b) & 1ull) cl_test(P[0], q0, t0, s_max, use_l1, use_avg);
+            if ((m1 >> b) & 1ull) cl_test(P[1], q0, t0, s_max, use_l1, use_avg);
           }
         }
         __syncthreads();

@@ -15,6 +15,10 @@ using namespace ppfm;
 constexpr float CL_ROT_DOT_MIN = 0.978f;
 constexpr float CL_ANGLE_MARGIN = 5e-3f;  // rad, far above float rounding of the angles
 constexpr float CL_DIST_MARGIN = 1e-5f;   // relative
+// Members whose quaternion norm lies in [CL_NLO, CL_NHI] (the unit
+// quaternions of rotation matrices) get the cheap rotation pre-test.
+constexpr float CL_NLO = 0.999f;
+constexpr float CL_NHI = 1.001f;
 
 struct ClPose {
   float tx, ty, tz, qx, qy, qz, qw;
@@ -62,10 +66,23 @@ PPF_HD float clamp_unit(float x) { return fminf(fmaxf(x, -1.0f), 1.0f); }
 // of the angle between q_i and q_j above that bound; triangle inequality on
 // the unit 3-sphere); a member farther than d_dist + rad from the ball's
 // centre cannot pass the translation test.  ok = false disables skipping.
+//
+// The per-member test uses two thresholds fixed per group.  Rotation: with
+// |q_j| in [NLO, NHI], kappa_j = 0.978/(nmax |q_j|) - 1e-5 >= kappa_lo (the
+// same with NHI), so an angle above Theta = acos(kappa_lo) + rho + margin
+// exceeds the member's own bound; and c.q_j < N cos(Theta) - 1e-6 <=
+// |q_j| cos(Theta) - 1e-6 implies that angle, with N = NLO when cos(Theta)
+// >= 0 and N = NHI otherwise (Theta < 3.1).  Translation:
+// with M = |centre| + d + rad, a member at |t_j| <= 2M has the margin
+// 1e-5 (M + |t_j|) <= 3e-5 M of the distance bound, and a member beyond 2M is
+// farther than d + rad + 1e-5 (M + |t_j|) anyway, so the squared distance is
+// compared with (d + rad + 4e-5 M)^2.
 struct ClWave {
   float cqx, cqy, cqz, cqw;
   float ctx, cty, ctz;
   float rho, nmax, rad, cmag;
+  float cos_thr;  // skip when |q_j|^2 in [NLO^2, NHI^2] and c.q_j < cos_thr
+  float b2;       // skip when |t_j - centre|^2 > b2 (unless use_l1)
   bool ok;
 };
 
@@ -101,32 +118,38 @@ PPF_HD void cl_wave_add(const ClWave &W, const ClPose &P, float &rho, float &nma
   rad = fmaxf(rad, sqrtf(dx * dx + dy * dy + dz * dz));
 }
 
-PPF_HD void cl_wave_finish(ClWave &W, float rho, float nmax, float rad) {
+PPF_HD void cl_wave_finish(ClWave &W, float rho, float nmax, float rad, float d_dist) {
   W.rho = rho + CL_ANGLE_MARGIN;
   W.nmax = nmax * (1.0f + CL_DIST_MARGIN);
   W.rad = rad;
   W.cmag = sqrtf(W.ctx * W.ctx + W.cty * W.cty + W.ctz * W.ctz);
+  W.cos_thr = -INFINITY;
+  W.b2 = INFINITY;
+  if (!W.ok) return;
+  const float kappa = CL_ROT_DOT_MIN / (W.nmax * CL_NHI) - 1e-5f;
+  if (kappa >= 1.0f) {
+    W.cos_thr = INFINITY;  // no member in the norm range can pass
+  } else if (kappa > -1.0f) {
+    const float th = acosf_own(kappa) + W.rho + CL_ANGLE_MARGIN;
+    if (th < 3.1f) {
+      const float c = cosf_own(th);
+      W.cos_thr = (c >= 0.0f ? CL_NLO : CL_NHI) * c - 1e-6f;
+    }
+  }
+  const float M = W.cmag + d_dist + W.rad;
+  const float B = d_dist + W.rad + 4e-5f * M;
+  if (B < 1e18f) W.b2 = B * B * (1.0f + 1e-6f);
 }
 
 // true: the member fails the test against every pose of the group.
 PPF_HD bool cluster_skip(const ClWave &W, float jqx, float jqy, float jqz, float jqw, float jtx,
-                         float jty, float jtz, float d_dist, int use_l1) {
-  if (!W.ok) return false;
-  const float n = sqrtf(jqx * jqx + jqy * jqy + jqz * jqz + jqw * jqw);
-  if (n > 0.0f && n < 1e15f) {
-    const float kappa = CL_ROT_DOT_MIN / (W.nmax * n) - 1e-5f;
-    if (kappa >= 1.0f) return true;
-    if (kappa > -1.0f) {
-      const float cj = (W.cqx * jqx + W.cqy * jqy + W.cqz * jqz + W.cqw * jqw) / n;
-      const float th = acosf_own(clamp_unit(cj));
-      if (th > acosf_own(kappa) + W.rho + CL_ANGLE_MARGIN) return true;
-    }
-  }
+                         float jty, float jtz, int use_l1) {
+  const float n2 = jqx * jqx + jqy * jqy + jqz * jqz + jqw * jqw;
+  const float cj = W.cqx * jqx + W.cqy * jqy + W.cqz * jqz + W.cqw * jqw;
+  if (n2 >= CL_NLO * CL_NLO && n2 <= CL_NHI * CL_NHI && cj < W.cos_thr) return true;
   if (!use_l1) {
     const float dx = jtx - W.ctx, dy = jty - W.cty, dz = jtz - W.ctz;
-    const float D = sqrtf(dx * dx + dy * dy + dz * dz);
-    const float jm = sqrtf(jtx * jtx + jty * jty + jtz * jtz);
-    if (D > d_dist + W.rad + CL_DIST_MARGIN * (W.cmag + jm + d_dist + W.rad)) return true;
+    if (dx * dx + dy * dy + dz * dz > W.b2) return true;
   }
   return false;
 }

@@ -105,7 +105,7 @@ int main() {
     float rho = 0, nmax = 0, rad = 0;
     for (int i = 0; i < 128; ++i)
       if (have[i]) ppfc::cl_wave_add(W, P[i], rho, nmax, rad);
-    ppfc::cl_wave_finish(W, rho, nmax, rad);
+    ppfc::cl_wave_finish(W, rho, nmax, rad, d);
     long long sk = 0, mm = 0;
     for (int m = 0; m < 20000; ++m) {
       // members: around a random group pose, at rotation angles and
@@ -148,7 +148,7 @@ int main() {
                   tz = (float)(P[i].tz + dist * uz / un);
       if (!(tx == tx)) continue;
       ++mm;
-      if (!ppfc::cluster_skip(W, q[0], q[1], q[2], q[3], tx, ty, tz, d, use_l1)) continue;
+      if (!ppfc::cluster_skip(W, q[0], q[1], q[2], q[3], tx, ty, tz, use_l1)) continue;
       ++sk;
       for (int k = 0; k < 128; ++k) {
         if (!have[k]) continue;
