@@ -309,20 +309,20 @@ __global__ void k_vote_rank_words2(const uint64_t *__restrict__ words, uint64_t 
 // mod 2^24 with v24 = phi_v + pi and phi24 = phi_u, then
 // a * 30 * 2^8 = (bin << 32) | frac exactly (the 24 x 24 bit multiplies
 // v_mul_u32_u24 / v_mul_hi_u32_u24).  phi_u carries <= 7.5e-7 rad of
-// quantization (22 bits), phi_v the float atan2 error (<= 7.7e-7 rad), the
-// truncation to 24 bits < 3.7e-7 rad, and nominal edges k*2pi/30 differ from
-// the float pipeline's k*D - pi_f by < 2.6e-7 rad: < 3.3e-6 rad = 1.6e-5 bin
-// in all.  Whenever frac is farther than FAST_MARGIN_BINS (4e-5 bin =
-// 8.4e-6 rad) from an edge the bin is decided; otherwise (and for degenerate u
-// or v) the exact float pipeline runs.  The wrap point theta = +-pi (position
-// 0 or 30) is always within the margin, so bin 30 only comes from the exact
-// path.
-constexpr float FAST_MARGIN_BINS = 4e-5f;
+// quantization (22 bits, rounded to nearest from a double atan2), v24 the
+// rounding to 24 bits of a double atan2 (<= 1.9e-7 rad), and nominal edges
+// k*2pi/30 differ from the float pipeline's k*D - pi_f by < 2.6e-7 rad:
+// < 2.4e-6 rad = 1.13e-5 bin in all.  Whenever frac is farther than
+// FAST_MARGIN_BINS (1.5e-5 bin = 3.1e-6 rad) from an edge the bin is decided;
+// otherwise (and for degenerate u or v) the exact float pipeline runs.  The
+// wrap point theta = +-pi (position 0 or 30) is always within the margin, so
+// bin 30 only comes from the exact path.
+constexpr float FAST_MARGIN_BINS = 1.5e-5f;
 // frac (as a 32-bit fraction) is decided when it lies in [OFF, 2^32 - OFF]
-// with OFF = floor(4e-5 * 2^32) + 1, i.e. when frac - OFF < RANGE (mod 2^32)
-constexpr uint32_t FAST_OFF = 171799u;
+// with OFF = floor(1.5e-5 * 2^32) + 1, i.e. when frac - OFF < RANGE (mod 2^32)
+constexpr uint32_t FAST_OFF = 64425u;
 constexpr uint32_t FAST_RANGE = (uint32_t)(4294967296ull - 2ull * (FAST_OFF - 1u) - 1ull);
-static_assert(FAST_OFF - 1u == (uint32_t)(4e-5 * 4294967296.0), "fast-bin margin");
+static_assert(FAST_OFF - 1u == (uint32_t)(1.5e-5 * 4294967296.0), "fast-bin margin");
 constexpr int VOTE_UNROLL = 4;
 constexpr int GT_LOG2 = 12;
 constexpr int GT_SIZE = 1 << GT_LOG2;  // hit-grouping table slots (2 x 16 KB of LDS)
@@ -428,10 +428,10 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
             if (EXACT || !(r2 > 1e-30f) || !(r2 < 1e30f)) {
               flag = 0x80000000u;
             } else {
-              const float phi = ppfm::atan2f_own(vz, vy);
-              const double turns = (double)phi * (4294967296.0 / 6.283185307179586);
-              // 24-bit turns of phi_v + pi
-              v32 = ((uint32_t)(long long)llrint(turns) + 0x80000000u) >> 8;
+              // 24-bit turns of phi_v + pi, rounded to nearest from a double
+              // atan2 of the float pair (error < 2^-25 turn)
+              const double turns = atan2((double)vz, (double)vy) * (16777216.0 / 6.283185307179586);
+              v32 = ((uint32_t)(long long)llrint(turns) + 0x800000u) & 0xffffffu;
             }
             hr = make_uint4(u | flag, v32, ppfm::fbits(vy), ppfm::fbits(vz));
           }

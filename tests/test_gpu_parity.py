@@ -204,6 +204,22 @@ def test_fast_bins_equal_exact_bins_dataset_like(ctx):
         assert np.allclose(M[:3, :3] @ m[0][mr] + M[:3, 3], s[0][sr], atol=1e-3)
 
 
+def test_fast_bins_equal_exact_bins_every_bin(ctx):
+    # threshold 0 keeps every (s_r, m_r, alpha) bin with a vote: a vote put in
+    # a neighbouring bin by the fast path would change two counts here
+    w = synth.dataset_like()
+    m, s, d = w.models[0], w.scenes[0], w.d_dists[0]
+    model = ppf_amd.Model(m, d, 0.0, ctx=ctx)
+    scene = ppf_amd.Scene(s, d, 400, ctx=ctx)
+    fast = model.ppf_lookup(scene)
+    exact = model.ppf_lookup(scene, exact_votes=True)
+    cf, nf = fast.votes()
+    ce, ne = exact.votes()
+    assert fast.info["votes"] == exact.info["votes"] > 10**9
+    assert len(cf) > 10**6
+    assert np.array_equal(cf, ce) and np.array_equal(nf, ne)
+
+
 def test_registration_api_matches_lookups(ctx):
     w1 = synth.plain_small(seed=5, n_model=300, n_clutter=300)
     w2 = synth.plain_small(seed=6, n_model=250, n_clutter=400)
