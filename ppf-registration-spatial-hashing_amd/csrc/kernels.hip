@@ -248,37 +248,19 @@ __global__ void k_records(const uint32_t *__restrict__ pair, int64_t N, const fl
   rec[j] = (q << REC_PHI_SHIFT) | slow | (m_r % (uint32_t)TM);
 }
 
-// Vote-record order.  Records are sorted by (key, tile, m_r mod 64), stably
-// (pair order within), then within each (bucket, tile) segment by (rank in
-// its class, class): the first rows of 64 hold one record of every class, and
-// a class only repeats in a row once fewer than 64 classes have records left.
-// With the accumulator at acc[bin * pitch + m], pitch = 2^k a multiple of 64,
-// an update's LDS bank is m mod 64 whatever its bin, so lanes with distinct
-// classes never share a bank (the bins of a wave's lanes differ from record
-// to record).  Any order gives the same votes; this one keeps the 64 atomic
-// adds of a wave instruction in distinct banks as far as possible.
-__global__ void k_vote_order_words(uint64_t *__restrict__ words, uint32_t *__restrict__ vals,
-                                   int64_t N, int32_t n_m, int32_t TM) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  const uint64_t w = words[j];
-  const uint32_t p = (uint32_t)w;
-  const uint32_t m_r = p / (uint32_t)n_m;
-  const uint32_t tile = m_r / (uint32_t)TM;
-  words[j] = (w & 0xffffffff00000000ull) | ((uint64_t)tile << 25) | ((uint64_t)(m_r & 63u) << 19);
-  vals[j] = p;
-}
-
-// words sorted by (key, tile, class) -> out: (key, tile, rank within the
-// class run, class) words for the second sort
-__global__ void k_vote_rank_words2(const uint64_t *__restrict__ words, uint64_t *__restrict__ out,
-                                   const uint32_t *__restrict__ hfirst, int64_t U,
-                                   const uint32_t *__restrict__ toff, int32_t T, int64_t N) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  const uint64_t w = words[j];
-  const uint32_t tc = (uint32_t)(w >> 19) & 0x1fffu;  // tile << 6 | class
-  int64_t lo = 0, hi = U - 1;  // bucket: last u with hfirst[u] <= j
+// Vote-record order.  Records are sorted by (bucket, tile, m_r mod 64),
+// stably (pair order within), then within each (bucket, tile) segment by
+// (rank in its class, class): the first rows of 64 hold one record of every
+// class, and a class only repeats in a row once fewer than 64 classes have
+// records left.  With the accumulator at acc[bin * pitch + m], pitch a
+// multiple of 64, an update's LDS bank is m mod 64 whatever its bin, so lanes
+// with distinct classes never share a bank (the bins of a wave's lanes differ
+// from record to record).  Any order gives the same votes; this one keeps the
+// 64 atomic adds of a wave instruction in distinct banks as far as possible.
+// The sort words carry the bucket index u (keys ascend with u) in as few bits
+// as U, T and the largest rank need, so the radix sorts take few passes.
+__device__ __forceinline__ uint32_t bucket_of(const uint32_t *__restrict__ hfirst, int64_t U, int64_t j) {
+  int64_t lo = 0, hi = U - 1;  // last u with hfirst[u] <= j
   while (lo < hi) {
     const int64_t mid = (lo + hi + 1) >> 1;
     if ((int64_t)hfirst[mid] <= j)
@@ -286,18 +268,80 @@ __global__ void k_vote_rank_words2(const uint64_t *__restrict__ words, uint64_t 
     else
       hi = mid - 1;
   }
+  return (uint32_t)lo;
+}
+
+// words in (key, pair) order -> (u << tb+6 | tile << 6 | class), vals = pair
+__global__ void k_vote_order_words(uint64_t *__restrict__ words, uint32_t *__restrict__ vals,
+                                   int64_t N, int32_t n_m, int32_t TM, const uint32_t *__restrict__ hfirst,
+                                   int64_t U, int tb) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint32_t p = (uint32_t)words[j];
+  const uint32_t m_r = p / (uint32_t)n_m;
+  const uint32_t tile = m_r / (uint32_t)TM;
+  const uint32_t u = bucket_of(hfirst, U, j);
+  words[j] = ((uint64_t)u << (tb + 6)) | ((uint64_t)tile << 6) | (uint64_t)(m_r & 63u);
+  vals[j] = p;
+}
+
+// words sorted by (u, tile, class) -> rank of each record within its
+// (u, tile, class) run
+__global__ void k_vote_ranks(const uint64_t *__restrict__ words, const uint32_t *__restrict__ toff,
+                             int32_t T, int tb, int64_t N, uint32_t *__restrict__ rank) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint32_t tcm = (1u << (tb + 6)) - 1u;
+  const uint64_t w = words[j];
+  const uint32_t tc = (uint32_t)w & tcm;  // tile << 6 | class
+  const uint64_t u = w >> (tb + 6);
   // first record of this (tile, class) in the bucket's tile segment
-  uint32_t a = toff[lo * (T + 1) + (tc >> 6)], b = (uint32_t)j;
+  uint32_t a = toff[u * (uint64_t)(T + 1) + (tc >> 6)], b = (uint32_t)j;
   while (a < b) {
     const uint32_t mid = (a + b) >> 1;
-    if (((uint32_t)(words[mid] >> 19) & 0x1fffu) < tc)
+    if (((uint32_t)words[mid] & tcm) < tc)
       a = mid + 1;
     else
       b = mid;
   }
-  const uint32_t rank = min((uint32_t)j - a, (1u << 19) - 1u);
-  out[j] = (w & 0xffffffff00000000ull) | ((uint64_t)(tc >> 6) << 25) | ((uint64_t)rank << 6) |
-           (uint64_t)(tc & 63u);
+  rank[j] = (uint32_t)j - a;
+}
+
+// the last record of each (u, tile, class) run writes the run's length
+__global__ void k_vote_run_counts(const uint64_t *__restrict__ words, const uint32_t *__restrict__ rank,
+                                  int64_t N, int32_t T, int tb, uint32_t *__restrict__ cnt) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint64_t w = words[j];
+  if (j + 1 < N && words[j + 1] == w) return;
+  const uint64_t u = w >> (tb + 6);
+  const uint64_t tile = (w >> 6) & ((1u << tb) - 1u);
+  cnt[((u * (uint64_t)T + tile) << 6) + (w & 63u)] = rank[j] + 1u;
+}
+
+// Within a (u, tile) segment with n_c records of class c, the record of rank
+// r and class c goes after every record of lower rank (sum over classes of
+// min(n_c', r)) and after the rank-r records of lower classes.
+__global__ void k_vote_place(const uint64_t *__restrict__ words, const uint32_t *__restrict__ rank,
+                             const uint32_t *__restrict__ vals_in, const uint32_t *__restrict__ toff,
+                             const uint32_t *__restrict__ cnt, int64_t N, int32_t T, int tb,
+                             uint32_t *__restrict__ vals_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint64_t w = words[j];
+  const uint64_t u = w >> (tb + 6);
+  const uint64_t tile = (w >> 6) & ((1u << tb) - 1u);
+  const uint32_t c = (uint32_t)(w & 63u), r = rank[j];
+  const uint4 *n4 = reinterpret_cast<const uint4 *>(cnt + ((u * (uint64_t)T + tile) << 6));
+  uint32_t pos = toff[u * (uint64_t)(T + 1) + tile];
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const uint4 n = n4[k];
+    const uint32_t nk[4] = {n.x, n.y, n.z, n.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pos += min(nk[e], r) + ((uint32_t)(4 * k + e) < c && nk[e] > r ? 1u : 0u);
+  }
+  vals_out[pos] = vals_in[j];
 }
 
 // --------------------------------------------------------------- voting
@@ -1300,24 +1344,36 @@ int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcou
 int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const CloudDev &m,
                int32_t TM, int32_t T, const uint32_t *hfirst, const uint32_t *hcount, int64_t U,
                uint32_t *toff) {
-  if (N <= 0) return 0;
+  if (N <= 0 || U <= 0) return 0;
+  auto bits = [](uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; };
+  const int ub = bits((uint64_t)U - 1), tb = bits((uint64_t)T - 1);
   hipLaunchKernelGGL(k_vote_order_words, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, vals, N,
-                     (int32_t)m.n, TM);
+                     (int32_t)m.n, TM, hfirst, U, tb);
   PPF_HIP(hipGetLastError());
-  PPF_TRY(radix_sort_u64(words, vals, N, 19, 64, ctx->sort, ctx->stream));
-  if (U <= 0) return 0;
+  PPF_TRY(radix_sort_u64(words, vals, N, 0, ub + tb + 6, ctx->sort, ctx->stream));
   // tile offsets (the same after the second sort: it reorders within segments)
   hipLaunchKernelGGL(k_tile_offsets, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, hfirst, hcount, vals,
                      U, (int32_t)m.n, TM, T, toff);
   PPF_HIP(hipGetLastError());
-  // ranks read other records' words, so write the new words to a copy
-  PPF_TRY(ctx->tmp_a.ensure((size_t)N * 8));
-  uint64_t *w2 = ctx->tmp_a.as<uint64_t>();
-  PPF_HIP(hipMemcpyAsync(w2, words, (size_t)N * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_vote_rank_words2, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, w2, hfirst,
-                     U, toff, T, N);
+  // vote order within each (u, tile) segment: (rank in class, class), placed
+  // directly from the run lengths
+  PPF_TRY(ctx->tmp_a.ensure((size_t)N * 4));
+  uint32_t *rank = ctx->tmp_a.as<uint32_t>();
+  hipLaunchKernelGGL(k_vote_ranks, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, toff, T, tb, N,
+                     rank);
   PPF_HIP(hipGetLastError());
-  PPF_TRY(radix_sort_u64(w2, vals, N, 0, 64, ctx->sort, ctx->stream));
+  const size_t n_cnt = (size_t)U * (size_t)T * 64;
+  PPF_TRY(ctx->sort.tmp_keys.ensure(n_cnt * 4));
+  PPF_TRY(ctx->sort.tmp_vals.ensure((size_t)N * 4));
+  uint32_t *cnt = ctx->sort.tmp_keys.as<uint32_t>();
+  uint32_t *vcopy = ctx->sort.tmp_vals.as<uint32_t>();
+  PPF_HIP(hipMemsetAsync(cnt, 0, n_cnt * 4, ctx->stream));
+  PPF_HIP(hipMemcpyAsync(vcopy, vals, (size_t)N * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_vote_run_counts, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, rank, N, T,
+                     tb, cnt);
+  hipLaunchKernelGGL(k_vote_place, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, rank, vcopy, toff,
+                     cnt, N, T, tb, vals);
+  PPF_HIP(hipGetLastError());
   return 0;
 }
 
