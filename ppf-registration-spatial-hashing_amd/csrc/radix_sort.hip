@@ -23,7 +23,7 @@ constexpr int RS_BLOCK = 256;
 constexpr int RS_WAVES = RS_BLOCK / WAVE;
 constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;  // 2048
-constexpr int RS_MAX_BLOCKS = 1024;
+constexpr int RS_MAX_BLOCKS = 4096;
 
 __global__ __launch_bounds__(RS_BLOCK) void k_radix_count(const uint64_t *__restrict__ keys,
                                                           int64_t n, int shift,
@@ -37,14 +37,13 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_count(const uint64_t *__rest
   const int64_t t1 = min(t0 + tiles_per_block, n_tiles);
   for (int64_t t = t0; t < t1; ++t) {
     const int64_t base = t * RS_TILE;
+    // all loads first (lanes past the end re-read the last key), then count
+    uint64_t kk[RS_ITEMS];
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-      int64_t idx = base + (int64_t)i * RS_BLOCK + tid;
-      if (idx < n) {
-        uint32_t d = (uint32_t)(keys[idx] >> shift) & 255u;
-        atomicAdd(&h[w][d], 1u);
-      }
-    }
+    for (int i = 0; i < RS_ITEMS; ++i) kk[i] = keys[min(base + (int64_t)i * RS_BLOCK + tid, n - 1)];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+      if (base + (int64_t)i * RS_BLOCK + tid < n) atomicAdd(&h[w][(uint32_t)(kk[i] >> shift) & 255u], 1u);
   }
   __syncthreads();
   uint32_t s = 0;
@@ -74,12 +73,17 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
     uint64_t k[RS_ITEMS];
     uint32_t v[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
+    // all loads first (lanes past the end re-read the last element)
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const int64_t idx = min(base_w + (int64_t)i * WAVE + l, n - 1);
+      k[i] = kin[idx];
+      if (VALS) v[i] = vin[idx];
+    }
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
       const int64_t idx = base_w + (int64_t)i * WAVE + l;
       const bool valid = idx < n;
-      k[i] = valid ? kin[idx] : 0ull;
-      if (VALS) v[i] = valid ? vin[idx] : 0u;
       const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
       uint64_t peers = __ballot(valid);
 #pragma unroll
