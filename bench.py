@@ -93,6 +93,45 @@ def cpu_baseline(w, budget_s: float):
                 votes_per_s=tot_v / tot_s)
 
 
+def run_large_hash(ctx, rpdf=100):
+    """Large-hash stress workload: one 30,000-point superquadric with 8 bumps
+    (9e8 model pairs) and a 200,000-point scene with three instances and 40%
+    clutter.  The model build (keys, hash-array sort, vote order, records) is
+    timed in full, twice; voting runs on every 100th scene point (2,000
+    reference points, 4e8 lookups) rather than every 5th (40,000 reference
+    points, 8e9 lookups, about four minutes of voting), so that the bench
+    stays within minutes; throughput per lookup is what it reports."""
+    w = synth.large_hash()
+    builds = []
+    m = None
+    for _ in range(2):
+        if m is not None:
+            m.close()
+        t = time.perf_counter()
+        m = ppf_amd.Model(w.models[0], w.d_dists[0], w.vote_count_threshold, ctx=ctx)
+        builds.append((time.perf_counter() - t) * 1e3)
+    info = m.info()
+    log(f"large-hash model built: {builds[-1]:.0f} ms")
+    scene = ppf_amd.Scene(w.scenes[0], 0.0, rpdf, ctx=ctx)
+    t = time.perf_counter()
+    r = m.ppf_lookup(scene)
+    r.pose()
+    dt = time.perf_counter() - t
+    i = dict(r.info)
+    r.close()
+    scene.close()
+    m.close()
+    log(f"large-hash sample lookup: {dt:.1f}s")
+    return {"model_points": len(w.models[0][0]), "model_pairs": info["n_pairs"],
+            "unique_keys": info["n_unique_keys"], "model_build_ms": builds[-1],
+            "model_pairs_per_s": info["n_pairs"] / (builds[-1] * 1e-3),
+            "scene_points": len(w.scenes[0][0]), "ref_point_df": rpdf,
+            "lookups": i["lookups"], "votes": i["votes"], "vote_ms": i["vote_ms"],
+            "lookups_per_s": i["lookups"] / (i["vote_ms"] * 1e-3),
+            "votes_per_s": i["votes"] / (i["vote_ms"] * 1e-3),
+            "n_kept": i["n_kept"], "cluster_ms": i["cluster_ms"], "lookup_s": dt}
+
+
 def run_forms(ctx, plain_models, plain_scene, w, log_every=True):
     """Batched and served forms beside the plain pipeline.
 
@@ -108,6 +147,7 @@ def run_forms(ctx, plain_models, plain_scene, w, log_every=True):
        the dataset-like models; flat-faced superquadrics with ~0.1 (huge
        buckets) keep 10^7-10^8 votes per job and take minutes per job in
        clustering, in the reference's algorithm as here.
+    3. The large-hash stress workload (run_large_hash).
     """
     out = {}
     # ---- 1. dataset-like jobs batched
@@ -229,6 +269,7 @@ def run_forms(ctx, plain_models, plain_scene, w, log_every=True):
         "server": {"rounds": st["rounds"], "max_requests_in_round": st["max_requests_in_round"],
                    "peak_inflight_bytes": st["peak_inflight_bytes"]},
     }
+    out["large_hash_sample"] = run_large_hash(ctx)
     return out
 
 
@@ -243,6 +284,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and served forms")
     ap.add_argument("--forms-only", action="store_true",
                     help="only the batched and served forms (no plain steps, no CPU baseline)")
+    ap.add_argument("--large-hash-only", action="store_true",
+                    help="only the large-hash stress workload (30,000-point model, 200,000-point scene)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     args = ap.parse_args()
@@ -253,6 +296,9 @@ def main():
             b = ctx.debug_copy(a)
         assert np.array_equal(a, b)
         log(f"copied {a.nbytes} bytes x3")
+        return
+    if args.large_hash_only:
+        print(json.dumps({"large_hash_sample": run_large_hash(ppf_amd.Context(0))}), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
