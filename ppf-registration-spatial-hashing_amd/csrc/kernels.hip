@@ -1036,17 +1036,20 @@ __device__ ClWave cluster_slot_bounds(const ClPose &p, bool have, float d_dist) 
   return W;
 }
 
-__device__ __forceinline__ void cl_test(ClPose &P, const float4 q, const float4 t, float s_max,
-                                        int use_l1, int use_avg) {
-  ppfc::cluster_test(P, q.x, q.y, q.z, q.w, t.x, t.y, t.z, t.w, s_max, use_l1, use_avg);
+// The flags are template parameters: compiled in as constants, the test of
+// the common case (no averaging) carries none of the averaging's code.
+template <int USE_L1, int USE_AVG>
+__device__ __forceinline__ void cl_test(ClPose &P, const float4 q, const float4 t, float s_max) {
+  ppfc::cluster_test(P, q.x, q.y, q.z, q.w, t.x, t.y, t.z, t.w, s_max, USE_L1, USE_AVG);
 }
 
+template <int use_l1, int use_avg>
 __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
     const float *__restrict__ trans, const float *__restrict__ quats, const float4 *__restrict__ cq,
     const float4 *__restrict__ ct, const uint32_t *__restrict__ adj,
     const uint32_t *__restrict__ ukeys, const uint32_t *__restrict__ ufirst,
     const uint32_t *__restrict__ ucount, int64_t U, const uint32_t *__restrict__ chunk_start,
-    const uint32_t *__restrict__ centers, float s_max, float d_dist, int use_l1, int use_avg,
+    const uint32_t *__restrict__ centers, float s_max, float d_dist,
     float *__restrict__ vc_out, float *__restrict__ trans_out,
     unsigned long long *__restrict__ n_tests) {
   __shared__ float4 s_q[CL_BLOCK];
@@ -1132,8 +1135,8 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
             const uint32_t b = (uint32_t)__builtin_ctzll(mask);
             mask &= mask - 1;
             const float4 q0 = s_q[g0 + b], t0 = s_t[g0 + b];
-            if ((m0 >> b) & 1ull) cl_test(P[0], q0, t0, s_max, use_l1, use_avg);
-            if ((m1 >> b) & 1ull) cl_test(P[1], q0, t0, s_max, use_l1, use_avg);
+            if ((m0 >> b) & 1ull) cl_test<use_l1, use_avg>(P[0], q0, t0, s_max);
+            if ((m1 >> b) & 1ull) cl_test<use_l1, use_avg>(P[1], q0, t0, s_max);
           }
         }
         __syncthreads();
@@ -1151,7 +1154,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
         const uint32_t bf = ufirst[bu], bn = ucount[bu];
         tests += bn;
         for (uint32_t jj = 0; jj < bn; ++jj)
-          cl_test(P[r], cq[bf + jj], ct[bf + jj], s_max, use_l1, use_avg);
+          cl_test<use_l1, use_avg>(P[r], cq[bf + jj], ct[bf + jj], s_max);
       }
     }
   }
@@ -1567,10 +1570,13 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   uint32_t n_items = 0;
   PPF_HIP(hipMemcpyAsync(&n_items, ctot, 4, hipMemcpyDeviceToHost, ctx->stream));
   PPF_HIP(hipStreamSynchronize(ctx->stream));
-  if (n_items > 0)
-    hipLaunchKernelGGL(k_cluster_tiled, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, cq,
-                       ct, adj, ukeys, ufirst, ucount, U, cstart, centers, sqrt_below_max(d_dist),
-                       d_dist, use_l1 ? 1 : 0, use_avg ? 1 : 0, vc, trans_out, ntest);
+  if (n_items > 0) {
+    auto kern = use_avg ? (use_l1 ? k_cluster_tiled<1, 1> : k_cluster_tiled<0, 1>)
+                        : (use_l1 ? k_cluster_tiled<1, 0> : k_cluster_tiled<0, 0>);
+    hipLaunchKernelGGL(kern, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, cq, ct, adj,
+                       ukeys, ufirst, ucount, U, cstart, centers, sqrt_below_max(d_dist), d_dist, vc,
+                       trans_out, ntest);
+  }
   hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
   PPF_HIP(hipGetLastError());
   long long mi_t[2] = {-1, 0};

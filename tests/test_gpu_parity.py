@@ -150,6 +150,24 @@ def test_poses_and_spatial_clustering_match_oracle(ctx, medium):
     assert np.array_equal(_bits(res.pose()), _bits(opose))
 
 
+@pytest.mark.parametrize("use_l1,use_avg", [(True, False), (False, True), (True, True)])
+def test_clustering_flags_match_oracle(ctx, medium, use_l1, use_avg):
+    # use_l1_norm drops the translation test; use_averaged_clusters averages
+    # the translations of the passing members (kernel.cu:735-756)
+    m, s, d = medium
+    model = ppf_amd.Model(m, d, 0.3, use_l1_norm=use_l1, use_averaged_clusters=use_avg, ctx=ctx)
+    res = model.ppf_lookup(ppf_amd.Scene(s, d, 5, ctx=ctx))
+    codes, counts = res.votes()
+    T = res.transformations()
+    vc, tr, q = res.clusters()
+    ref = O.cluster_spatial(T, codes, counts, d, use_l1=use_l1, use_avg=use_avg)
+    assert np.array_equal(_bits(vc), _bits(ref["vote_counts"]))
+    assert np.array_equal(_bits(tr), _bits(ref["trans"]))
+    assert res.info["max_idx"] == ref["max_idx"]
+    opose, _ = O.register(m, s, d, 5, 0.3, False, use_l1=use_l1, use_avg=use_avg)
+    assert np.array_equal(_bits(res.pose()), _bits(opose))
+
+
 def test_cpu_clustering_pose_matches_oracle(ctx, medium):
     m, s, d = medium
     model = ppf_amd.Model(m, d, 0.3, cpu_clustering=True, ctx=ctx)
