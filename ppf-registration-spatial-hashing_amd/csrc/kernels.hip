@@ -393,9 +393,53 @@ __device__ __forceinline__ bool better_vote(uint32_t c, unsigned long long k, ui
   return c > bc || (c == bc && c > 0u && k < bk);
 }
 
+__device__ __forceinline__ int job_of_item(const VoteJob *__restrict__ jobs, int n_jobs, uint64_t it) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].item_begin <= it)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Estimated cost of every item (reference row): the bucket sizes of the keys
+// of every 16th partner, one wave per item.  Writes (~cost << 32 | item), so
+// that an ascending sort orders the items heaviest first: persistent blocks
+// then finish on the cheap rows instead of waiting for a late heavy one.
+constexpr int COST_STRIDE = 16;
+__global__ __launch_bounds__(256) void k_row_cost(const VoteJob *__restrict__ jobs, int n_jobs,
+                                                  uint64_t n_items, uint64_t *__restrict__ keys) {
+  const uint64_t item = (uint64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+  const int lane = threadIdx.x % WAVE;
+  if (item >= n_items) return;
+  const VoteJob &J = jobs[job_of_item(jobs, n_jobs, item)];
+  const uint32_t r = (uint32_t)(item - J.item_begin) + (uint32_t)J.r_begin;
+  const int32_t s_r = (int32_t)r * J.rpdf, n_s = J.n_s, T = J.T;
+  const float srx = gld(J.sx + s_r), sry = gld(J.sy + s_r), srz = gld(J.sz + s_r);
+  const float snx = gld(J.snx + s_r), sny = gld(J.sny + s_r), snz = gld(J.snz + s_r);
+  const float snn = gld(J.snn + s_r);
+  uint64_t c = 0;
+  for (int32_t s_i = lane * COST_STRIDE; s_i < n_s; s_i += WAVE * COST_STRIDE) {
+    if (s_i == s_r) continue;
+    float f[4];
+    ppfg::ppf_feature_q(srx, sry, srz, snx, sny, snz, snn, gld(J.sx + s_i), gld(J.sy + s_i), gld(J.sz + s_i),
+                        gld(J.snx + s_i), gld(J.sny + s_i), gld(J.snz + s_i), gld(J.snn + s_i), J.d_dist, f);
+    const uint32_t key = ppfm::ppf_key(f[0], f[1], f[2], f[3]);
+    if (key == 0u) continue;
+    const uint32_t u = htab_find(J.htab, J.hshift, J.hmask, key);
+    if (u == 0xffffffffu) continue;
+    c += gld(J.toff + (size_t)u * (T + 1) + T) - gld(J.toff + (size_t)u * (T + 1));
+  }
+  c = wave_sum_u64(c);
+  if (lane == 0) keys[item] = ((uint64_t)(0xffffffffu - (uint32_t)min(c, (uint64_t)0xffffffffu)) << 32) | item;
+}
+
 template <bool EXACT>
 __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
-                                                     uint64_t n_items,
+                                                     uint64_t n_items, const uint64_t *__restrict__ order,
                                                      unsigned long long *__restrict__ queue,
                                                      uint4 *__restrict__ hit_scratch,
                                                      uint32_t hit_cap, uint32_t stride_u4,
@@ -414,19 +458,10 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
 
   for (;;) {
     if (tid == 0) {
-      const unsigned long long it = atomicAdd(queue, 1ull);
+      unsigned long long it = atomicAdd(queue, 1ull);
+      if (it < n_items && order) it = (uint32_t)order[it];
       sh.item = it;
-      if (it < n_items) {
-        int lo = 0, hi = n_jobs - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (jobs[mid].item_begin <= it)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
-        sh.job = lo;
-      }
+      if (it < n_items) sh.job = job_of_item(jobs, n_jobs, it);
       sh.nh = 0;
     }
     __syncthreads();
@@ -1425,8 +1460,19 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
                          ctx->stream));
   PPF_HIP(hipMemsetAsync(ctx->queue.ptr, 0, 16, ctx->stream));
   PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  // heaviest rows first, when there are more rows than blocks
+  const uint64_t *order = nullptr;
+  if (n_items > (uint64_t)G && n_items < ((uint64_t)1 << 32)) {
+    PPF_TRY(ctx->tmp_c.ensure((size_t)n_items * 8));
+    uint64_t *keys = ctx->tmp_c.as<uint64_t>();
+    hipLaunchKernelGGL(k_row_cost, dim3((unsigned)((n_items + 3) / 4)), dim3(256), 0, ctx->stream,
+                       ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items, keys);
+    PPF_HIP(hipGetLastError());
+    PPF_TRY(radix_sort_u64(keys, nullptr, (int64_t)n_items, 32, 64, ctx->sort, ctx->stream));
+    order = keys;
+  }
   hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream,
-                     ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items,
+                     ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items, order,
                      ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4,
                      misc_words);
   PPF_HIP(hipGetLastError());
