@@ -971,6 +971,45 @@ __global__ void k_cluster_order_keys(const float *__restrict__ quats, const uint
   ovals[k] = i;
 }
 
+// Estimated cost of every clustering chunk: its poses times the members of
+// its cell's 26 neighbour buckets.  Writes (~cost << 32 | chunk), so that an
+// ascending sort launches the heaviest chunks first.
+__global__ void k_cluster_chunk_cost(const uint32_t *__restrict__ chunk_start, int64_t U,
+                                     uint32_t n_chunks, const uint32_t *__restrict__ ukeys,
+                                     const uint32_t *__restrict__ ufirst, const uint32_t *__restrict__ ucount,
+                                     const uint32_t *__restrict__ centers, const uint32_t *__restrict__ adj,
+                                     uint64_t *__restrict__ keys) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_chunks) return;
+  int64_t lo = 0, hi = U - 1;  // cell: last u with chunk_start[u] <= b
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (chunk_start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const uint32_t c = b - chunk_start[lo], n = ucount[lo];
+  const uint32_t poses = min(512u, n - c * 512u);
+  const uint32_t i0 = centers[ufirst[lo]];
+  uint64_t nb = 0;
+  for (int q = 0; q < 27; ++q) {
+    const uint32_t h = adj[27 * (size_t)i0 + q];
+    if (h == 0u) continue;
+    int64_t a = 0, e = U;
+    while (a < e) {
+      const int64_t mid = (a + e) >> 1;
+      if (ukeys[mid] < h)
+        a = mid + 1;
+      else
+        e = mid;
+    }
+    if (a < U && ukeys[a] == h) nb += ucount[a];
+  }
+  const uint64_t cost = min((uint64_t)poses * nb, (uint64_t)0xffffffffu);
+  keys[b] = ((uint64_t)(0xffffffffu - (uint32_t)cost) << 32) | b;
+}
+
 // Member data in cell order: quaternion, and translation + weight.
 __global__ void k_cluster_gather(const uint32_t *__restrict__ members, int64_t P,
                                  const float *__restrict__ trans, const float *__restrict__ quats,
@@ -1084,7 +1123,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
     const float4 *__restrict__ ct, const uint32_t *__restrict__ adj,
     const uint32_t *__restrict__ ukeys, const uint32_t *__restrict__ ufirst,
     const uint32_t *__restrict__ ucount, int64_t U, const uint32_t *__restrict__ chunk_start,
-    const uint32_t *__restrict__ centers, float s_max, float d_dist,
+    const uint32_t *__restrict__ centers, const uint64_t *__restrict__ order, float s_max, float d_dist,
     float *__restrict__ vc_out, float *__restrict__ trans_out,
     unsigned long long *__restrict__ n_tests) {
   __shared__ float4 s_q[CL_BLOCK];
@@ -1094,8 +1133,8 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   __shared__ int64_t s_bucket;
   const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
   if (tid == 0) {
-    // which (cell, chunk) this block owns
-    const uint32_t b = blockIdx.x;
+    // which (cell, chunk) this block owns (heaviest chunks first)
+    const uint32_t b = order ? (uint32_t)order[blockIdx.x] : blockIdx.x;
     int64_t lo = 0, hi = U - 1;
     while (lo < hi) {
       const int64_t mid = (lo + hi + 1) >> 1;
@@ -1617,11 +1656,17 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   PPF_HIP(hipMemcpyAsync(&n_items, ctot, 4, hipMemcpyDeviceToHost, ctx->stream));
   PPF_HIP(hipStreamSynchronize(ctx->stream));
   if (n_items > 0) {
+    PPF_TRY(ctx->tmp_c.ensure((size_t)n_items * 8));
+    uint64_t *order = ctx->tmp_c.as<uint64_t>();
+    hipLaunchKernelGGL(k_cluster_chunk_cost, dim3(nblk(n_items, 256)), dim3(256), 0, ctx->stream, cstart, U,
+                       n_items, ukeys, ufirst, ucount, centers, adj, order);
+    PPF_HIP(hipGetLastError());
+    PPF_TRY(radix_sort_u64(order, nullptr, n_items, 32, 64, ctx->sort, ctx->stream));
     auto kern = use_avg ? (use_l1 ? k_cluster_tiled<1, 1> : k_cluster_tiled<0, 1>)
                         : (use_l1 ? k_cluster_tiled<1, 0> : k_cluster_tiled<0, 0>);
     hipLaunchKernelGGL(kern, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, cq, ct, adj,
-                       ukeys, ufirst, ucount, U, cstart, centers, sqrt_below_max(d_dist), d_dist, vc,
-                       trans_out, ntest);
+                       ukeys, ufirst, ucount, U, cstart, centers, order, sqrt_below_max(d_dist), d_dist,
+                       vc, trans_out, ntest);
   }
   hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
   PPF_HIP(hipGetLastError());
