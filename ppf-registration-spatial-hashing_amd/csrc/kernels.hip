@@ -924,12 +924,17 @@ __global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long 
                                                          : ppfm::cell_hash(c[0] + a, c[1] + b, c[2] + k);
 }
 
-// Work items of the tiled clustering: (unique cell hash u, 512-pose chunk).
+// Clustering blocks: CL_PER poses per thread, CL_CHUNK poses per block.
+constexpr int CL_BLOCK = 256;
+constexpr int CL_PER = 2;
+constexpr int CL_CHUNK = CL_BLOCK * CL_PER;
+
+// Work items of the tiled clustering: (unique cell hash u, CL_CHUNK-pose chunk).
 __global__ void k_cluster_chunks(const uint32_t *__restrict__ ucount, int64_t U,
                                  uint32_t *__restrict__ nchunk) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= U) return;
-  nchunk[u] = (ucount[u] + 511u) / 512u;
+  nchunk[u] = (ucount[u] + (uint32_t)CL_CHUNK - 1u) / (uint32_t)CL_CHUNK;
 }
 
 // Order of the poses of a cell for forming clustering chunks: a 4-D Morton
@@ -990,7 +995,7 @@ __global__ void k_cluster_chunk_cost(const uint32_t *__restrict__ chunk_start, i
       hi = mid - 1;
   }
   const uint32_t c = b - chunk_start[lo], n = ucount[lo];
-  const uint32_t poses = min(512u, n - c * 512u);
+  const uint32_t poses = min((uint32_t)CL_CHUNK, n - c * (uint32_t)CL_CHUNK);
   const uint32_t i0 = centers[ufirst[lo]];
   uint64_t nb = 0;
   for (int q = 0; q < 27; ++q) {
@@ -1047,9 +1052,7 @@ __global__ void k_cluster_gather(const uint32_t *__restrict__ members, int64_t P
 // above float rounding (5e-3 rad, 1e-5 relative), and a member is skipped
 // only when a bound proves that it fails for all poses of the wave; skipped
 // members would add nothing, so every sum keeps its order and bits.
-constexpr int CL_BLOCK = 256;
-constexpr int CL_PER = 2;
-constexpr int CL_CHUNK = CL_BLOCK * CL_PER;
+
 using ppfc::ClPose;
 using ppfc::ClWave;
 
