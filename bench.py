@@ -161,7 +161,8 @@ def run_forms(ctx, plain_models, plain_scene, w, log_every=True):
     out["dataset_like_batched"] = {"jobs": len(plain_models), "s": dt,
                                    "registrations_per_s": len(plain_models) / dt,
                                    "lookups_per_s": info["lookups"] / dt,
-                                   "vote_launches": info["vote_launches"]}
+                                   "vote_launches": info["vote_launches"], "vote_ms": info["vote_ms"],
+                                   "finish_ms": info["finish_ms"]}
     log(f"batched dataset-like: {dt:.2f}s")
 
     # ---- 2. request set
@@ -264,7 +265,8 @@ def run_forms(ctx, plain_models, plain_scene, w, log_every=True):
         "plain": {"s": plain_s, "registrations_per_s": n_targets / plain_s, "lookups_per_s": lookups / plain_s},
         "batched": {"s": batch_s, "registrations_per_s": n_targets / batch_s,
                     "lookups_per_s": binfo["lookups"] / batch_s,
-                    "speedup_vs_plain": plain_s / batch_s, "vote_launches": binfo["vote_launches"]},
+                    "speedup_vs_plain": plain_s / batch_s, "vote_launches": binfo["vote_launches"],
+                    "vote_ms": binfo["vote_ms"], "finish_ms": binfo["finish_ms"]},
         "served": served,
         "server": {"rounds": st["rounds"], "max_requests_in_round": st["max_requests_in_round"],
                    "peak_inflight_bytes": st["peak_inflight_bytes"]},
