@@ -97,7 +97,7 @@ void ctx_release(ppf_ctx_s *c) {
   c->sort.scan.release();
   c->sort.tmp_keys.release();
   c->sort.tmp_vals.release();
-  for (DevBuf *b : {&c->tmp_a, &c->tmp_b, &c->tmp_c, &c->tmp_d, &c->hits, &c->queue}) b->release();
+  for (DevBuf *b : {&c->tmp_a, &c->tmp_b, &c->tmp_c, &c->tmp_d, &c->hits, &c->queue, &c->sel}) b->release();
   for (hipEvent_t *ev : {&c->ev0, &c->ev1, &c->ev2, &c->ev3, &c->ev4, &c->ev5, &c->ev6})
     if (*ev) {
       (void)hipEventDestroy(*ev);
@@ -379,10 +379,11 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
   r->n_m = m->cloud.n;
   r->cpu = p->cpu_clustering != 0;
   r->info.max_idx = -1;
-  ppfi::DevBuf sel;
+  // a buffer of the context: a freed device buffer would synchronize the
+  // whole device, and lanes finish lookups concurrently
   const size_t sel_n = std::max<uint32_t>(n_cand, 1);
-  TRY(sel.ensure(sel_n * 12));
-  unsigned long long *scode = sel.as<unsigned long long>();
+  TRY(ctx->sel.ensure(sel_n * 12));
+  unsigned long long *scode = ctx->sel.as<unsigned long long>();
   uint32_t *scount = reinterpret_cast<uint32_t *>(scode + sel_n);
   int64_t P = 0;
   HIPC(hipEventRecord(ctx->ev4, st));

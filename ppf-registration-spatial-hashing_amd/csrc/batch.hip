@@ -133,7 +133,6 @@ int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_para
         err[li] = std::string("batch job ") + std::to_string(b.index) + ": " + ppf_last_error();
         break;
       }
-      b.cand.release();
       results[order[k]] = std::move(r);
     }
   };

@@ -34,6 +34,7 @@ struct ppf_ctx_s {
   size_t max_lds = 0;
   ppfi::SortScratch sort;
   ppfi::DevBuf tmp_a, tmp_b, tmp_c, tmp_d, hits, queue;
+  ppfi::DevBuf sel;  // kept votes of the lookup being finished (grows, never shrinks)
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
   std::vector<ppf_ctx_s *> lanes;  // root only
