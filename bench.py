@@ -391,6 +391,10 @@ def main():
             "traffic": vote_traffic,
             "alg_bytes_per_launch": alg_bytes / n_launch,
             "traffic_source": traffic_src,
+            # the counters' bytes over the same launch time: what HBM really moved
+            "measured_gbs": (vote_traffic / (vote_ms / n_launch * 1e-3) / 1e9) if vote_traffic else None,
+            "measured_frac": (vote_traffic / (vote_ms / n_launch * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                             if vote_traffic else None,
         },
     }
     if not args.no_forms:
