@@ -305,7 +305,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(voting would shard by scene reference point; see DESIGN.md)", file=sys.stderr)
+              "(voting would shard by scene reference point)", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.time()
