@@ -931,27 +931,34 @@ int ppf_registration(ppf_ctx ctx, int32_t n_scenes, const float *const *scene_po
     }
   }
   ppf_lookup_params prm{vote_count_threshold, cpu_clustering, use_l1_norm, use_averaged_clusters, 0};
-  for (int32_t i = 0; i < n_scenes; ++i) {
+  // every model against one scene in one batched lookup: one vote launch,
+  // the finishing stages overlapped on lanes; each pose is the one
+  // ppf_lookup gives for the pair
+  std::vector<ppf_scene> sv((size_t)n_models);
+  std::vector<ppf_result> res((size_t)n_models, nullptr);
+  for (int32_t i = 0; i < n_scenes && n_models > 0; ++i) {
     ppf_scene sc = nullptr;
     int rc = ppf_scene_upload(ctx, scene_points[i], scene_normals[i], scene_sizes[i], rpdf, &sc);
     if (rc) {
       cleanup();
       return rc;
     }
-    for (int32_t j = 0; j < n_models; ++j) {
-      ppf_result res = nullptr;
-      rc = ppf_lookup(models[j], sc, &prm, &res);
+    std::fill(sv.begin(), sv.end(), sc);
+    rc = ppf_batch_lookup(ctx, n_models, models.data(), sv.data(), &prm, res.data(), nullptr);
+    for (int32_t j = 0; j < n_models && rc == PPF_OK; ++j) {
       float *dst = poses_out + ((size_t)i * n_models + j) * 16;
-      if (rc) {
-        ppf_scene_destroy(sc);
-        cleanup();
-        return rc;
-      }
-      int st = ppf_result_pose(res, dst);
+      const int st = ppf_result_pose(res[(size_t)j], dst);
       if (status_out) status_out[(size_t)i * n_models + j] = st;
-      ppf_result_destroy(res);
+    }
+    for (auto &r : res) {
+      ppf_result_destroy(r);
+      r = nullptr;
     }
     ppf_scene_destroy(sc);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
   }
   cleanup();
   return PPF_OK;
