@@ -1436,13 +1436,16 @@ int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const Cl
                      U, (int32_t)m.n, TM, T, toff);
   PPF_HIP(hipGetLastError());
   // vote order within each (u, tile) segment: (rank in class, class), placed
-  // directly from the run lengths
+  // directly from the run lengths.  It only spreads LDS banks; with more
+  // (u, tile, class) runs than records (many tiny buckets) the run table would
+  // outgrow the records, and the (u, tile, class, pair) order stays.
+  const size_t n_cnt = (size_t)U * (size_t)T * 64;
+  if (n_cnt > (size_t)N) return 0;
   PPF_TRY(ctx->tmp_a.ensure((size_t)N * 4));
   uint32_t *rank = ctx->tmp_a.as<uint32_t>();
   hipLaunchKernelGGL(k_vote_ranks, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, words, toff, T, tb, N,
                      rank);
   PPF_HIP(hipGetLastError());
-  const size_t n_cnt = (size_t)U * (size_t)T * 64;
   PPF_TRY(ctx->sort.tmp_keys.ensure(n_cnt * 4));
   PPF_TRY(ctx->sort.tmp_vals.ensure((size_t)N * 4));
   uint32_t *cnt = ctx->sort.tmp_keys.as<uint32_t>();
