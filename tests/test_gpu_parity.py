@@ -97,6 +97,18 @@ def multi_tile():
     return w.models[0], w.scenes[0], w.d_dists[0]
 
 
+def test_multi_tile_votes_match_oracle(ctx, multi_tile):
+    # 1,300 model points in 3 tiles: the vote order placed from run lengths
+    m, s, d = multi_tile
+    model = ppf_amd.Model(m, d, 0.1, ctx=ctx)
+    assert model.info()["n_tiles"] == 3
+    res = model.ppf_lookup(ppf_amd.Scene(s, d, 5, ctx=ctx))
+    codes, counts = res.votes()
+    oc, on, st = O.vote(m, s, d, 5, 0.1)
+    assert res.info["votes"] == st["votes"] and res.info["max_count"] == st["max_count"]
+    assert np.array_equal(codes, oc) and np.array_equal(counts, on)
+
+
 @pytest.mark.parametrize("exact", [False, True])
 def test_per_reference_argmax_matches_oracle(ctx, multi_tile, exact):
     """Score pass (include/kernel.h:78-82): one bin per reference row, the
