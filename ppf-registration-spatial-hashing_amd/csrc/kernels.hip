@@ -644,20 +644,28 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
               (uint32_t)__builtin_amdgcn_readfirstlane((int)min((uint32_t)WAVE, gc - s0));
           uint4 my = make_uint4(0u, 0u, 0u, 0u);
           if ((uint32_t)lane < sc) my = gld4(ghits + gs + s0 + lane);
-          for (uint32_t j0 = b + lane; j0 < e; j0 += VOTE_UNROLL * WAVE) {
+          for (uint32_t jb = b; jb < e; jb += VOTE_UNROLL * WAVE) {
             // per record: phi_u in 24-bit turns, byte address of its
-            // accumulator column, and the fast-path range (0: never fast)
+            // accumulator column, and the fast-path range (0: never fast).
+            // Slot q holds records jb + 64 q + lane; the first `left[q]`
+            // lanes of it are records (a uniform count, so the lane masks
+            // are scalar).  Lanes past the end re-read the last record.
+            const uint32_t j0 = jb + (uint32_t)lane;
             uint32_t ph[VOTE_UNROLL], row4[VOTE_UNROLL], rg[VOTE_UNROLL];
             uint64_t vmask[VOTE_UNROLL];
+            int32_t left[VOTE_UNROLL];
+            uint32_t rr[VOTE_UNROLL];
+#pragma unroll
+            for (int q = 0; q < VOTE_UNROLL; ++q) rr[q] = gld(rec + min(j0 + q * WAVE, e - 1u));
 #pragma unroll
             for (int q = 0; q < VOTE_UNROLL; ++q) {
-              const uint32_t j = j0 + q * WAVE;
-              const bool valid = j < e;
-              const uint32_t r = valid ? gld(rec + j) : 0u;
+              left[q] = (int32_t)(e - jb) - q * WAVE;
+              vmask[q] = left[q] >= WAVE ? ~0ull : (left[q] <= 0 ? 0ull : (1ull << left[q]) - 1ull);
+              const bool valid = lane < left[q];
+              const uint32_t r = rr[q];
               ph[q] = (r >> (REC_PHI_SHIFT - 2)) & 0xfffffcu;  // phi_u in 24-bit turns
               row4[q] = acc_base + ((r & REC_M_MASK) << 2);
               rg[q] = (valid && !(r & REC_SLOW_BIT)) ? FAST_RANGE : 0u;
-              vmask[q] = __ballot(valid);
             }
             for (uint32_t k = 0; k < sc; ++k) {
               const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)my.x, (int)k);
@@ -689,7 +697,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
                 const float vz = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.w, (int)k));
 #pragma unroll
                 for (int q = 0; q < VOTE_UNROLL; ++q) {
-                  bool need = ((vmask[q] >> lane) & 1u) != 0u;
+                  bool need = lane < left[q];
                   if (need && !EXACT && (hx >> 31) == 0u) {
                     const uint32_t a = (v24 - ph[q]) & 0xffffffu;
                     need = !(a * (30u << 8) - FAST_OFF < rg[q]);
