@@ -445,7 +445,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
                                                      uint32_t hit_cap, uint32_t stride_u4,
                                                      uint32_t misc_words) {
   // dynamic LDS only, so that the accumulator starts at LDS address 0:
-  // [accumulator tile | spare word | ...][VoteShared at misc_words]
+  // [accumulator tile | ...][VoteShared at misc_words]
   extern __shared__ uint32_t acc[];
   VoteShared &sh = *reinterpret_cast<VoteShared *>(acc + misc_words);
   const int tid = threadIdx.x, wv = tid / WAVE, lane = tid % WAVE;
@@ -609,13 +609,10 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
     unsigned long long wave_votes = 0;
     const int32_t T = J.T, TM = J.TM, n_m = J.n_m;
     // bin-major accumulator acc[bin * pitch + m], pitch = 2^k a multiple of
-    // 64: the LDS bank of an update is m mod 64.  LDS byte addresses:
-    // accumulator base, and 64 spare words (one per lane) past it.
+    // 64: the LDS bank of an update is m mod 64 (LDS byte addresses below)
     const uint32_t pitch = (uint32_t)J.acc_pitch, pitch4 = pitch * 4u;
     const uint32_t acc_base = (uint32_t)(uintptr_t)acc;
     const int32_t nb = N_BINS * (int32_t)pitch;
-    const uint32_t spare4 = acc_base + ((uint32_t)nb + (uint32_t)lane) * 4u;
-    const uint32_t one = 1u;
     const uint32_t *__restrict__ toff = J.toff;
     const uint32_t *__restrict__ rec = J.rec;
     for (int32_t t = 0; t < T; ++t) {
@@ -651,7 +648,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
             // lanes of it are records (a uniform count, so the lane masks
             // are scalar).  Lanes past the end re-read the last record.
             const uint32_t j0 = jb + (uint32_t)lane;
-            uint32_t ph[VOTE_UNROLL], row4[VOTE_UNROLL], rg[VOTE_UNROLL];
+            uint32_t ph[VOTE_UNROLL], row4[VOTE_UNROLL], rg[VOTE_UNROLL], inc[VOTE_UNROLL];
             uint64_t vmask[VOTE_UNROLL];
             int32_t left[VOTE_UNROLL];
             uint32_t rr[VOTE_UNROLL];
@@ -664,8 +661,12 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
               const bool valid = lane < left[q];
               const uint32_t r = rr[q];
               ph[q] = (r >> (REC_PHI_SHIFT - 2)) & 0xfffffcu;  // phi_u in 24-bit turns
-              row4[q] = acc_base + ((r & REC_M_MASK) << 2);
-              rg[q] = (valid && !(r & REC_SLOW_BIT)) ? FAST_RANGE : 0u;
+              const bool fastrec = valid && !(r & REC_SLOW_BIT);
+              // lanes without a fast record add 0, in their own bank's column
+              row4[q] = acc_base + ((fastrec ? (r & REC_M_MASK) : (uint32_t)lane) << 2);
+              rg[q] = fastrec ? FAST_RANGE : 0u;
+              inc[q] = fastrec ? 1u : 0u;
+              asm volatile("" : "+v"(inc[q]));  // keep it a register, not a mask
             }
             for (uint32_t k = 0; k < sc; ++k) {
               const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)my.x, (int)k);
@@ -678,17 +679,16 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
               } else {
 #pragma unroll
                 for (int q = 0; q < VOTE_UNROLL; ++q) {
-                  // a = 24-bit turns of theta + pi; a * 30 * 2^8 = (bin << 32) | frac
+                  // a = 24-bit turns of theta + pi; a * 30 * 2^8 = (bin << 32) | frac.
+                  // Every lane adds inc to the bin of a; the float path below
+                  // moves the undecided votes whose bin differs.
                   const uint32_t a = v24 - ph[q];
                   const bool ok = (a & 0xffffffu) * (30u << 8) - FAST_OFF < rg[q];
-                  const uint64_t okm = __ballot(ok);
-                  bad |= ~okm & vmask[q];
-                  // undecided lanes add to the spare word past the accumulator
-                  uint32_t bin, addr;
+                  bad |= ~__ballot(ok) & vmask[q];
+                  uint32_t bin;
                   asm("v_mul_hi_u32_u24 %0, 0x1e00, %1" : "=v"(bin) : "v"(a));
-                  addr = row4[q] + __umul24(bin, pitch4);
-                  asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(addr) : "v"(spare4), "v"(addr), "s"(okm));
-                  asm volatile("ds_add_u32 %0, %1" : : "v"(addr), "v"(one) : "memory");
+                  const uint32_t addr = row4[q] + __umul24(bin, pitch4);
+                  asm volatile("ds_add_u32 %0, %1" : : "v"(addr), "v"(inc[q]) : "memory");
                 }
               }
               if (bad) {
@@ -698,9 +698,14 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
 #pragma unroll
                 for (int q = 0; q < VOTE_UNROLL; ++q) {
                   bool need = lane < left[q];
+                  // bin the fast loop added this vote to (fast records only)
+                  bool moved = false;
+                  uint32_t approx = 0u;
                   if (need && !EXACT && (hx >> 31) == 0u) {
                     const uint32_t a = (v24 - ph[q]) & 0xffffffu;
                     need = !(a * (30u << 8) - FAST_OFF < rg[q]);
+                    moved = rg[q] != 0u;
+                    approx = (a * 30u) >> 24;
                   }
                   if (need) {
                     const uint32_t j = j0 + q * WAVE;
@@ -718,7 +723,10 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
                                          uy, uz);
                     const uint32_t bin = ppfg::alpha_bin(uy, uz, vy, vz);
                     const uint32_t mloc = m_r - (uint32_t)tile_base;
-                    atomicAdd(&acc[bin * pitch + mloc], 1u);
+                    if (!moved || bin != approx) {
+                      if (moved) atomicSub(&acc[approx * pitch + mloc], 1u);
+                      atomicAdd(&acc[bin * pitch + mloc], 1u);
+                    }
                   }
                 }
               }
@@ -1478,12 +1486,12 @@ int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mfr
 }
 
 // Words of dynamic LDS before VoteShared: the largest accumulator (31 rows of
-// pitch 2^k >= 64, or 2^k + 1 for model files written with that pitch) plus
-// 64 spare words, or the hit-grouping table, rounded to 8 bytes.
+// pitch 2^k >= 64, or 2^k + 1 for model files written with that pitch), or
+// the hit-grouping table, rounded to 8 bytes.
 uint32_t vote_misc_words(int max_tm) {
   uint32_t pw = 64;
   while ((int)pw < max_tm) pw <<= 1;
-  const uint32_t w = std::max((pw + 1u) * N_BINS + (uint32_t)WAVE, (uint32_t)2 * GT_SIZE);
+  const uint32_t w = std::max((pw + 1u) * N_BINS, (uint32_t)2 * GT_SIZE);
   return (w + 1u) & ~1u;
 }
 
