@@ -512,7 +512,8 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
               const double turns = atan2((double)vz, (double)vy) * (16777216.0 / 6.283185307179586);
               v32 = ((uint32_t)(long long)llrint(turns) + 0x800000u) & 0xffffffu;
             }
-            hr = make_uint4(u | flag, v32, ppfm::fbits(vy), ppfm::fbits(vz));
+            // the float-path flag rides on both u and v24 (bit 31)
+            hr = make_uint4(u | flag, v32 | flag, ppfm::fbits(vy), ppfm::fbits(vz));
           }
         }
       }
@@ -669,10 +670,11 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
               asm volatile("" : "+v"(inc[q]));  // keep it a register, not a mask
             }
             for (uint32_t k = 0; k < sc; ++k) {
-              const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)my.x, (int)k);
+              // 24-bit turns of phi_v + pi; bit 31: every vote of this hit
+              // takes the float path (degenerate v, or exact mode)
               const uint32_t v24 = (uint32_t)__builtin_amdgcn_readlane((int)my.y, (int)k);
               uint64_t bad = 0;
-              if (EXACT || (hx >> 31) != 0u) {
+              if (EXACT || (v24 >> 31) != 0u) {
                 // degenerate v (or exact mode): every vote of this hit exact
 #pragma unroll
                 for (int q = 0; q < VOTE_UNROLL; ++q) bad |= vmask[q];
@@ -701,7 +703,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
                   // bin the fast loop added this vote to (fast records only)
                   bool moved = false;
                   uint32_t approx = 0u;
-                  if (need && !EXACT && (hx >> 31) == 0u) {
+                  if (need && !EXACT && (v24 >> 31) == 0u) {
                     const uint32_t a = (v24 - ph[q]) & 0xffffffu;
                     need = !(a * (30u << 8) - FAST_OFF < rg[q]);
                     moved = rg[q] != 0u;
