@@ -32,10 +32,13 @@ struct ClPose {
 // float whose square root rounds below d_dist).  jt.w is the member's weight.
 PPF_HD void cluster_test(ClPose &P, float jqx, float jqy, float jqz, float jqw, float jtx, float jty,
                          float jtz, float wj, float s_max, int use_l1, int use_avg) {
+  // |8 (1 - d4)| < T is |1 - d4| < T/8: scaling by a power of two is exact
+  // in both products (and overflows to infinity on both sides)
   const float rot_thresh_sq = ROT_THRESH * ROT_THRESH;
+  const float rot_thresh_sq8 = rot_thresh_sq * 0.125f;
   const float d4 = P.qx * jqx + P.qy * jqy + P.qz * jqz + P.qw * jqw;
-  const float qd = fabsf(8.0f * (1.0f - d4));
-  if (qd < rot_thresh_sq) {
+  const float qd = fabsf(1.0f - d4);
+  if (qd < rot_thresh_sq8) {
     if (!use_l1) {
       const float dx = P.tx - jtx, dy = P.ty - jty, dz = P.tz - jtz;
       const float ss = dx * dx + dy * dy + dz * dz;
