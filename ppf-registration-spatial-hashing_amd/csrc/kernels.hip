@@ -384,6 +384,7 @@ struct VoteShared {
   unsigned long long wcode[VOTE_BLOCK / WAVE];
   uint32_t best;                       // the reference row's best so far
   unsigned long long best_code;
+  uint32_t cls[64];  // groups per work class, then the class cursors
 };
 
 // (count, code) order of the per-reference-row maximum: larger count, then
@@ -536,6 +537,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       tkey[i] = 0u;
       tcnt[i] = 0u;
     }
+    if (tid < 64) sh.cls[tid] = 0u;
     if (tid == 0) sh.ovf = 0u;
     __syncthreads();
     for (uint32_t h = tid; h < nh; h += VOTE_BLOCK) {
@@ -576,20 +578,40 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
         sh.wsum[wv][1] = ui;
       }
       __syncthreads();
-      uint32_t cb = ci - cs, ub = ui - us, utot = 0;
+      uint32_t cb = ci - cs, utot = 0;
       for (int k = 0; k < VOTE_BLOCK / WAVE; ++k) {
-        if (k < wv) {
-          cb += sh.wsum[k][0];
-          ub += sh.wsum[k][1];
-        }
+        if (k < wv) cb += sh.wsum[k][0];
         utot += sh.wsum[k][1];
       }
+      // heaviest groups first, so that the waves of a tile finish together:
+      // work class = floor(log2(hits x bucket records)), counted per class,
+      // then placed in descending class order (the order of the groups does
+      // not change any count: the accumulator adds are integer atomics)
+      uint32_t cl[PER];
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
+        cl[q] = 0u;
         if (c[q]) {
-          groups[ub] = make_uint2(cb, c[q]);
-          ++ub;
+          const uint32_t u = tkey[tid * PER + q] - 1u;
+          const uint32_t *tu = J.toff + (size_t)u * (J.T + 1);
+          const uint64_t wk = (uint64_t)c[q] * (uint64_t)(gld(tu + J.T) - gld(tu));
+          cl[q] = wk ? 63u - (uint32_t)__builtin_clzll(wk) : 0u;
+          atomicAdd(&sh.cls[cl[q]], 1u);
         }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t run = 0;
+        for (int k = 63; k >= 0; --k) {
+          const uint32_t n = sh.cls[k];
+          sh.cls[k] = run;
+          run += n;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (c[q]) groups[atomicAdd(&sh.cls[cl[q]], 1u)] = make_uint2(cb, c[q]);
         tcnt[tid * PER + q] = cb;  // cursor
         cb += c[q];
       }
