@@ -964,8 +964,10 @@ __global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long 
                                                          : ppfm::cell_hash(c[0] + a, c[1] + b, c[2] + k);
 }
 
-// Clustering blocks: CL_PER poses per thread, CL_CHUNK poses per block.
-constexpr int CL_BLOCK = 256;
+// Clustering blocks: CL_PER poses per thread, CL_CHUNK poses per block.  One
+// wave per block: the waves of a larger block wait for each other at every
+// member tile, and their pose slots test different numbers of members.
+constexpr int CL_BLOCK = 64;
 constexpr int CL_PER = 2;
 constexpr int CL_CHUNK = CL_BLOCK * CL_PER;
 
@@ -1069,9 +1071,8 @@ __global__ void k_cluster_gather(const uint32_t *__restrict__ members, int64_t P
 }
 
 // Poses that share a voxel cell share the 26 neighbour hashes, so one block
-// takes up to 512 poses of one cell (128 per wave, two per thread, in the
-// rotation order above) and streams every neighbour bucket through LDS
-// once.  Each thread walks the bucket in member order, which keeps the
+// takes up to CL_CHUNK poses of one cell (two per thread, in the rotation
+// order above) and streams every neighbour bucket through LDS once.  Each thread walks the bucket in member order, which keeps the
 // summation order of rot_clustering_kernel (kernel.cu:724-756).  Chunks
 // whose poses do not all lie in one cell (an FNV collision between cells)
 // walk their own neighbour lists per pose.
