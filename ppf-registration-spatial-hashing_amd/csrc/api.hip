@@ -556,7 +556,7 @@ int ppf_model_load(ppf_ctx ctx, const char *path, ppf_model *out) {
                 pitch_ok(h.TM, h.acc_pitch) && h.d_dist > 0.0f && h.hshift >= 1 && h.hshift <= 31 &&
                 (1ull << (32 - h.hshift)) == h.hcap,
             "ppf_model_load: inconsistent model file header");
-  const size_t lds_points = (ctx->max_lds - 1024) / (N_BINS * 4) / 32 * 32;
+  const size_t lds_points = (ctx->max_lds - VOTE_LDS_RESERVE) / (N_BINS * 4) / 32 * 32;
   ARG_CHECK((size_t)h.TM <= lds_points, "ppf_model_load: model tiles exceed this device's LDS");
   TRY(set_device(ctx));
   std::unique_ptr<ppf_model_s> m(new ppf_model_s());
@@ -644,7 +644,7 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   hipStream_t st = ctx->stream;
   TRY(upload_cloud(ctx, points, normals, n, m->cloud, &m->frames, 1, n));
   // accumulator tile: TM model points x 31 bins x 4 B must fit the LDS of one block
-  const int64_t lds_points = (int64_t)((ctx->max_lds - 1024) / (N_BINS * 4)) / 32 * 32;
+  const int64_t lds_points = (int64_t)((ctx->max_lds - VOTE_LDS_RESERVE) / (N_BINS * 4)) / 32 * 32;
   m->TM = (int32_t)std::min<int64_t>(n, std::min<int64_t>(DEFAULT_TILE_POINTS, lds_points));
   ARG_CHECK(m->TM >= 32, "ppf_model_build: device LDS too small for the vote accumulator");
   m->T = (int32_t)((n + m->TM - 1) / m->TM);

@@ -368,6 +368,9 @@ constexpr uint32_t FAST_OFF = 64425u;
 constexpr uint32_t FAST_RANGE = (uint32_t)(4294967296ull - 2ull * (FAST_OFF - 1u) - 1ull);
 static_assert(FAST_OFF - 1u == (uint32_t)(1.5e-5 * 4294967296.0), "fast-bin margin");
 constexpr int VOTE_UNROLL = 4;
+// Undecided votes of a wave wait in an LDS queue (record index, hit, the bin
+// the fast loop used) and take the float pipeline 64 at a time.
+constexpr uint32_t VQ_CAP = WAVE;
 constexpr int GT_LOG2 = 12;
 constexpr int GT_SIZE = 1 << GT_LOG2;  // hit-grouping table slots (2 x 16 KB of LDS)
 
@@ -386,6 +389,9 @@ struct VoteShared {
   unsigned long long best_code;
   uint32_t cls[64];  // groups per work class, then the class cursors
 };
+static_assert(sizeof(VoteShared) % 8 == 0, "the vote queues follow VoteShared");
+static_assert(sizeof(VoteShared) + (VOTE_BLOCK / WAVE) * VQ_CAP * 8 + 8 <= VOTE_LDS_RESERVE,
+              "vote block LDS beside the accumulator");
 
 // (count, code) order of the per-reference-row maximum: larger count, then
 // smaller code (the first maximum in code order)
@@ -638,8 +644,31 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
     const int32_t nb = N_BINS * (int32_t)pitch;
     const uint32_t *__restrict__ toff = J.toff;
     const uint32_t *__restrict__ rec = J.rec;
+    uint2 *wq = reinterpret_cast<uint2 *>(&sh + 1) + (size_t)wv * VQ_CAP;
     for (int32_t t = 0; t < T; ++t) {
       const int32_t tile_base = t * TM;
+      // the float pipeline for one vote: record j against a hit's v = (vy,
+      // vz); a vote the fast loop added to bin `approx` (moved) is moved
+      // when its exact bin differs
+      auto exact_vote = [&](uint32_t j, float vy, float vz, bool moved, uint32_t approx) {
+        const uint32_t pr = gld(J.pair + j);
+        const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
+        const float *Fm = J.mframe + (size_t)m_r * 16;
+        float r1[4], r2[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          r1[kk] = gld(Fm + 4 + kk);
+          r2[kk] = gld(Fm + 8 + kk);
+        }
+        float uy, uz;
+        ppfg::frame_apply_yz(r1, r2, gld(J.mx + m_i), gld(J.my + m_i), gld(J.mz + m_i), uy, uz);
+        const uint32_t bin = ppfg::alpha_bin(uy, uz, vy, vz);
+        const uint32_t mloc = m_r - (uint32_t)tile_base;
+        if (!moved || bin != approx) {
+          if (moved) atomicSub(&acc[approx * pitch + mloc], 1u);
+          atomicAdd(&acc[bin * pitch + mloc], 1u);
+        }
+      };
       for (int32_t i = tid; i < nb; i += VOTE_BLOCK) acc[i] = 0u;
       if (tid == 0) sh.next = 0;
       __syncthreads();
@@ -664,6 +693,20 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
               (uint32_t)__builtin_amdgcn_readfirstlane((int)min((uint32_t)WAVE, gc - s0));
           uint4 my = make_uint4(0u, 0u, 0u, 0u);
           if ((uint32_t)lane < sc) my = gld4(ghits + gs + s0 + lane);
+          // queued undecided votes: (record, hit k | approx bin << 6 | moved << 11)
+          uint32_t qn = 0;
+          auto flush = [&]() {
+            // the wave's queue writes before its reads (and the reads before
+            // later writes)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const bool act = (uint32_t)lane < qn;
+            const uint2 en = act ? wq[lane] : make_uint2(0u, 0u);
+            const int kk = (int)(en.y & 63u);
+            const float vy = ppfm::bitsf((uint32_t)__shfl((int)my.z, kk, WAVE));
+            const float vz = ppfm::bitsf((uint32_t)__shfl((int)my.w, kk, WAVE));
+            if (act) exact_vote(en.x, vy, vz, ((en.y >> 11) & 1u) != 0u, (en.y >> 6) & 31u);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          };
           for (uint32_t jb = b; jb < e; jb += VOTE_UNROLL * WAVE) {
             // per record: phi_u in 24-bit turns, byte address of its
             // accumulator column, and the fast-path range (0: never fast).
@@ -716,45 +759,40 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
                 }
               }
               if (bad) {
-                // rare: the exact float pipeline for the undecided votes
-                const float vy = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.z, (int)k));
-                const float vz = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.w, (int)k));
+                if (EXACT || (v24 >> 31) != 0u) {
+                  // every vote of this hit takes the float pipeline, now
+                  const float vy = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.z, (int)k));
+                  const float vz = ppfm::bitsf((uint32_t)__builtin_amdgcn_readlane((int)my.w, (int)k));
 #pragma unroll
-                for (int q = 0; q < VOTE_UNROLL; ++q) {
-                  bool need = lane < left[q];
-                  // bin the fast loop added this vote to (fast records only)
-                  bool moved = false;
-                  uint32_t approx = 0u;
-                  if (need && !EXACT && (v24 >> 31) == 0u) {
+                  for (int q = 0; q < VOTE_UNROLL; ++q)
+                    if (lane < left[q]) exact_vote(j0 + q * WAVE, vy, vz, false, 0u);
+                } else {
+                  // undecided votes (a few lanes of a few hits) join the
+                  // queue with the bin the fast loop added them to
+#pragma unroll
+                  for (int q = 0; q < VOTE_UNROLL; ++q) {
                     const uint32_t a = (v24 - ph[q]) & 0xffffffu;
-                    need = !(a * (30u << 8) - FAST_OFF < rg[q]);
-                    moved = rg[q] != 0u;
-                    approx = (a * 30u) >> 24;
-                  }
-                  if (need) {
-                    const uint32_t j = j0 + q * WAVE;
-                    const uint32_t pr = gld(J.pair + j);
-                    const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
-                    const float *Fm = J.mframe + (size_t)m_r * 16;
-                    float r1[4], r2[4];
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) {
-                      r1[kk] = gld(Fm + 4 + kk);
-                      r2[kk] = gld(Fm + 8 + kk);
-                    }
-                    float uy, uz;
-                    ppfg::frame_apply_yz(r1, r2, gld(J.mx + m_i), gld(J.my + m_i), gld(J.mz + m_i),
-                                         uy, uz);
-                    const uint32_t bin = ppfg::alpha_bin(uy, uz, vy, vz);
-                    const uint32_t mloc = m_r - (uint32_t)tile_base;
-                    if (!moved || bin != approx) {
-                      if (moved) atomicSub(&acc[approx * pitch + mloc], 1u);
-                      atomicAdd(&acc[bin * pitch + mloc], 1u);
+                    const bool need = lane < left[q] && !(a * (30u << 8) - FAST_OFF < rg[q]);
+                    const uint64_t nm = __ballot(need);
+                    if (nm != 0ull) {
+                      const uint32_t cnt = (uint32_t)__popcll(nm);
+                      if (qn + cnt > VQ_CAP) {
+                        flush();
+                        qn = 0;
+                      }
+                      if (need)
+                        wq[qn + (uint32_t)__popcll(nm & lt)] =
+                            make_uint2(j0 + q * WAVE, k | (((a * 30u) >> 24) << 6) | (rg[q] != 0u ? 1u << 11 : 0u));
+                      qn += cnt;
                     }
                   }
                 }
               }
             }
+          }
+          if (qn != 0u) {  // before the next chunk of hits replaces my
+            flush();
+            qn = 0;
           }
         }
       }
@@ -1520,12 +1558,18 @@ uint32_t vote_misc_words(int max_tm) {
   return (w + 1u) & ~1u;
 }
 
+// Dynamic LDS of k_vote: accumulator or grouping table, VoteShared, and the
+// waves' queues of undecided votes.
+size_t vote_lds_bytes(uint32_t misc_words) {
+  return (size_t)misc_words * 4 + sizeof(VoteShared) + (size_t)(VOTE_BLOCK / WAVE) * VQ_CAP * sizeof(uint2);
+}
+
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
               int64_t max_ns, bool exact, float *vote_ms) {
   *vote_ms = 0.0f;
   if (n_items == 0 || n_jobs <= 0) return 0;
   const uint32_t misc_words = vote_misc_words(max_tm);
-  const size_t lds = (size_t)misc_words * 4 + sizeof(VoteShared);
+  const size_t lds = vote_lds_bytes(misc_words);
   if (lds > ctx->max_lds) {
     set_error("run_votes: accumulator tile exceeds the LDS limit");
     return PPF_ERR_ARG;
@@ -1570,7 +1614,7 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
 
 int vote_slots(ppf_ctx ctx, int max_tm, int *slots) {
   const uint32_t misc_words = vote_misc_words(max_tm);
-  const size_t lds = (size_t)misc_words * 4 + sizeof(VoteShared);
+  const size_t lds = vote_lds_bytes(misc_words);
   auto kern = k_vote<false>;
   PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;

@@ -18,6 +18,9 @@ constexpr int MAX_TILE_POINTS = 512;
 constexpr int DEFAULT_TILE_POINTS = 512;
 constexpr int N_BINS = 31;
 constexpr int VOTE_BLOCK = 1024;
+// LDS of a vote block beside its accumulator tile (shared state, the waves'
+// queues of undecided votes), in bytes; kernels.hip checks the bound
+constexpr size_t VOTE_LDS_RESERVE = 12288;
 
 // A context is one HIP stream with its own scratch buffers on one device.
 // The context a caller creates is a root; batched and served work also run
