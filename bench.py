@@ -53,6 +53,17 @@ def hbm_traffic(kernel: str):
     return float(k["bytes_per_dispatch"]), "profiles/dataset_like_hbm_traffic.md"
 
 
+def valu_insts(kernel: str):
+    """Wave64 VALU instructions per launch of `kernel` on this workload, from
+    the committed SQ_INSTS_VALU counter run, or None when none is recorded."""
+    path = os.path.join(ROOT, "profiles", "dataset_like_sq_counters.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["kernels"][kernel]["valu_insts_per_dispatch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(w, budget_s: float):
     """Oracle (hoisted-frame OpenMP port of the reference vote loop) on a
     stride sample of the scene's reference points, all four models."""
@@ -351,6 +362,7 @@ def main():
     alg_bytes = 8.0 * lookups + 4.0 * votes
     achieved = alg_bytes / n_launch / (vote_ms / n_launch * 1e-3) / 1e9
     vote_traffic, traffic_src = hbm_traffic("k_vote<false>" if not args.exact_votes else "k_vote<true>")
+    valu = valu_insts("k_vote<false>") if not args.exact_votes else None
     out = {
         "metric": METRIC,
         "value": lookups / elapsed,
@@ -395,6 +407,11 @@ def main():
             "measured_gbs": (vote_traffic / (vote_ms / n_launch * 1e-3) / 1e9) if vote_traffic else None,
             "measured_frac": (vote_traffic / (vote_ms / n_launch * 1e-3) / 1e9 / HBM_PEAK_GBS)
                              if vote_traffic else None,
+            # what bounds the kernel: its share of the chip's VALU issue slots
+            # (recorded instruction count x 4 cycles per wave64 instruction
+            # over 1,024 SIMDs at 2.4 GHz for the launch time measured here)
+            "valu_issue_share": (valu * 4.0 / (1024 * 2.4e9 * vote_ms / n_launch * 1e-3)) if valu else None,
+            "valu_source": "profiles/dataset_like_sq_counters.md" if valu else None,
         },
     }
     if not args.no_forms:
