@@ -1110,8 +1110,9 @@ __global__ void k_cluster_gather(const uint32_t *__restrict__ members, int64_t P
 
 // Poses that share a voxel cell share the 26 neighbour hashes, so one block
 // takes up to CL_CHUNK poses of one cell (two per thread, in the rotation
-// order above) and streams every neighbour bucket through LDS once.  Each thread walks the bucket in member order, which keeps the
-// summation order of rot_clustering_kernel (kernel.cu:724-756).  Chunks
+// order above) and streams every neighbour bucket through LDS once.  Each
+// thread walks the bucket in member order, which keeps the summation order
+// of rot_clustering_kernel (kernel.cu:724-756).  Chunks
 // whose poses do not all lie in one cell (an FNV collision between cells)
 // walk their own neighbour lists per pose.
 //
