@@ -400,16 +400,17 @@ class Model(_Owned):
         _check(lib().ppf_model_info(self.handle, C.byref(i)))
         return {k: getattr(i, k) for k, _ in ModelInfo._fields_}
 
-    def hash_array(self):
+    def hash_array(self, with_pair_map: bool = True):
         """(hashkeys, counts, firstHashkeyIndex, hashkeyToDataMap) --
-        ParallelHashArray getters (include/impl/parallel_hash_array.hpp:94-112)."""
+        ParallelHashArray getters (include/impl/parallel_hash_array.hpp:94-112).
+        with_pair_map=False skips the n*n pair map (returned as None)."""
         U = int(self.info()["n_unique_keys"])
         hk = np.zeros(U, np.uint32)
         cnt = np.zeros(U, np.uint32)
         first = np.zeros(U, np.uint32)
-        k2d = np.zeros(self.n * self.n, np.uint32)
+        k2d = np.zeros(self.n * self.n, np.uint32) if with_pair_map else None
         _check(lib().ppf_model_hash_array(self.handle, hk.ctypes.data, cnt.ctypes.data,
-                                          first.ctypes.data, k2d.ctypes.data))
+                                          first.ctypes.data, _ptr(k2d)))
         return hk, cnt, first, k2d
 
     def SetModelPointVoteWeights(self, weights):

@@ -72,6 +72,36 @@ def test_model_hash_array_matches_oracle(ctx, medium):
     assert info["n_unique_keys"] == len(ok) and info["n_pairs"] == len(m[0]) ** 2
 
 
+def test_large_model_hash_array_full_size(ctx):
+    """The 30,000-point model (9.0e8 pairs, the large-hash workload) at full
+    size: the hash array's invariants (every pair counted once, keys strictly
+    increasing, first = exclusive scan of counts) and, for sampled pairs, the
+    oracle's key of the pair is one of the device's keys."""
+    w = synth.large_hash()
+    pts, nrm = w.models[0]
+    d = w.d_dists[0]
+    n = len(pts)
+    model = ppf_amd.Model(w.models[0], d, ctx=ctx)
+    hk, cnt, first, _ = model.hash_array(with_pair_map=False)
+    assert model.info()["n_pairs"] == n * n
+    assert np.all(hk[1:] > hk[:-1])
+    assert np.all(cnt >= 1)
+    assert int(cnt.astype(np.int64).sum()) == n * n
+    excl = np.concatenate([[0], np.cumsum(cnt.astype(np.int64))[:-1]])
+    assert np.array_equal(first.astype(np.int64), excl)
+    if hk[0] == 0:
+        assert cnt[0] >= n  # the diagonal has key 0
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        i, j = (int(x) for x in rng.integers(0, n, 2))
+        if i == j:
+            continue
+        _, q = O.compute_ppf(pts[i], nrm[i], pts[j], nrm[j], d)
+        key = O.feature_key(q)
+        pos = int(np.searchsorted(hk, key))
+        assert pos < len(hk) and hk[pos] == key, (i, j, key)
+
+
 def test_scene_keys_match_oracle(ctx, medium):
     _, s, d = medium
     scene = ppf_amd.Scene(s, d, 5, ctx=ctx)
