@@ -159,6 +159,13 @@ int ppf_result_transforms(ppf_result result, float *transforms);
  * transformation_rots (model.cu:202-244).  Any pointer may be NULL. */
 int ppf_result_clusters(ppf_result result, float *vote_counts_out, float *translations,
                         float *quaternions);
+/* CPU clustering outputs: Model::cpu_transformations, the (at most three) best
+ * clusters of clusterPoses, most votes first (transformation_clustering.cpp:
+ * 96-122): poses_out [3][16] averaged poses, votes_out [3] cluster vote
+ * sums, *n_out their number.  poses_out and votes_out may be NULL.
+ * PPF_ERR_STATE when the lookup used GPU clustering. */
+int ppf_result_cpu_clusters(ppf_result result, float *poses_out, uint32_t *votes_out,
+                            int32_t *n_out);
 /* Final pose of the (scene, model) pair as ppf_registration extracts it
  * (src/cuda/ppf.cu:74-93). */
 int ppf_result_pose(ppf_result result, float *pose16);

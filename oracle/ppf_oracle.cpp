@@ -856,6 +856,28 @@ int oracle_cluster_greedy(const float *T, const uint32_t *counts, int64_t P, flo
   return 0;
 }
 
+// Pieces of the two clustering paths, exposed so that tests pin them against
+// float64 closed forms: the pose test's rotation angle
+// (transformation_clustering.cpp:127-137) and hrotmat2quat (kernel.cu:128-144).
+float oracle_rot_angle_between(const float *A9, const float *B9) {
+  float A[3][3], B[3][3];
+  for (int i = 0; i < 9; ++i) {
+    A[i / 3][i % 3] = A9[i];
+    B[i / 3][i % 3] = B9[i];
+  }
+  return rot_angle_between(A, B);
+}
+
+void oracle_hrotmat2quat(const float *T16, float *q4) {
+  float T[4][4];
+  for (int i = 0; i < 16; ++i) T[i / 4][i % 4] = T16[i];
+  const float4 q = hrotmat2quat(T);
+  q4[0] = q.x;
+  q4[1] = q.y;
+  q4[2] = q.z;
+  q4[3] = q.w;
+}
+
 int oracle_register(const float *m_pts, const float *m_nrm, int64_t n_m, const float *s_pts,
                     const float *s_nrm, int64_t n_s, float d_dist, int rpdf,
                     float vote_count_threshold, int cpu_clustering, int use_l1_norm,

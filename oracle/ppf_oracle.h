@@ -89,6 +89,8 @@ int oracle_vote_throughput(const float *m_pts, const float *m_nrm, int64_t n_m,
                            const int64_t *ref_list, int64_t n_ref, int64_t *stats,
                            double *seconds);
 
+float oracle_rot_angle_between(const float *A9, const float *B9);
+void oracle_hrotmat2quat(const float *T16, float *q4);
 int oracle_num_threads(void);
 int oracle_voxel_downsample(const float *pts, const float *nrm, int64_t n, float leaf, float *out_pts,
                             float *out_nrm, int64_t *n_out);

@@ -63,6 +63,9 @@ def lib():
                                              C.c_float, _i64p, C.c_int64, _i64p,
                                              C.POINTER(C.c_double)]
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_rot_angle_between.argtypes = [_f32p, _f32p]
+        L.oracle_rot_angle_between.restype = C.c_float
+        L.oracle_hrotmat2quat.argtypes = [_f32p, _f32p]
         L.oracle_voxel_downsample.argtypes = [_f32p, _f32p, C.c_int64, C.c_float, _f32p, _f32p,
                                               C.POINTER(C.c_int64)]
         L.oracle_numerics.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_int64]
@@ -191,6 +194,19 @@ def cluster_greedy(T, counts, trans_thresh, rot_thresh):
                                 C.byref(nc), C.byref(nr))
     k = nr.value
     return out[:16 * k].reshape(k, 4, 4), votes[:k], int(nc.value)
+
+
+def rot_angle_between(A, B) -> float:
+    """Angle of A^-1 B as the greedy clustering's pose test measures it."""
+    return float(lib().oracle_rot_angle_between(_c(A, np.float32).reshape(-1),
+                                                _c(B, np.float32).reshape(-1)))
+
+
+def hrotmat2quat(T) -> np.ndarray:
+    """Scalar-first quaternion of a 4x4 as the GPU clustering computes it."""
+    q = np.zeros(4, np.float32)
+    lib().oracle_hrotmat2quat(_c(T, np.float32).reshape(-1), q)
+    return q
 
 
 def register(model, scene, d_dist, rpdf, thr, cpu_clustering, use_l1=False, use_avg=False,

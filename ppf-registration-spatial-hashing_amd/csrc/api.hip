@@ -230,8 +230,12 @@ bool within_bounds(const HostPose &a, const HostPose &b, float tt, float rt) {
   return pd < tt && ang < rt;
 }
 
+// Up to three clusters, most votes first: cluster vote sums, and per cluster
+// the translation mean and the normalised mean of the quaternion coefficients
+// (transformation_clustering.cpp:96-122).
 void greedy_cluster(const std::vector<float> &T, const std::vector<uint32_t> &counts, float tt,
-                    float rt, float pose_out[16], int64_t *n_clusters) {
+                    float rt, float poses_out[3][16], uint32_t votes_out[3], int32_t *n_out,
+                    int64_t *n_clusters) {
   const size_t P = counts.size();
   std::vector<HostPose> poses(P);
   for (size_t i = 0; i < P; ++i) {
@@ -267,37 +271,41 @@ void greedy_cluster(const std::vector<float> &T, const std::vector<uint32_t> &co
                      return a.second > b.second;
                    });
   *n_clusters = (int64_t)first.size();
-  // result[0]: average of the winning cluster (translation mean, quaternion
-  // coefficient mean, normalised)
-  const auto &cl = members[votes[0].first];
-  float ta[3] = {0, 0, 0}, qa[4] = {0, 0, 0, 0};
-  for (size_t pi : cl) {
-    for (int r = 0; r < 3; ++r) ta[r] += poses[pi].t[r];
-    float x, y, z, w;
-    quat_of(poses[pi].R, x, y, z, w);
-    qa[0] += x;
-    qa[1] += y;
-    qa[2] += z;
-    qa[3] += w;
+  const int32_t nres = (int32_t)std::min<size_t>(3, first.size());
+  *n_out = nres;
+  for (int32_t ci = 0; ci < nres; ++ci) {
+    const auto &cl = members[votes[(size_t)ci].first];
+    float ta[3] = {0, 0, 0}, qa[4] = {0, 0, 0, 0};
+    for (size_t pi : cl) {
+      for (int r = 0; r < 3; ++r) ta[r] += poses[pi].t[r];
+      float x, y, z, w;
+      quat_of(poses[pi].R, x, y, z, w);
+      qa[0] += x;
+      qa[1] += y;
+      qa[2] += z;
+      qa[3] += w;
+    }
+    const float sz = (float)cl.size();
+    for (int r = 0; r < 3; ++r) ta[r] /= sz;
+    for (int r = 0; r < 4; ++r) qa[r] /= sz;
+    const float qn = sqrtf(qa[0] * qa[0] + qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]);
+    const float x = qa[0] / qn, y = qa[1] / qn, z = qa[2] / qn, w = qa[3] / qn;
+    const float tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    const float twx = tx * w, twy = ty * w, twz = tz * w;
+    const float txx = tx * x, txy = ty * x, txz = tz * x;
+    const float tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    float R[3][3] = {{1 - (tyy + tzz), txy - twz, txz + twy},
+                     {txy + twz, 1 - (txx + tzz), tyz - twx},
+                     {txz - twy, tyz + twx, 1 - (txx + tyy)}};
+    float *po = poses_out[ci];
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) po[4 * r + c] = R[r][c];
+      po[4 * r + 3] = ta[r];
+    }
+    po[12] = po[13] = po[14] = 0.0f;
+    po[15] = 1.0f;
+    votes_out[ci] = votes[(size_t)ci].second;
   }
-  const float sz = (float)cl.size();
-  for (int r = 0; r < 3; ++r) ta[r] /= sz;
-  for (int r = 0; r < 4; ++r) qa[r] /= sz;
-  const float qn = sqrtf(qa[0] * qa[0] + qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]);
-  const float x = qa[0] / qn, y = qa[1] / qn, z = qa[2] / qn, w = qa[3] / qn;
-  const float tx = 2 * x, ty = 2 * y, tz = 2 * z;
-  const float twx = tx * w, twy = ty * w, twz = tz * w;
-  const float txx = tx * x, txy = ty * x, txz = tz * x;
-  const float tyy = ty * y, tyz = tz * y, tzz = tz * z;
-  float R[3][3] = {{1 - (tyy + tzz), txy - twz, txz + twy},
-                   {txy + twz, 1 - (txx + tzz), tyz - twx},
-                   {txz - twy, tyz + twx, 1 - (txx + tyy)}};
-  for (int r = 0; r < 3; ++r) {
-    for (int c = 0; c < 3; ++c) pose_out[4 * r + c] = R[r][c];
-    pose_out[4 * r + 3] = ta[r];
-  }
-  pose_out[12] = pose_out[13] = pose_out[14] = 0.0f;
-  pose_out[15] = 1.0f;
 }
 
 
@@ -407,7 +415,9 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
       HIPC(hipMemcpyAsync(T.data(), r->T.ptr, (size_t)P * 64, hipMemcpyDeviceToHost, st));
       HIPC(hipMemcpyAsync(counts.data(), r->counts.ptr, (size_t)P * 4, hipMemcpyDeviceToHost, st));
       HIPC(hipStreamSynchronize(st));
-      greedy_cluster(T, counts, m->d_dist, ppfm::D_ANGLE0, r->pose, &r->info.n_clusters);
+      greedy_cluster(T, counts, m->d_dist, ppfm::D_ANGLE0, r->cpu_poses, r->cpu_votes, &r->n_cpu,
+                     &r->info.n_clusters);
+      std::memcpy(r->pose, r->cpu_poses[0], 64);
     } else {
       TRY(r->vc.alloc(ctx, (size_t)P * 4));
       TRY(r->trans.alloc(ctx, (size_t)P * 12));
@@ -449,13 +459,14 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
 
 // ---------------------------------------------------------- model files
 // Little-endian file: a 128-byte header, then the model's arrays in a fixed
-// order, with a 64-bit FNV-1a checksum of those bytes in the header.  The
+// order, with a 64-bit FNV-1a checksum in the header over the header itself
+// (checksum field zero) and then the arrays.  The
 // arrays are this build's device layout (version 1), so a loaded model is
 // used as is, without rebuilding its hash array.
 namespace {
 
 constexpr char MODEL_MAGIC[8] = {'P', 'P', 'F', 'M', 'O', 'D', 'E', 'L'};
-constexpr uint32_t MODEL_VERSION = 1;
+constexpr uint32_t MODEL_VERSION = 2;
 
 // accumulator pitch of a tile of TM points: 2^k >= max(TM, 64); files written
 // with 2^k + 1 (same records, more LDS bank conflicts) load as they are
@@ -520,14 +531,17 @@ int ppf_model_save(ppf_model m, const char *path) {
   h.hcap = m->hcap;
   h.hshift = m->hshift;
   std::vector<std::vector<char>> host;
-  uint64_t ck = 14695981039346656037ull, bytes = 0;
+  uint64_t bytes = 0;
   for (auto &a : model_arrays(m)) {
     host.emplace_back(a.second);
     if (a.second) HIPC(hipMemcpy(host.back().data(), a.first->ptr, a.second, hipMemcpyDeviceToHost));
-    ck = fnv1a64(ck, host.back().data(), a.second);
     bytes += a.second;
   }
   h.payload_bytes = bytes;
+  // the checksum covers the header (checksum field zero) and then the arrays
+  h.checksum = 0;
+  uint64_t ck = fnv1a64(14695981039346656037ull, &h, sizeof h);
+  for (auto &b : host) ck = fnv1a64(ck, b.data(), b.size());
   h.checksum = ck;
   FILE *f = std::fopen(path, "wb");
   ARG_CHECK(f, std::string("ppf_model_save: cannot open ") + path);
@@ -552,7 +566,7 @@ int ppf_model_load(ppf_ctx ctx, const char *path, ppf_model *out) {
   ARG_CHECK(h.n_angle == 30, "ppf_model_load: model quantized with another angle count");
   ARG_CHECK(h.n_points >= 2 && h.n_points <= PPF_MAX_MODEL_POINTS && h.N == h.n_points * h.n_points &&
                 h.U >= 1 && h.U <= h.N && h.T >= 1 && h.TM >= 1 && h.TM <= MAX_TILE_POINTS &&
-                (int64_t)h.T * h.TM >= h.n_points && h.hcap >= 2 * h.U && (h.hcap & (h.hcap - 1)) == 0 &&
+                (int64_t)h.T == (h.n_points + h.TM - 1) / h.TM && h.hcap >= 2 * h.U && (h.hcap & (h.hcap - 1)) == 0 &&
                 pitch_ok(h.TM, h.acc_pitch) && h.d_dist > 0.0f && h.hshift >= 1 && h.hshift <= 31 &&
                 (1ull << (32 - h.hshift)) == h.hcap,
             "ppf_model_load: inconsistent model file header");
@@ -569,7 +583,9 @@ int ppf_model_load(ppf_ctx ctx, const char *path, ppf_model *out) {
   m->U = h.U;
   m->hcap = h.hcap;
   m->hshift = h.hshift;
-  uint64_t ck = 14695981039346656037ull, bytes = 0;
+  ModelFileHeader h0 = h;
+  h0.checksum = 0;
+  uint64_t ck = fnv1a64(14695981039346656037ull, &h0, sizeof h0), bytes = 0;
   std::vector<char> buf;
   for (auto &a : model_arrays(m.get())) {
     buf.resize(a.second);
@@ -886,6 +902,20 @@ int ppf_result_clusters(ppf_result r, float *vc, float *trans, float *quats) {
   if (vc) HIPC(hipMemcpy(vc, r->vc.ptr, (size_t)r->P * 4, hipMemcpyDeviceToHost));
   if (trans) HIPC(hipMemcpy(trans, r->trans.ptr, (size_t)r->P * 12, hipMemcpyDeviceToHost));
   if (quats) HIPC(hipMemcpy(quats, r->quats.ptr, (size_t)r->P * 16, hipMemcpyDeviceToHost));
+  return PPF_OK;
+}
+
+int ppf_result_cpu_clusters(ppf_result r, float *poses_out, uint32_t *votes_out, int32_t *n_out) {
+  ARG_CHECK(r && n_out, "ppf_result_cpu_clusters: NULL");
+  if (!r->cpu) {
+    set_error("ppf_result_cpu_clusters: lookup used GPU clustering");
+    return PPF_ERR_STATE;
+  }
+  *n_out = r->n_cpu;
+  for (int32_t i = 0; i < r->n_cpu; ++i) {
+    if (poses_out) std::memcpy(poses_out + 16 * i, r->cpu_poses[i], 64);
+    if (votes_out) votes_out[i] = r->cpu_votes[i];
+  }
   return PPF_OK;
 }
 

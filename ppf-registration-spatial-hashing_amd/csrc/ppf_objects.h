@@ -144,6 +144,10 @@ struct ppf_result_s {
   TBuf codes, counts, T, vc, trans, quats;
   float pose[16] = {0};
   bool cpu = false;
+  // CPU clustering: the (up to) three best clusters' averaged poses and votes
+  int32_t n_cpu = 0;
+  float cpu_poses[3][16] = {};
+  uint32_t cpu_votes[3] = {};
 };
 
 // ------------------------------------------------------------- launchers

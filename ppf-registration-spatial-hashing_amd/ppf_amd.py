@@ -121,6 +121,7 @@ def lib():
     L.ppf_result_transforms.argtypes = [_vp, _f32p]
     L.ppf_result_clusters.argtypes = [_vp, _vp, _vp, _vp]
     L.ppf_result_pose.argtypes = [_vp, _f32p]
+    L.ppf_result_cpu_clusters.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int32)]
     L.ppf_registration.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp, _vp,
                                    C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.c_int32, _vp,
                                    _f32p, _vp]
@@ -337,6 +338,17 @@ class LookupResult(_Owned):
         q = np.zeros(max(P, 1) * 4, np.float32)
         _check(lib().ppf_result_clusters(self.handle, vc.ctypes.data, tr.ctypes.data, q.ctypes.data))
         return vc[:P], tr[:3 * P].reshape(P, 3), q[:4 * P].reshape(P, 4)
+
+    def cpu_transformations(self):
+        """(poses [k][4][4], votes [k]) of the k <= 3 best CPU clusters
+        (Model::cpu_transformations, transformation_clustering.cpp:96-122)."""
+        poses = np.zeros(3 * 16, np.float32)
+        votes = np.zeros(3, np.uint32)
+        n = C.c_int32()
+        _check(lib().ppf_result_cpu_clusters(self.handle, poses.ctypes.data, votes.ctypes.data,
+                                             C.byref(n)))
+        k = int(n.value)
+        return poses[:16 * k].reshape(k, 4, 4), votes[:k]
 
     def pose(self) -> np.ndarray:
         out = np.zeros(16, np.float32)

@@ -1,7 +1,10 @@
 """The C++ facade (include/ppf_amd.hpp) used the way the reference's host code
-uses Scene/Model/ppf_registration (include/scene.h:15-16, include/model.h:17-37,
-include/ppf.h:9-15): compiled and linked against libppf_amd.so on the CPU, run
-on the GPU and compared bit for bit with the Python binding's results."""
+uses Scene/Model/ppf_registration (include/scene.h:15-16, include/model.h:17-113,
+include/ppf.h:9-15): tests/cpp/facade_main.cpp holds src/cuda/ppf.cu:63-93's
+extraction of the pose from Model's public results and alignment.cpp:290-298's
+ppf_registration call, changed only in includes and type names.  Compiled and
+linked against libppf_amd.so on the CPU, run on the GPU and compared bit for
+bit with the Python binding's results."""
 import os
 import subprocess
 
@@ -55,7 +58,8 @@ def test_facade_matches_python_binding(facade_bin, tmp_path, cpu_clustering):
     raw = of.read_bytes()
     t1 = np.frombuffer(raw[:64], np.float32).reshape(4, 4)
     t2 = np.frombuffer(raw[64:128], np.float32).reshape(4, 4)
-    n_kept, max_idx, votes = np.frombuffer(raw[128:152], np.int64)
+    n_kept, max_idx, votes, n_cpu = np.frombuffer(raw[128:160], np.int64)
+    w0 = np.frombuffer(raw[160:164], np.float32)[0]
 
     ctx = ppf_amd.Context(0)
     try:
@@ -64,8 +68,12 @@ def test_facade_matches_python_binding(facade_bin, tmp_path, cpu_clustering):
         res = model.ppf_lookup(scene)
         ref = res.pose()
         assert res.info["n_kept"] == n_kept and res.info["votes"] == votes
+        codes, counts = res.votes()
+        assert w0 == np.float32(counts[0])  # weights default to 1 (model.cu:67)
         if not cpu_clustering:
-            assert res.info["max_idx"] == max_idx
+            assert res.info["max_idx"] == max_idx and n_cpu == 0
+        else:
+            assert 1 <= n_cpu <= 3
     finally:
         ctx.close()
     assert t1.tobytes() == ref.astype(np.float32).tobytes()

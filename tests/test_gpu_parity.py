@@ -217,6 +217,15 @@ def test_cpu_clustering_pose_matches_oracle(ctx, medium):
     opose, st = O.register(m, s, d, 5, 0.3, True)
     assert res.info["n_clusters"] == st["cluster"]
     assert np.allclose(res.pose(), opose, atol=1e-5, rtol=0)
+    # all of Model::cpu_transformations (up to three clusters) against the
+    # oracle's greedy clustering of the same poses and counts
+    _, counts = res.votes()
+    d_angle0 = np.float32(np.float32(2.0 * np.float32(3.141592654)) / np.float32(30.0))
+    oposes, ovotes, onc = O.cluster_greedy(res.transformations(), counts, d, d_angle0)
+    poses, votes = res.cpu_transformations()
+    assert onc == res.info["n_clusters"] and len(poses) == min(3, onc)
+    assert votes.tolist() == ovotes.tolist()
+    assert np.allclose(poses, oposes, atol=1e-5, rtol=0)
 
 
 def test_plain_small_full_against_golden(ctx):

@@ -38,9 +38,15 @@ def test_damaged_files_rejected(ctx, tmp_path):
     flipped = bytearray(data)
     flipped[200] ^= 0x40
     (tmp_path / "flip.ppfm").write_bytes(bytes(flipped))
+    # the low mantissa bit of d_dist (header offset 24): a consistent header
+    # that only the checksum over the header catches
+    hdr = bytearray(data)
+    hdr[24] ^= 0x01
+    (tmp_path / "hdr.ppfm").write_bytes(bytes(hdr))
     (tmp_path / "short.ppfm").write_bytes(bytes(data[:len(data) // 2]))
     (tmp_path / "magic.ppfm").write_bytes(b"NOTMODEL" + bytes(data[8:]))
-    for name, msg in (("flip", "checksum"), ("short", "ends early"), ("magic", "not a model")):
+    for name, msg in (("flip", "checksum"), ("hdr", "checksum"), ("short", "ends early"),
+                      ("magic", "not a model")):
         with pytest.raises(ppf_amd.PPFError) as e:
             ppf_amd.Model.load(str(tmp_path / f"{name}.ppfm"), ctx=ctx)
         assert e.value.code == ppf_amd.PPF_ERR_ARG and msg in str(e.value)

@@ -285,3 +285,178 @@ def test_timing_form_counts_match_vote_histogram():
     for k in ("lookups", "hits", "votes", "entries"):
         assert a[k] == b[k], k
     assert a["votes"] > 0 and a["lookups"] == len(refs) * (len(s[0]) - 1)
+
+
+# ---- discriminating pins of the two clustering paths' geometry
+D_ANGLE0 = float(np.float32(np.float32(2.0 * np.float32(3.141592654)) / np.float32(30.0)))
+
+
+def _rx(t):
+    c, s = math.cos(t), math.sin(t)
+    return np.array([[1, 0, 0], [0, c, -s], [0, s, c]], np.float64)
+
+
+def _rz(t):
+    c, s = math.cos(t), math.sin(t)
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]], np.float64)
+
+
+def _quat64(R):
+    """Unit quaternion (w, x, y, z) of a float64 rotation (closed form)."""
+    w = math.sqrt(max(0.0, 1.0 + np.trace(R))) / 2
+    x = math.copysign(math.sqrt(max(0.0, 1 + R[0, 0] - R[1, 1] - R[2, 2])) / 2, R[2, 1] - R[1, 2])
+    y = math.copysign(math.sqrt(max(0.0, 1 - R[0, 0] + R[1, 1] - R[2, 2])) / 2, R[0, 2] - R[2, 0])
+    z = math.copysign(math.sqrt(max(0.0, 1 - R[0, 0] - R[1, 1] + R[2, 2])) / 2, R[1, 0] - R[0, 1])
+    return np.array([w, x, y, z])
+
+
+def _rot64(q):
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.mark.parametrize("theta", [D_ANGLE0 - 1e-3, D_ANGLE0 + 1e-3, 0.5, 3.0])
+def test_rot_angle_between_closed_form(theta):
+    """The pose test's angle of A^-1 B (transformation_clustering.cpp:131-133)
+    equals theta for B = A Rx(theta) and B = Rz(theta) A, for any A: a product
+    without the inverse, or the angle of one operand, gives another value."""
+    rng = np.random.default_rng(31)
+    for _ in range(20):
+        A = synth.random_rotation(rng)
+        for B in (A @ _rx(theta), _rz(theta) @ A):
+            got = O.rot_angle_between(A.astype(np.float32), B.astype(np.float32))
+            assert abs(got - theta) < 2e-5, (got, theta)
+        # the angle is of A^-1 B, not of A B or B alone
+        assert abs(O.rot_angle_between(A.astype(np.float32), (A @ _rx(theta)).astype(np.float32)) -
+                   O.rot_angle_between(np.eye(3, dtype=np.float32), (A @ A @ _rx(theta)).astype(np.float32))) > 1e-3 \
+            or abs(np.trace(A) - 3) < 1e-2
+
+
+def _pose(R, t):
+    T = np.eye(4, dtype=np.float32)
+    T[:3, :3] = R
+    T[:3, 3] = t
+    return T
+
+
+def test_greedy_compares_with_first_member_only():
+    """transformation_clustering.cpp:74-92: a pose joins the first cluster whose
+    FIRST pose is within bounds.  Pose 3 is within bounds of pose 2 (a member of
+    cluster 1) but not of pose 1, so it opens cluster 2 -- by translation, and
+    separately by rotation."""
+    rng = np.random.default_rng(32)
+    R = synth.random_rotation(rng)
+    T = np.stack([_pose(R, [0, 0, 0]), _pose(R, [0.4, 0, 0]), _pose(R, [0.8, 0, 0])])
+    _, votes, nc = O.cluster_greedy(T, np.array([10, 9, 8], np.uint32), 0.5, 0.2)
+    assert nc == 2 and votes.tolist() == [19, 8]
+    T = np.stack([_pose(R, [0, 0, 0]), _pose(R @ _rx(0.15), [0, 0, 0]), _pose(R @ _rx(0.30), [0, 0, 0])])
+    _, votes, nc = O.cluster_greedy(T, np.array([10, 9, 8], np.uint32), 1.0, 0.2)
+    assert nc == 2 and votes.tolist() == [19, 8]
+    # the same chain in the opposite vote order: pose 2 (middle) comes first
+    # and takes both neighbours
+    _, votes, nc = O.cluster_greedy(T, np.array([8, 10, 9], np.uint32), 1.0, 0.2)
+    assert nc == 1 and votes.tolist() == [27]
+
+
+def _shepperd64(R):
+    """(w, x, y, z) of a rotation matrix by Shepperd's method as Eigen's
+    Quaternion(Matrix3) applies it: the trace branch when the trace is
+    positive, else the branch of the largest diagonal element.  Signs are
+    those of the branch (no canonical hemisphere)."""
+    t = np.trace(R)
+    if t > 0:
+        s = math.sqrt(t + 1.0)
+        w = 0.5 * s
+        s = 0.5 / s
+        return np.array([w, (R[2, 1] - R[1, 2]) * s, (R[0, 2] - R[2, 0]) * s, (R[1, 0] - R[0, 1]) * s])
+    i = 0
+    if R[1, 1] > R[0, 0]:
+        i = 1
+    if R[2, 2] > R[i, i]:
+        i = 2
+    j, k = (i + 1) % 3, (i + 2) % 3
+    c = np.zeros(3)
+    s = math.sqrt(R[i, i] - R[j, j] - R[k, k] + 1.0)
+    c[i] = 0.5 * s
+    s = 0.5 / s
+    w = (R[k, j] - R[j, k]) * s
+    c[j] = (R[j, i] + R[i, j]) * s
+    c[k] = (R[k, i] + R[i, k]) * s
+    return np.array([w, c[0], c[1], c[2]])
+
+
+def test_greedy_average_is_unweighted_quaternion_mean():
+    """transformation_clustering.cpp:105-120: translation mean and the
+    normalised mean of the quaternion coefficients (Eigen's matrix->quaternion,
+    no sign alignment), unweighted by votes.  Near the identity (trace > 0) the
+    rotations Rx(0), Rx(0.1), Rx(0.2) average to Rx(0.1) exactly (half-angle
+    quaternions symmetric about 0.05); a vote-weighted mean (10:1:1) would give
+    ~Rx(0.025).  For random rotations the result equals the float64 mean of
+    Shepperd quaternions, signs as each branch gives them."""
+    rng = np.random.default_rng(33)
+    for k in range(12):
+        R = _rz(rng.uniform(-0.5, 0.5)) if k < 4 else synth.random_rotation(rng)
+        ts = rng.uniform(-1, 1, (3, 3))
+        Rs = [(R @ _rx(a)).astype(np.float32) for a in (0.0, 0.1, 0.2)]
+        T = np.stack([_pose(Ri, t) for Ri, t in zip(Rs, ts)])
+        poses, votes, nc = O.cluster_greedy(T, np.array([10, 1, 1], np.uint32), 5.0, 0.5)
+        assert nc == 1 and votes.tolist() == [12]
+        assert np.allclose(poses[0][:3, 3], ts.astype(np.float32).mean(axis=0), atol=1e-6)
+        want = _rot64(sum(_shepperd64(Ri.astype(np.float64)) for Ri in Rs))
+        assert np.allclose(poses[0][:3, :3], want, atol=3e-6)
+        if k < 4:
+            assert np.allclose(poses[0][:3, :3], R @ _rx(0.1), atol=3e-6)
+            weighted = _rot64(sum(w * _shepperd64(Ri.astype(np.float64)) for w, Ri in zip((10, 1, 1), Rs)))
+            assert not np.allclose(poses[0][:3, :3], weighted, atol=1e-3)
+    # two rotations about different axes
+    for _ in range(10):
+        R1 = synth.random_rotation(rng).astype(np.float32)
+        R2 = (R1.astype(np.float64) @ _rz(0.3) @ _rx(-0.2)).astype(np.float32)
+        T = np.stack([_pose(R1, [0, 0, 0]), _pose(R2, [0, 0, 0])])
+        poses, _, nc = O.cluster_greedy(T, np.array([3, 7], np.uint32), 5.0, 1.0)
+        assert nc == 1
+        want = _rot64(_shepperd64(R1.astype(np.float64)) + _shepperd64(R2.astype(np.float64)))
+        assert np.allclose(poses[0][:3, :3], want, atol=3e-6)
+
+
+def _hrotmat2quat_f32(T):
+    """kernel.cu:128-144 in numpy float32, one IEEE operation at a time."""
+    f = np.float32
+    T = T.astype(np.float32)
+    t = f(f(T[0, 0] + T[1, 1]) + T[2, 2])
+    q = np.empty(4, np.float32)
+    q[0] = f(0.5) * np.sqrt(f(1) + t)
+    q[1] = np.copysign(f(0.5) * np.sqrt(f(f(f(1) + T[0, 0]) - T[1, 1]) - T[2, 2]), f(T[2, 1] - T[1, 2]))
+    q[2] = np.copysign(f(0.5) * np.sqrt(f(f(f(1) - T[0, 0]) + T[1, 1]) - T[2, 2]), f(T[0, 2] - T[2, 0]))
+    q[3] = np.copysign(f(0.5) * np.sqrt(f(f(f(1) - T[0, 0]) - T[1, 1]) + T[2, 2]), f(T[1, 0] - T[0, 1]))
+    dot = f(f(f(q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3])
+    return q, np.sqrt(dot)
+
+
+def test_hrotmat2quat_divides_by_sqrt_of_norm():
+    """hrotmat2quat divides by sqrt(norm(q)) = |q|^(1/2) (kernel.cu:138).  The
+    four square-root arguments sum to 4 for ANY matrix, so |q| = 1 up to
+    rounding and the quirk only shows in the low bits: on general 3x3 blocks
+    (entries uniform in [-1, 1], as far from a rotation as the radicands
+    allow) the oracle equals the float32 restatement of kernel.cu:128-144 bit
+    for bit, near-rotations included, and dividing by |q| instead differs on
+    some of them (~1 in 900)."""
+    rng = np.random.default_rng(34)
+    differs = 0
+    for i in range(12000):
+        T = np.eye(4, dtype=np.float32)
+        if i % 4 == 0:
+            T[:3, :3] = synth.random_rotation(rng) + rng.normal(scale=1e-3, size=(3, 3))
+        else:
+            T[:3, :3] = rng.uniform(-1, 1, (3, 3))
+        with np.errstate(invalid="ignore"):
+            q, nrm = _hrotmat2quat_f32(T)
+        if not np.all(np.isfinite(q)):
+            continue
+        want = q / np.sqrt(nrm)
+        got = O.hrotmat2quat(T)
+        assert got.tobytes() == want.astype(np.float32).tobytes()
+        differs += (q / nrm).astype(np.float32).tobytes() != want.tobytes()
+    assert differs > 0
