@@ -132,6 +132,7 @@ int ppf_lookup(ppf_model model, ppf_scene scene, const ppf_lookup_params *params
                ppf_result *out);
 int ppf_result_destroy(ppf_result result);
 
+
 typedef struct {
   int32_t status;          /* PPF_OK or PPF_ERR_NO_VOTES */
   int64_t lookups;         /* scene pairs looked up: R*(n_s-1) */
@@ -147,8 +148,13 @@ typedef struct {
   double pose_ms;          /* one pose per kept vote */
   double cluster_ms;       /* clustering (GPU or host) and pose extraction */
   int64_t cluster_tests;   /* GPU clustering: (pose, neighbour pose) pairs tested */
+  int64_t vote_rows_ungrouped;  /* plain lookups: reference rows whose hits spanned more model
+                                   buckets than the vote kernel groups, voted hit by hit */
+  int64_t cluster_mixed_chunks; /* GPU clustering: pose chunks spanning several cells that
+                                   share a spatial hash (FNV collision) */
 } ppf_result_info_t;
 int ppf_result_info(ppf_result result, ppf_result_info_t *info);
+
 
 /* Kept votes in reference order (count descending, code ascending):
  * Model::votes / Model::voteCounts after ComputeUniqueVotes (model.cu:148-170). */
@@ -319,6 +325,24 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
  *       known-size reference for hardware traffic counters. */
 int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
                         int32_t bit_lo, int32_t bit_hi);
+
+/* Diagnostic knobs of a context (for tests).  knob 0: the number of distinct
+ * model buckets a reference row may group before the vote kernel votes that
+ * row hit by hit (the path a row takes when its hits span more buckets than
+ * the kernel's grouping table holds); 0 restores the table's size.  Results
+ * do not depend on it. */
+int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value);
+
+/* GPU spatial-hash clustering (Model::ClusterTransformations + max_element,
+ * src/cuda/model.cu:202-244) of host poses: T [P][16] row-major, codes [P]
+ * (the model point m_r = bits 6..31 selects weights[m_r]; weights may be NULL
+ * for all 1), counts [P].  Outputs vote_counts_out [P], transformation_trans
+ * [P][3], transformation_rots [P][4]; stats_out = {max_idx, pair tests,
+ * chunks spanning several cells that share a hash}. */
+int ppf_debug_cluster(ppf_ctx ctx, const float *T, const uint64_t *codes, const uint32_t *counts,
+                      const float *weights, int64_t n_weights, int64_t P, float d_dist, int32_t use_l1,
+                      int32_t use_avg, float *vc_out, float *trans_out, float *quats_out,
+                      int64_t stats_out[3]);
 
 #ifdef __cplusplus
 }

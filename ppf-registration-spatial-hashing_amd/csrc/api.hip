@@ -368,6 +368,8 @@ ppfk::VoteJob vote_job(ppf_model m, ppf_scene s, float thr, unsigned long long *
   J.votes = reinterpret_cast<unsigned long long *>(meta + 2);
   J.item_begin = 0;
   J.per_ref = 0;
+  J.ungrouped = nullptr;
+  J.group_cap = m->ctx->root->dbg_group_cap;
   return J;
 }
 }  // namespace ppfi
@@ -427,7 +429,7 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
                                 r->counts.as<uint32_t>(), m->weights.as<float>(), P, m->d_dist,
                                 p->use_l1_norm != 0, p->use_averaged_clusters != 0, r->vc.as<float>(),
                                 r->trans.as<float>(), r->quats.as<float>(), &mi,
-                                &r->info.cluster_tests));
+                                &r->info.cluster_tests, &r->info.cluster_mixed_chunks));
       r->info.max_idx = mi;
       float t3[3];
       HIPC(hipMemcpyAsync(r->pose, r->T.as<float>() + mi * 16, 64, hipMemcpyDeviceToHost, st));
@@ -791,15 +793,16 @@ int ppf_scene_keys(ppf_scene s, float d_dist, uint32_t *keys_out) {
 
 
 // ----------------------------------------------------------------- lookup
-int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result *out) {
-  ARG_CHECK(m && s && p && out, "ppf_lookup: NULL argument");
-  *out = nullptr;
-  ARG_CHECK(m->ctx == s->ctx, "ppf_lookup: model and scene belong to different contexts");
-  ppf_ctx ctx = m->ctx;
+}  // extern "C"
+
+namespace {
+// Model::ppf_lookup on `ctx` (the model's context, or a lane of it: its
+// stream, events and scratch).  fixed_cap > 0: the candidate buffer has
+// exactly that capacity (no growth); max_blocks > 0 caps the vote grid.
+int lookup_core(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *p, uint32_t fixed_cap,
+                int max_blocks, ppf_result_s *r) {
   hipStream_t st = ctx->stream;
-  TRY(set_device(ctx));
-  std::unique_ptr<ppf_result_s> r(new ppf_result_s());
-  r->ctx = ctx;
+  r->ctx = ctx->root;
   r->n_m = m->cloud.n;
   r->cpu = p->cpu_clustering != 0;
   HIPC(hipEventRecord(ctx->ev2, st));
@@ -810,38 +813,46 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   ppfi::DevBuf cand;
   // candidates per reference point are bounded by the pruning; start at
   // 2048 per reference point (at least 8M), grow on overflow
-  uint32_t cap = p->candidate_capacity
+  uint32_t cap = fixed_cap ? fixed_cap
+                 : p->candidate_capacity
                      ? round_cap(p->candidate_capacity)
                      : round_cap((uint64_t)std::max<int64_t>(CAND_CAP_MIN, s->R * (int64_t)CAND_PER_REF));
   uint32_t meta_h[2] = {0, 0};
   int64_t votes = 0;
   float vote_ms = 0.0f;
   for (int attempt = 0;; ++attempt) {
-    // [meta: 16 B, 256 B reserved][codes: cap x u64][counts: cap x u32].  The
+    // [meta: 32 B, 256 B reserved][codes: cap x u64][counts: cap x u32].  The
     // counters come first so that the 64-bit vote total stays 8-byte aligned
     // for any cap (a misaligned 64-bit atomic faults).
     TRY(cand.ensure(CAND_HEADER + (size_t)cap * 12));
-    uint32_t *meta = cand.as<uint32_t>();  // [0] candidates emitted, [1] running maximum, [2..3] votes
+    // [0] candidates emitted, [1] running maximum, [2..3] votes, [4] rows
+    // voted hit by hit
+    uint32_t *meta = cand.as<uint32_t>();
     unsigned long long *cc = reinterpret_cast<unsigned long long *>(cand.as<char>() + CAND_HEADER);
     uint32_t *cn = reinterpret_cast<uint32_t *>(cc + cap);
-    const uint32_t preset[4] = {0u, meta_h[1], 0u, 0u};
-    HIPC(hipMemcpyAsync(meta, preset, 16, hipMemcpyHostToDevice, st));
+    const uint32_t preset[8] = {0u, meta_h[1], 0u, 0u, 0u, 0u, 0u, 0u};
+    HIPC(hipMemcpyAsync(meta, preset, 32, hipMemcpyHostToDevice, st));
     ppfk::VoteJob J = ppfi::vote_job(m, s, p->vote_count_threshold, cc, cn, meta, cap);
     J.per_ref = p->per_reference_argmax != 0;
+    J.ungrouped = meta + 4;
     float ms = 0.0f;
-    TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)s->R, m->TM, s->cloud.n, p->exact_votes != 0, &ms));
+    TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)s->R, m->TM, s->cloud.n, p->exact_votes != 0, &ms,
+                        max_blocks));
     vote_ms += ms;
-    uint32_t mh[4];
-    HIPC(hipMemcpy(mh, meta, 16, hipMemcpyDeviceToHost));
+    uint32_t mh[8];
+    HIPC(hipMemcpyAsync(mh, meta, 32, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
     meta_h[0] = mh[0];
     meta_h[1] = mh[1];
     votes = (int64_t)(((uint64_t)mh[3] << 32) | mh[2]);
+    r->info.vote_rows_ungrouped = mh[4];
     if (meta_h[0] <= cap) break;
-    if (attempt >= 2 || meta_h[0] > CAND_CAP_MAX) {
-      set_error("ppf_lookup: kept-vote buffer overflow (" + std::to_string(meta_h[0]) + " candidates)");
+    if (attempt >= 2 || meta_h[0] > CAND_CAP_MAX || (fixed_cap && attempt >= 1)) {
+      set_error("ppf_lookup: kept-vote buffer overflow (" + std::to_string(meta_h[0]) + " candidates" +
+                (fixed_cap ? ", more than the fixed capacity of " + std::to_string(fixed_cap) : "") + ")");
       return PPF_ERR_NOMEM;
     }
-    if (attempt >= 1) cap = round_cap(meta_h[0]);
+    if (attempt >= 1 && !fixed_cap) cap = round_cap(meta_h[0]);
   }
   const uint32_t n_cand = meta_h[0];
   r->info.lookups = (int64_t)s->R * (s->cloud.n - 1);
@@ -852,10 +863,23 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   const unsigned long long *ccode =
       reinterpret_cast<const unsigned long long *>(cand.as<char>() + CAND_HEADER);
   const uint32_t *ccount = reinterpret_cast<const uint32_t *>(ccode + cap);
-  TRY(ppfi::finish_lookup(ctx, m, s, p, ccode, ccount, n_cand, meta_h[1], r.get()));
+  TRY(ppfi::finish_lookup(ctx, m, s, p, ccode, ccount, n_cand, meta_h[1], r));
   float tm = 0.0f;
   HIPC(hipEventElapsedTime(&tm, ctx->ev2, ctx->ev3));
   r->info.total_ms = tm;
+  return PPF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result *out) {
+  ARG_CHECK(m && s && p && out, "ppf_lookup: NULL argument");
+  *out = nullptr;
+  ARG_CHECK(m->ctx == s->ctx, "ppf_lookup: model and scene belong to different contexts");
+  TRY(set_device(m->ctx));
+  std::unique_ptr<ppf_result_s> r(new ppf_result_s());
+  TRY(lookup_core(m->ctx, m, s, p, 0, 0, r.get()));
   *out = r.release();
   return PPF_OK;
 }
@@ -1009,6 +1033,55 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
   TRY(ppfk::debug_numerics(ctx, op, da.as<float>(), db.as<float>(), dout.as<float>(), n));
   HIPC(hipMemcpyAsync(out, dout.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
+  return PPF_OK;
+}
+
+int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value) {
+  ARG_CHECK(ctx, "ppf_debug_set: NULL context");
+  ARG_CHECK(knob == 0, "ppf_debug_set: unknown knob");
+  ARG_CHECK(value >= 0 && value <= 0xffffffffll, "ppf_debug_set: value out of range");
+  ctx->root->dbg_group_cap = (uint32_t)value;
+  return PPF_OK;
+}
+
+int ppf_debug_cluster(ppf_ctx ctx, const float *T, const uint64_t *codes, const uint32_t *counts,
+                      const float *weights, int64_t n_weights, int64_t P, float d_dist, int32_t use_l1,
+                      int32_t use_avg, float *vc_out, float *trans_out, float *quats_out,
+                      int64_t stats_out[3]) {
+  ARG_CHECK(ctx && T && codes && counts && vc_out && trans_out && quats_out && stats_out,
+            "ppf_debug_cluster: NULL argument");
+  ARG_CHECK(P >= 1 && P < ((int64_t)1 << 31), "ppf_debug_cluster: need 1 <= P < 2^31 poses");
+  ARG_CHECK(d_dist > 0.0f, "ppf_debug_cluster: d_dist must be > 0");
+  for (int64_t i = 0; i < P; ++i)
+    ARG_CHECK(!weights || (int64_t)(((uint32_t)codes[i]) >> 6) < n_weights,
+              "ppf_debug_cluster: a code's model point has no weight");
+  TRY(set_device(ctx));
+  hipStream_t st = ctx->stream;
+  ppfi::DevBuf dT, dc, dn, dw, dvc, dtr, dq;
+  TRY(dT.ensure((size_t)P * 64));
+  TRY(dc.ensure((size_t)P * 8));
+  TRY(dn.ensure((size_t)P * 4));
+  TRY(dvc.ensure((size_t)P * 4));
+  TRY(dtr.ensure((size_t)P * 12));
+  TRY(dq.ensure((size_t)P * 16));
+  HIPC(hipMemcpy(dT.ptr, T, (size_t)P * 64, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dc.ptr, codes, (size_t)P * 8, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dn.ptr, counts, (size_t)P * 4, hipMemcpyHostToDevice));
+  if (weights) {
+    TRY(dw.ensure((size_t)n_weights * 4));
+    HIPC(hipMemcpy(dw.ptr, weights, (size_t)n_weights * 4, hipMemcpyHostToDevice));
+  }
+  int64_t mi = -1, tests = 0, mixed = 0;
+  TRY(ppfk::cluster_spatial(ctx, dT.as<float>(), dc.as<unsigned long long>(), dn.as<uint32_t>(),
+                            weights ? dw.as<float>() : nullptr, P, d_dist, use_l1 != 0, use_avg != 0,
+                            dvc.as<float>(), dtr.as<float>(), dq.as<float>(), &mi, &tests, &mixed));
+  HIPC(hipMemcpyAsync(vc_out, dvc.ptr, (size_t)P * 4, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(trans_out, dtr.ptr, (size_t)P * 12, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(quats_out, dq.ptr, (size_t)P * 16, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  stats_out[0] = mi;
+  stats_out[1] = tests;
+  stats_out[2] = mixed;
   return PPF_OK;
 }
 

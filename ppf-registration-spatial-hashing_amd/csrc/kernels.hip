@@ -380,6 +380,7 @@ struct VoteShared {
   unsigned long long item;
   int job;
   uint32_t nh, next, ovf, ngroups;
+  uint32_t ndist, pad0;
   uint32_t wmax[VOTE_BLOCK / WAVE];
   uint32_t wsum[VOTE_BLOCK / WAVE][2];
   float thr;
@@ -544,8 +545,12 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       tcnt[i] = 0u;
     }
     if (tid < 64) sh.cls[tid] = 0u;
-    if (tid == 0) sh.ovf = 0u;
+    if (tid == 0) {
+      sh.ovf = 0u;
+      sh.ndist = 0u;
+    }
     __syncthreads();
+    const uint32_t group_cap = J.group_cap;
     for (uint32_t h = tid; h < nh; h += VOTE_BLOCK) {
       const uint32_t u = gld(reinterpret_cast<const uint32_t *>(hits + h)) & 0x7fffffffu;
       const uint32_t key = u + 1u;
@@ -556,7 +561,12 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
           break;
         }
         const uint32_t prev = atomicCAS(&tkey[slot], 0u, key);
-        if (prev == 0u || prev == key) break;
+        if (prev == 0u) {
+          // a new group; beyond group_cap groups the row is voted hit by hit
+          if (group_cap != 0u && atomicAdd(&sh.ndist, 1u) >= group_cap) sh.ovf = 1u;
+          break;
+        }
+        if (prev == key) break;
         slot = (slot + 1u) & (GT_SIZE - 1);
       }
       if (slot == 0xffffffffu)
@@ -567,6 +577,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
     }
     __syncthreads();
     const bool identity = sh.ovf != 0u;  // table full: one group per hit
+    if (identity && tid == 0 && J.ungrouped) atomicAdd(J.ungrouped, 1u);
     if (!identity) {
       // exclusive scans of the slot counts (group starts) and of the used
       // slots (group ids); each thread owns GT_SIZE / VOTE_BLOCK slots
@@ -1208,7 +1219,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
     const uint32_t *__restrict__ ucount, int64_t U, const uint32_t *__restrict__ chunk_start,
     const uint32_t *__restrict__ centers, const uint64_t *__restrict__ order, float s_max, float d_dist,
     float *__restrict__ vc_out, float *__restrict__ trans_out,
-    unsigned long long *__restrict__ n_tests) {
+    unsigned long long *__restrict__ n_tests, unsigned long long *__restrict__ n_mixed) {
   __shared__ float4 s_q[CL_BLOCK];
   __shared__ float4 s_t[CL_BLOCK];
   __shared__ int s_u, s_c, s_uniform;
@@ -1300,6 +1311,9 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
       }
     }
   } else {
+    // the chunk's poses lie in several cells that share a hash (an FNV
+    // collision): each pose walks its own neighbour list
+    if (tid == 0) atomicAdd(n_mixed, 1ull);
 #pragma unroll
     for (int r = 0; r < CL_PER; ++r) {
       if (!have[r]) continue;
@@ -1565,8 +1579,15 @@ size_t vote_lds_bytes(uint32_t misc_words) {
   return (size_t)misc_words * 4 + sizeof(VoteShared) + (size_t)(VOTE_BLOCK / WAVE) * VQ_CAP * sizeof(uint2);
 }
 
+// hits + sorted hits (16 B each), slot ids (4 B), groups (8 B x GT_SIZE), in uint4
+uint32_t vote_hit_stride_u4(uint32_t hit_cap) {
+  return 2 * hit_cap + (uint32_t)(((size_t)(hit_cap + (hit_cap & 1u)) * 4 + (size_t)GT_SIZE * 8 + 15) / 16);
+}
+
+size_t vote_hit_bytes_per_block(int64_t max_ns) { return (size_t)vote_hit_stride_u4((uint32_t)max_ns) * 16; }
+
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
-              int64_t max_ns, bool exact, float *vote_ms) {
+              int64_t max_ns, bool exact, float *vote_ms, int max_blocks) {
   *vote_ms = 0.0f;
   if (n_items == 0 || n_jobs <= 0) return 0;
   const uint32_t misc_words = vote_misc_words(max_tm);
@@ -1579,13 +1600,12 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
   PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;
   PPF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, VOTE_BLOCK, lds));
-  const int G = ctx->n_cu * std::max(1, per_cu);
+  int G = ctx->n_cu * std::max(1, per_cu);
+  if (max_blocks > 0) G = std::min(G, max_blocks);
   PPF_TRY(ctx->tmp_a.ensure(sizeof(VoteJob) * n_jobs));
   PPF_TRY(ctx->queue.ensure(16));
   const uint32_t hit_cap = (uint32_t)max_ns;
-  // hits + sorted hits (16 B each), slot ids (4 B), groups (8 B x GT_SIZE)
-  const uint32_t stride_u4 =
-      2 * hit_cap + (uint32_t)(((size_t)(hit_cap + (hit_cap & 1u)) * 4 + (size_t)GT_SIZE * 8 + 15) / 16);
+  const uint32_t stride_u4 = vote_hit_stride_u4(hit_cap);
   PPF_TRY(ctx->hits.ensure((size_t)G * stride_u4 * 16));
   PPF_HIP(hipMemcpyAsync(ctx->tmp_a.ptr, h_jobs, sizeof(VoteJob) * n_jobs, hipMemcpyHostToDevice,
                          ctx->stream));
@@ -1685,9 +1705,10 @@ int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const
 int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
-                    int64_t *max_idx, int64_t *pair_tests) {
+                    int64_t *max_idx, int64_t *pair_tests, int64_t *mixed_chunks) {
   *max_idx = -1;
   *pair_tests = 0;
+  *mixed_chunks = 0;
   if (P <= 0) return 0;
   // cq, ct (2 x float4 P) words, okeys (2 x u64 P) trans (3P) w (P) adj (27P)
   // ukeys/ufirst/ucount (3P) members, centers (2P) argmax (8) test counter (8)
@@ -1720,7 +1741,8 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   p += (size_t)P * 4;
   long long *am = reinterpret_cast<long long *>(((uintptr_t)p + 7) & ~(uintptr_t)7);
   unsigned long long *ntest = reinterpret_cast<unsigned long long *>(am + 1);
-  PPF_HIP(hipMemsetAsync(ntest, 0, 8, ctx->stream));
+  unsigned long long *nmixed = ntest + 1;
+  PPF_HIP(hipMemsetAsync(ntest, 0, 16, ctx->stream));
   hipLaunchKernelGGL(k_cluster_prep, dim3(nblk(P, 128)), dim3(128), 0, ctx->stream, T, codes, counts,
                      weights, P, d_dist, trans, quats, w, words, adj);
   PPF_HIP(hipGetLastError());
@@ -1758,15 +1780,16 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
                         : (use_l1 ? k_cluster_tiled<1, 0> : k_cluster_tiled<0, 0>);
     hipLaunchKernelGGL(kern, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, cq, ct, adj,
                        ukeys, ufirst, ucount, U, cstart, centers, order, sqrt_below_max(d_dist), d_dist,
-                       vc, trans_out, ntest);
+                       vc, trans_out, ntest, nmixed);
   }
   hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
   PPF_HIP(hipGetLastError());
-  long long mi_t[2] = {-1, 0};
-  PPF_HIP(hipMemcpyAsync(mi_t, am, 16, hipMemcpyDeviceToHost, ctx->stream));
+  long long mi_t[3] = {-1, 0, 0};
+  PPF_HIP(hipMemcpyAsync(mi_t, am, 24, hipMemcpyDeviceToHost, ctx->stream));
   PPF_HIP(hipStreamSynchronize(ctx->stream));
   *max_idx = mi_t[0];
   *pair_tests = mi_t[1];
+  *mixed_chunks = mi_t[2];
   return 0;
 }
 

@@ -41,6 +41,10 @@ struct ppf_ctx_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
   std::vector<ppf_ctx_s *> lanes;  // root only
+  // root only, diagnostics (ppf_debug_set): distinct model buckets a
+  // reference row may group before it is voted hit by hit (0 = the LDS
+  // grouping table's size)
+  uint32_t dbg_group_cap = 0;
 
   // Account n bytes against the root's budget (false: over budget).
   bool reserve(uint64_t n) {
@@ -174,6 +178,8 @@ struct VoteJob {
   int32_t per_ref;  // 1: emit only each reference row's first maximum
   unsigned long long *votes;  // total votes of the job (statistics)
   uint64_t item_begin;
+  uint32_t *ungrouped;  // rows voted hit by hit (grouping table overflow); may be NULL
+  uint32_t group_cap;   // distinct buckets a row may group before it is voted hit by hit; 0 = table size
 };
 
 int prep_cloud(ppf_ctx ctx, const float *d_aos_pts, const float *d_aos_nrm, int64_t n,
@@ -201,8 +207,12 @@ int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const Cl
                uint32_t *toff);
 int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
                   const CloudDev &m, int32_t TM, uint32_t *rec);
+// max_blocks > 0 caps the persistent grid (each block holds max_ns hits of
+// scratch); results do not depend on the grid size.
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
-              int64_t max_ns, bool exact, float *vote_ms);
+              int64_t max_ns, bool exact, float *vote_ms, int max_blocks = 0);
+// Device scratch of one vote block for scenes of up to max_ns points.
+size_t vote_hit_bytes_per_block(int64_t max_ns);
 // Blocks of a vote launch that run at once (reference rows voted in parallel).
 int vote_slots(ppf_ctx ctx, int max_tm, int *slots);
 int select_and_sort(ppf_ctx ctx, const unsigned long long *cand_code, const uint32_t *cand_count,
@@ -213,7 +223,7 @@ int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const
 int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
-                    int64_t *max_idx, int64_t *pair_tests);
+                    int64_t *max_idx, int64_t *pair_tests, int64_t *mixed_chunks);
 int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *out, int64_t n);
 int debug_copy(ppf_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n);  // n multiple of 4
 

@@ -59,7 +59,8 @@ class ResultInfo(C.Structure):
                 ("n_candidates", C.c_int64), ("n_kept", C.c_int64), ("max_idx", C.c_int64),
                 ("n_clusters", C.c_int64), ("vote_ms", C.c_double), ("total_ms", C.c_double),
                 ("select_ms", C.c_double), ("pose_ms", C.c_double), ("cluster_ms", C.c_double),
-                ("cluster_tests", C.c_int64)]
+                ("cluster_tests", C.c_int64), ("vote_rows_ungrouped", C.c_int64),
+                ("cluster_mixed_chunks", C.c_int64)]
 
 
 class BatchInfo(C.Structure):
@@ -146,6 +147,9 @@ def lib():
     L.ppf_server_stats.argtypes = [_vp, C.POINTER(ServerStats)]
     L.ppf_debug_numerics.argtypes = [_vp, C.c_int32, _f32p, _vp, _f32p, C.c_int64]
     L.ppf_debug_primitive.argtypes = [_vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32]
+    L.ppf_debug_set.argtypes = [_vp, C.c_int32, C.c_int64]
+    L.ppf_debug_cluster.argtypes = [_vp, _f32p, _vp, _vp, _vp, C.c_int64, C.c_int64, C.c_float,
+                                    C.c_int32, C.c_int32, _f32p, _f32p, _f32p, _vp]
     sizes = (C.c_uint32 * 7)()
     L.ppf_abi_sizes(sizes)
     want = tuple(C.sizeof(t) for t in (ModelInfo, LookupParams, ResultInfo, BatchInfo, ServerConfig,
@@ -253,6 +257,31 @@ class Context:
         out = np.zeros_like(x)
         _check(lib().ppf_debug_primitive(self.handle, 2, x.ctypes.data, out.ctypes.data, len(x), 0, 0))
         return out
+
+    def debug_set(self, knob: int, value: int):
+        """ppf_debug_set: knob 0 = distinct buckets a reference row may group
+        before it is voted hit by hit (0 = the grouping table's size)."""
+        _check(lib().ppf_debug_set(self.handle, int(knob), int(value)))
+
+    def debug_cluster(self, T, codes, counts, d_dist: float, use_l1: bool = False,
+                      use_avg: bool = False, weights=None):
+        """GPU spatial-hash clustering of host poses: dict of vote_counts,
+        trans, quats, max_idx, tests, mixed_chunks."""
+        T = np.ascontiguousarray(T, np.float32).reshape(-1)
+        codes = np.ascontiguousarray(codes, np.uint64)
+        counts = np.ascontiguousarray(counts, np.uint32)
+        P = len(codes)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        vc = np.zeros(P, np.float32)
+        tr = np.zeros(3 * P, np.float32)
+        q = np.zeros(4 * P, np.float32)
+        st = np.zeros(3, np.int64)
+        _check(lib().ppf_debug_cluster(self.handle, T, codes.ctypes.data, counts.ctypes.data,
+                                       None if w is None else w.ctypes.data,
+                                       0 if w is None else len(w), P, float(d_dist), int(use_l1),
+                                       int(use_avg), vc, tr, q, st.ctypes.data))
+        return dict(vote_counts=vc, trans=tr.reshape(P, 3), quats=q.reshape(P, 4),
+                    max_idx=int(st[0]), tests=int(st[1]), mixed_chunks=int(st[2]))
 
     def debug_sort(self, keys: np.ndarray, vals: Optional[np.ndarray], bit_lo: int, bit_hi: int):
         k = np.ascontiguousarray(keys, dtype=np.uint64).copy()

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <chrono>
 #include <thread>
 #include <vector>
 
@@ -276,14 +277,18 @@ static void test_cancel_timeout_shutdown() {
   std::unique_ptr<Request> r;
   CHECK(s->wait(a, 0.001, &r) == PPF_ERR_TIMEOUT);
   CHECK(s->cancel(b) == PPF_OK);
+  // a is admitted by the scheduler thread; on a loaded host that can take a
+  // while, and only an admitted request refuses cancellation
+  for (int i = 0; i < 30000 && s->stats().active < 1; ++i)
+    std::this_thread::sleep_for(std::chrono::milliseconds(1));
   CHECK(s->cancel(a) == PPF_ERR_STATE);
-  CHECK(s->wait(b, 1.0, &r) == PPF_OK && r->state == State::Cancelled && r->status == PPF_ERR_CANCELLED);
+  CHECK(s->wait(b, 30.0, &r) == PPF_OK && r->state == State::Cancelled && r->status == PPF_ERR_CANCELLED);
   uint64_t c2 = s->submit(make_req(5000, 5, {1}));  // queued behind a
   s->shutdown();                                     // a completes, c2 is cancelled
-  CHECK(s->wait(a, 1.0, &r) == PPF_OK && r->state == State::Done);
-  CHECK(s->wait(c2, 1.0, &r) == PPF_OK && r->state == State::Cancelled);
+  CHECK(s->wait(a, 30.0, &r) == PPF_OK && r->state == State::Done);
+  CHECK(s->wait(c2, 30.0, &r) == PPF_OK && r->state == State::Cancelled);
   uint64_t late = s->submit(make_req(100, 1, {0}));
-  CHECK(s->wait(late, 1.0, &r) == PPF_OK && r->state == State::Cancelled);
+  CHECK(s->wait(late, 30.0, &r) == PPF_OK && r->state == State::Cancelled);
   CHECK(s->wait(12345, 0.0, &r) == PPF_ERR_ARG);
   {
     std::lock_guard<std::mutex> g(be.mu);

@@ -460,3 +460,37 @@ def test_hrotmat2quat_divides_by_sqrt_of_norm():
         assert got.tobytes() == want.astype(np.float32).tobytes()
         differs += (q / nrm).astype(np.float32).tobytes() != want.tobytes()
     assert differs > 0
+
+
+def test_spatial_clustering_shared_hash_cells_bruteforce():
+    """Two distinct cells with one FNV hash (kernel.cu:663-699): the reference
+    finds neighbours by hash, so a pose also counts members of the other cell
+    whenever that cell's hash is one of its 26 neighbour hashes.  The oracle
+    equals an O(P^2) numpy scan with that rule."""
+    import collide
+    T, codes, counts, d, (a, b) = collide.colliding_poses(seed=3)
+    out = O.cluster_spatial(T, codes, counts, d)
+    P = len(T)
+    tr = T[:, :3, 3].astype(np.float32)
+    cells = np.trunc(tr - np.fmod(tr, np.float32(d))).astype(np.int64)
+    hashes = collide.fnv_cells(cells)
+    assert np.array_equal(out["hashes"], hashes)
+    assert len(set(hashes.tolist())) < len({tuple(c) for c in cells.tolist()})  # a shared hash
+    q = out["quats"].astype(np.float64)
+    rot_sq = np.float32(2 * np.float32(np.float32(2 * np.float32(3.141592654)) / np.float32(30))) ** 2
+    offs = np.array([(x, y, z) for x in (-1, 0, 1) for y in (-1, 0, 1) for z in (-1, 0, 1)
+                     if (x, y, z) != (0, 0, 0)])
+    order = np.lexsort((np.arange(P), hashes))  # members in (hash, index) order
+    for i in range(P):
+        nh = collide.fnv_cells(cells[i] + offs)
+        vc = 1.0
+        for h in nh:
+            for j in order[hashes[order] == h]:
+                qd = abs(8 * (1 - float(np.dot(q[i], q[j]))))
+                if qd < rot_sq and np.linalg.norm(tr[i].astype(np.float64) - tr[j]) < d:
+                    vc += float(counts[j])
+        assert abs(out["vote_counts"][i] - vc) < 1e-3, i
+    # poses of the two colliding cells see each other only through the hash
+    ia = np.nonzero((cells == a).all(1))[0]
+    ib = np.nonzero((cells == b).all(1))[0]
+    assert len(ia) and len(ib)
