@@ -155,6 +155,22 @@ typedef struct {
 } ppf_result_info_t;
 int ppf_result_info(ppf_result result, ppf_result_info_t *info);
 
+/* Allocation-free Model::ppf_lookup.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same stages and the
+ * same results as ppf_lookup + ppf_result_pose/ppf_result_info, but every
+ * buffer it needs beyond the model, the scene and the caller's outputs is a
+ * piece of one static device array of PPF_NOALLOC_SCRATCH_BYTES per device
+ * (held until the process ends); it calls no hipMalloc/hipFree.  Limits: the
+ * scene has at most PPF_NOALLOC_MAX_SCENE_POINTS points (PPF_ERR_ARG above),
+ * and a lookup whose buffers do not fit the array (many kept votes) returns
+ * PPF_ERR_NOMEM with a message.  All calls on one GPU share its array: calls
+ * that overlap on one GPU (from several threads or contexts) race.  Returns
+ * PPF_OK, or PPF_ERR_NO_VOTES with a zero pose, like ppf_result_pose. */
+#define PPF_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define PPF_NOALLOC_MAX_SCENE_POINTS 8192
+int ppf_lookup_noalloc(ppf_model model, ppf_scene scene, const ppf_lookup_params *params,
+                       float *pose_out, ppf_result_info_t *info_out);
 
 /* Kept votes in reference order (count descending, code ascending):
  * Model::votes / Model::voteCounts after ComputeUniqueVotes (model.cu:148-170). */

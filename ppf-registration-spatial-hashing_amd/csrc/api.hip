@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <utility>
 
 #include "ppf_geom.h"
@@ -18,8 +19,37 @@ namespace ppfi {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
+Arena *&current_arena() {
+  static thread_local Arena *a = nullptr;
+  return a;
+}
+
+void *arena_take(size_t n) {
+  Arena *a = current_arena();
+  const size_t off = (a->used + 255) & ~(size_t)255;
+  if (off > a->cap || n > a->cap - off) {
+    set_error("device scratch exhausted: this call needs more than the " + std::to_string(a->cap >> 20) +
+              " MiB of the allocation-free scratch (" + std::to_string(off + n) + " bytes)");
+    return nullptr;
+  }
+  a->used = off + n;
+  return a->base + off;
+}
+
 int DevBuf::ensure(size_t n) {
   if (n <= bytes && ptr) return 0;
+  if (current_arena()) {
+    if (ptr && !arena) {
+      set_error("internal: an allocated buffer used inside the scratch arena");
+      return PPF_ERR_STATE;
+    }
+    void *p = arena_take(std::max<size_t>(n, 256));
+    if (!p) return PPF_ERR_NOMEM;
+    ptr = p;
+    bytes = std::max<size_t>(n, 256);
+    arena = true;
+    return 0;
+  }
   release();
   size_t want = std::max<size_t>(n, 256);
   hipError_t e = hipMalloc(&ptr, want);
@@ -32,9 +62,10 @@ int DevBuf::ensure(size_t n) {
   return 0;
 }
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr && !arena) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+  arena = false;
 }
 }  // namespace ppfi
 
@@ -42,6 +73,13 @@ int TBuf::alloc(ppf_ctx c, size_t n) {
   release();
   ctx = c->root;
   size_t want = std::max<size_t>(n, 256);
+  if (current_arena()) {
+    ptr = arena_take(want);
+    if (!ptr) return PPF_ERR_NOMEM;
+    bytes = want;
+    arena = true;
+    return 0;
+  }
   if (!ctx->reserve(want)) {
     set_error("device memory budget exceeded (" + std::to_string(ctx->bytes.load() + want) + " > " +
               std::to_string(ctx->budget) + " bytes)");
@@ -58,12 +96,13 @@ int TBuf::alloc(ppf_ctx c, size_t n) {
   return 0;
 }
 void TBuf::release() {
-  if (ptr) {
+  if (ptr && !arena) {
     (void)hipFree(ptr);
     if (ctx) ctx->unreserve(bytes);
   }
   ptr = nullptr;
   bytes = 0;
+  arena = false;
 }
 
 namespace ppfi {
@@ -93,6 +132,11 @@ void ctx_release(ppf_ctx_s *c) {
     delete l;
   }
   c->lanes.clear();
+  if (c->noalloc) {
+    ctx_release(c->noalloc);
+    delete c->noalloc;
+    c->noalloc = nullptr;
+  }
   c->sort.counts.release();
   c->sort.scan.release();
   c->sort.tmp_keys.release();
@@ -605,6 +649,68 @@ int ppf_model_load(ppf_ctx ctx, const char *path, ppf_model *out) {
 
 }  // extern "C"
 
+// ------------------------------------------------- allocation-free lookup
+// Device scratch of ppf_lookup_noalloc: one static array per device (each GPU
+// holds its own copy until the process ends).  In host code `scratch` names a
+// host-side stand-in, never the device memory: host code uses only the
+// address hipGetSymbolAddress returns for the current device, kept per device
+// in g_scratch_dev.
+static __device__ __align__(256) unsigned char scratch[PPF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[MAX_DEVICES] = {};
+// Share of the scratch the vote blocks' hit lists may take; the grid is
+// capped so that they fit.
+constexpr size_t NOALLOC_HIT_BYTES = PPF_NOALLOC_SCRATCH_BYTES * 3 / 8;
+constexpr uint32_t NOALLOC_CAND_CAP = 1u << 20;
+
+// Set-up at context creation, with the context's device current: the address
+// of this device's scratch array (once per device) and the context's
+// allocation-free lane (its own stream and events; its scratch buffers are
+// only ever pieces of the array).
+int noalloc_setup(ppf_ctx_s *c) {
+  ARG_CHECK(c->device >= 0 && c->device < MAX_DEVICES, "ppf_ctx_create: device index too large");
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+      void *ptr = nullptr;
+      HIPC(hipSetDevice(c->device));
+      HIPC(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      ARG_CHECK(ptr != nullptr, "ppf_ctx_create: no address for the device scratch array");
+      g_scratch_dev[c->device] = ptr;
+    }
+  }
+  auto *l = new ppf_ctx_s();
+  l->root = c;
+  c->noalloc = l;
+  return ctx_init(l, c->device);
+}
+
+// Makes the device's scratch array the calling thread's arena for one call
+// and forgets the lane's pieces of it before and after.
+struct ArenaScope {
+  ppfi::Arena arena;
+  ppf_ctx_s *lane;
+  static void forget(ppf_ctx_s *l) {
+    for (DevBuf *b : {&l->tmp_a, &l->tmp_b, &l->tmp_c, &l->tmp_d, &l->hits, &l->queue, &l->sel,
+                      &l->sort.counts, &l->sort.scan, &l->sort.tmp_keys, &l->sort.tmp_vals})
+      b->release();
+  }
+  ArenaScope(void *base, ppf_ctx_s *l) : lane(l) {
+    arena.base = static_cast<char *>(base);
+    arena.cap = PPF_NOALLOC_SCRATCH_BYTES;
+    forget(lane);
+    ppfi::current_arena() = &arena;
+  }
+  ~ArenaScope() {
+    ppfi::current_arena() = nullptr;
+    forget(lane);
+  }
+};
+}  // namespace
+
 extern "C" {
 
 const char *ppf_last_error(void) { return g_err.c_str(); }
@@ -628,6 +734,7 @@ int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
   auto *c = new ppf_ctx_s();
   c->budget = hbm_budget_bytes;
   int rc = ctx_init(c, std::min(n - 1, std::max(0, device)));
+  if (rc == 0) rc = noalloc_setup(c);
   if (rc) {
     ctx_release(c);
     delete c;
@@ -882,6 +989,41 @@ int ppf_lookup(ppf_model m, ppf_scene s, const ppf_lookup_params *p, ppf_result 
   TRY(lookup_core(m->ctx, m, s, p, 0, 0, r.get()));
   *out = r.release();
   return PPF_OK;
+}
+
+int ppf_lookup_noalloc(ppf_model m, ppf_scene s, const ppf_lookup_params *p, float *pose_out,
+                       ppf_result_info_t *info_out) {
+  ARG_CHECK(m && s && p && pose_out && info_out, "ppf_lookup_noalloc: NULL argument");
+  ARG_CHECK(m->ctx == s->ctx, "ppf_lookup_noalloc: model and scene belong to different contexts");
+  ARG_CHECK(s->cloud.n <= PPF_NOALLOC_MAX_SCENE_POINTS,
+            "ppf_lookup_noalloc: scene of " + std::to_string(s->cloud.n) + " points exceeds the limit of " +
+                std::to_string(PPF_NOALLOC_MAX_SCENE_POINTS) + " (use ppf_lookup)");
+  ppf_ctx root = m->ctx->root;
+  ARG_CHECK(root->noalloc, "ppf_lookup_noalloc: context has no allocation-free lane");
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    base = g_scratch_dev[root->device];
+  }
+  ARG_CHECK(base, "ppf_lookup_noalloc: device scratch not set up");
+  TRY(set_device(root));
+  // the model's and scene's uploads (on the root stream) are complete: both
+  // constructors synchronize before returning
+  const size_t per_block = ppfk::vote_hit_bytes_per_block(s->cloud.n);
+  const int max_blocks = (int)std::max<size_t>(1, NOALLOC_HIT_BYTES / per_block);
+  int rc;
+  {
+    ArenaScope scope(base, root->noalloc);
+    ppf_result_s r;
+    rc = lookup_core(root->noalloc, m, s, p, NOALLOC_CAND_CAP, max_blocks, &r);
+    if (rc == PPF_OK) {
+      std::memcpy(pose_out, r.pose, 64);
+      *info_out = r.info;
+    }
+    HIPC(hipStreamSynchronize(root->noalloc->stream));
+  }
+  if (rc) return rc;
+  return info_out->status == PPF_OK ? PPF_OK : info_out->status;
 }
 
 int ppf_result_destroy(ppf_result r) {

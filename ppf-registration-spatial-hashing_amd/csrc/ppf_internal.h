@@ -28,11 +28,26 @@ void set_error(const std::string &msg);
     if (r_ != 0) return r_; \
   } while (0)
 
+// -------------------------------------------------------- scratch arena
+// While a thread has an arena set (ArenaScope), DevBuf::ensure and
+// TBuf::alloc carve 256-byte aligned pieces of the arena's device memory in
+// order instead of allocating; the pieces are never freed, the arena is reset
+// as a whole.  Used by the allocation-free lookup entry point.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+};
+Arena *&current_arena();  // thread-local; nullptr when no arena is active
+void *arena_take(size_t n);  // nullptr (and an error message) when the arena is full
+
 // -------------------------------------------------------- device buffers
-// A growable device buffer owned by whoever holds it; never shrinks.
+// A growable device buffer owned by whoever holds it; never shrinks.  Inside
+// an arena it is a piece of the arena (arena = true) and release() only
+// forgets it.
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool arena = false;
   int ensure(size_t n);  // grow to at least n bytes (contents not kept)
   void release();
   template <class T>

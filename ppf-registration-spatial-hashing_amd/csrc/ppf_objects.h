@@ -41,6 +41,9 @@ struct ppf_ctx_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
   std::vector<ppf_ctx_s *> lanes;  // root only
+  // root only: the lane of ppf_lookup_noalloc, whose scratch buffers are only
+  // ever pieces of the device's static scratch array
+  ppf_ctx_s *noalloc = nullptr;
   // root only, diagnostics (ppf_debug_set): distinct model buckets a
   // reference row may group before it is voted hit by hit (0 = the LDS
   // grouping table's size)
@@ -72,6 +75,7 @@ struct TBuf {
   ppf_ctx ctx = nullptr;
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool arena = false;  // a piece of the current scratch arena (not budgeted, never freed)
   int alloc(ppf_ctx c, size_t n);
   void release();
   template <class T>
@@ -82,9 +86,10 @@ struct TBuf {
   TBuf() = default;
   TBuf(const TBuf &) = delete;
   TBuf &operator=(const TBuf &) = delete;
-  TBuf(TBuf &&o) noexcept : ctx(o.ctx), ptr(o.ptr), bytes(o.bytes) {
+  TBuf(TBuf &&o) noexcept : ctx(o.ctx), ptr(o.ptr), bytes(o.bytes), arena(o.arena) {
     o.ptr = nullptr;
     o.bytes = 0;
+    o.arena = false;
   }
   TBuf &operator=(TBuf &&o) noexcept {
     if (this != &o) {
@@ -92,8 +97,10 @@ struct TBuf {
       ctx = o.ctx;
       ptr = o.ptr;
       bytes = o.bytes;
+      arena = o.arena;
       o.ptr = nullptr;
       o.bytes = 0;
+      o.arena = false;
     }
     return *this;
   }

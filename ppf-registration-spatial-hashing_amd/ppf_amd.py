@@ -117,6 +117,7 @@ def lib():
     L.ppf_scene_keys.argtypes = [_vp, C.c_float, _u32p]
     L.ppf_lookup.argtypes = [_vp, _vp, C.POINTER(LookupParams), C.POINTER(_vp)]
     L.ppf_result_destroy.argtypes = [_vp]
+    L.ppf_lookup_noalloc.argtypes = [_vp, _vp, C.POINTER(LookupParams), _f32p, C.POINTER(ResultInfo)]
     L.ppf_result_info.argtypes = [_vp, C.POINTER(ResultInfo)]
     L.ppf_result_votes.argtypes = [_vp, _vp, _vp]
     L.ppf_result_transforms.argtypes = [_vp, _f32p]
@@ -468,6 +469,21 @@ class Model(_Owned):
         self.result = LookupResult(h, self.cpu_clustering, self.ctx)
         return self.result
 
+    def ppf_lookup_noalloc(self, scene: Scene, exact_votes: bool = False,
+                           per_reference_argmax: bool = False):
+        """Allocation-free lookup (ppf_lookup_noalloc): (pose 4x4, info dict).
+        Scenes of at most NOALLOC_MAX_SCENE_POINTS points; the device scratch
+        is one static array per GPU shared by all such calls."""
+        prm = LookupParams(self.vote_count_threshold, int(self.cpu_clustering),
+                           int(self.use_l1_norm), int(self.use_averaged_clusters),
+                           int(exact_votes), 0, int(per_reference_argmax))
+        pose = np.zeros(16, np.float32)
+        info = ResultInfo()
+        rc = lib().ppf_lookup_noalloc(self.handle, scene.handle, C.byref(prm), pose, C.byref(info))
+        if rc not in (PPF_OK, PPF_ERR_NO_VOTES):
+            _check(rc)
+        return pose.reshape(4, 4), {k: getattr(info, k) for k, _ in ResultInfo._fields_}
+
     # reference-style accessors of the last lookup
     @property
     def max_idx(self) -> int:
@@ -557,6 +573,7 @@ def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_
     return res, {k: getattr(info, k) for k, _ in BatchInfo._fields_}
 
 
+NOALLOC_MAX_SCENE_POINTS = 8192
 PPF_ERR_TIMEOUT = 6
 PPF_ERR_CANCELLED = 7
 
