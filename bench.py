@@ -1,25 +1,44 @@
 #!/usr/bin/env python
 """Benchmark of the PPF registration hot path on one MI355X.
 
-Workload (the dataset-like configuration): four seeded superquadric models of
-3k-8k points, one 50,000-point scene holding all four on a ground plane,
-ref_point_df 5 (10,000 scene reference points), vote threshold 0.3, GPU
-spatial-hash clustering.  Models and scene are built/uploaded before the timed
-region, so inputs are resident in HBM.  One step = the plain per-scene
-pipeline registering all four models in the scene (four Model::ppf_lookup
-calls, each ending with its pose on the host).
+Headline workload (the batched configuration): 16 seeded superquadric models
+of 1k-20k points (log-uniform) built once and resident on the device, 64
+scenes of 5k-120k points (log-uniform) each holding 1-4 of them on a ground
+plane with occlusion and noise, ref_point_df 5, threshold 0.3, GPU
+spatial-hash clustering.  All 148 (scene, model) jobs of that configuration
+take about half an hour of GPU time (one job alone -- a 33,856-point scene
+against the 16,863-point model, 1.2e14 votes and 6.5e7 kept poses -- takes
+150 s), so a step here is the fixed subset of jobs with n_scene * n_model
+<= 6e7 (SUBSET_MAX_PRODUCT), in generator order, voted, thresholded, posed
+and clustered by ONE ppf_batch_lookup call.  value = registrations/s of the
+batched form over the timed steps; the plain per-scene pipeline (one
+ppf_lookup per job, in sequence) runs the same jobs beside it.
 
-Prints one JSON line: value = scene pair-lookups/s over the timed steps, with
-registrations/s and p50/p99 per-registration latency beside it, the roofline
-of the vote kernel (algorithmic bytes 8*L + 4*V per launch over its HIP-event
-duration) and a CPU baseline (the oracle's OpenMP vote loop on a bounded
-sample of the same scene's reference points, extrapolated per lookup).
+Also in the line:
+* the same comparison on the wider subset n_scene * n_model <= 1e8 (one pass each);
+* the served form: requests of the served configuration (scenes of 5k-100k
+  points with 1-8 target models, restricted to targets within the headline's
+  size bound) arriving as Poisson processes of 50, 200 and 2,000 req/s, with
+  p50/p99 latency, against the plain pipeline serving the same arrivals one
+  request at a time (run, not simulated);
+* the large-hash stress workload: the 30,000-point model's build (9e8 pairs)
+  and its lookup on a 2,000-row sample of the 200,000-point scene;
+* the roofline of the dominant kernel (k_vote or k_cluster_tiled, whichever
+  takes more device time in the timed steps): HBM bytes per launch from the
+  committed rocprofv3 counter runs of this workload over the launch duration
+  measured here with HIP events, and the kernel's share of the VALU issue
+  slots (both kernels are instruction-bound);
+* a CPU baseline: the oracle (OpenMP port of the reference's loops) on one
+  job of the subset, stage by stage: model build, the vote loop on a stride
+  sample of reference rows (extrapolated per lookup), poses, and the
+  reference's --cpu_clustering greedy clustering of the kept votes.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import threading
 import time
@@ -34,84 +53,169 @@ import synth  # noqa: E402
 
 METRIC = "vote pair-lookups/s + registrations/s; p50/p99 latency, batched vs plain"
 HBM_PEAK_GBS = 8000.0
+SUBSET_MAX_PRODUCT = 6e7
+WIDE_MAX_PRODUCT = 1e8
+PROFILE_JSON = os.path.join(ROOT, "profiles", "batched_subset_counters.json")
+VALU_SLOTS_PER_S = 1024 * 2.4e9 / 4.0  # wave64 VALU instructions the chip issues per second
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def hbm_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    FETCH_SIZE/WRITE_SIZE counter runs (scaled by the copy-kernel calibration
-    recorded beside them), or None when no such run is recorded."""
-    path = os.path.join(ROOT, "profiles", "dataset_like_hbm_traffic.json")
+def counters(kernel: str):
+    """(HBM bytes, VALU instructions) per dispatch of `kernel` on the headline
+    workload, from the committed rocprofv3 counter runs (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel calibration recorded beside them)."""
     try:
-        with open(path) as f:
+        with open(PROFILE_JSON) as f:
             k = json.load(f)["kernels"][kernel]
     except (OSError, KeyError, ValueError):
         return None, None
-    return float(k["bytes_per_dispatch"]), "profiles/dataset_like_hbm_traffic.md"
+    return k.get("hbm_bytes_per_dispatch"), k.get("valu_insts_per_dispatch")
 
 
-def valu_insts(kernel: str):
-    """Wave64 VALU instructions per launch of `kernel` on this workload, from
-    the committed SQ_INSTS_VALU counter run, or None when none is recorded."""
-    path = os.path.join(ROOT, "profiles", "dataset_like_sq_counters.json")
-    try:
-        with open(path) as f:
-            return float(json.load(f)["kernels"][kernel]["valu_insts_per_dispatch"])
-    except (OSError, KeyError, ValueError):
-        return None
+def kernel_roofline(name, ms_per_launch, alg_bytes_per_launch=None):
+    traffic, valu = counters(name)
+    out = {"kernel": name, "bound": "valu", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "ms_per_launch": ms_per_launch, "traffic": traffic,
+           "traffic_source": os.path.relpath(PROFILE_JSON, ROOT) if traffic else None}
+    s = ms_per_launch * 1e-3
+    if traffic and s > 0:
+        out["achieved"] = traffic / s / 1e9
+        out["frac"] = out["achieved"] / HBM_PEAK_GBS
+    else:
+        out["achieved"] = out["frac"] = None
+    out["valu_issue_share"] = (valu / (VALU_SLOTS_PER_S * s)) if valu and s > 0 else None
+    if alg_bytes_per_launch is not None:
+        # the survey's algorithmic bytes (8 B per lookup + 4 B per vote) over
+        # the launch time, as a ratio to the HBM peak: >1 means the kernel
+        # serves the per-vote pair ids from L2/LDS instead of HBM
+        out["alg_bytes_per_launch"] = alg_bytes_per_launch
+        out["alg_rate_over_hbm_peak"] = alg_bytes_per_launch / s / 1e9 / HBM_PEAK_GBS if s > 0 else None
+    for key in ("frac", "valu_issue_share"):
+        if out.get(key) is not None:
+            assert out[key] <= 1.0, f"{name}: {key} {out[key]} > 1"
+    return out
 
 
-def cpu_baseline(w, budget_s: float):
-    """Oracle (hoisted-frame OpenMP port of the reference vote loop) on a
-    stride sample of the scene's reference points, all four models."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import ppf_oracle as O
-    scene = w.scenes[0]
-    n_s = len(scene[0])
-    refs_all = np.arange(0, n_s, w.rpdf, dtype=np.int64)
-    tot_l = tot_v = 0
-    tot_s = 0.0
-    sampled = 0
-    per_model = budget_s / len(w.models)
-    for m, d in zip(w.models, w.d_dists):
-        om = O.OracleModel(m, d)
-        # one stride-sampled reference point first, then as many as fit the
-        # per-model budget (the two samples are disjoint)
-        k = 1
-        refs = refs_all[len(refs_all) // 2: len(refs_all) // 2 + 1]
-        r = om.vote_throughput(scene, refs)
-        left = per_model - r["seconds"]
-        if r["seconds"] > 0 and left > r["seconds"]:
-            k2 = int(min(len(refs_all) - 1, left / r["seconds"]))
-            if k2 >= 1:
-                more = refs_all[:: max(1, len(refs_all) // k2)][:k2]
-                more = more[more != refs[0]]
-                r2 = om.vote_throughput(scene, more)
-                r = {key: r[key] + r2[key] for key in r}
-                refs = np.concatenate([refs, more])
-        tot_l += r["lookups"]
-        tot_v += r["votes"]
-        tot_s += r["seconds"]
-        sampled += len(refs)
-        del om
-    return dict(value=tot_l / tot_s, unit="pair-lookups/s", cores=O.num_threads(), kind="port",
-                sample=f"oracle vote loop (OpenMP, hoisted frames) on {sampled} of "
-                       f"{len(refs_all) * len(w.models)} (model, reference point) rows, "
-                       f"{tot_s:.1f} s, extrapolated per lookup",
-                votes_per_s=tot_v / tot_s)
+def subset(w, max_product):
+    jobs = []
+    for si, ids in enumerate(w.scene_models):
+        for mi in ids:
+            if len(w.scenes[si][0]) * len(w.models[mi][0]) <= max_product:
+                jobs.append((si, mi))
+    return jobs
+
+
+def run_batched(ctx, models, scenes, jobs, thr):
+    t = time.perf_counter()
+    res, info = ppf_amd.batch_lookup([models[mi] for _, mi in jobs], [scenes[si] for si, _ in jobs], thr,
+                                     ctx=ctx)
+    infos = []
+    for r in res:
+        r.pose()
+        infos.append(dict(r.info))
+        r.close()
+    return time.perf_counter() - t, info, infos
+
+
+def run_plain(models, scenes, jobs):
+    lat, infos = [], []
+    t0 = time.perf_counter()
+    for si, mi in jobs:
+        t = time.perf_counter()
+        r = models[mi].ppf_lookup(scenes[si])
+        r.pose()
+        lat.append(time.perf_counter() - t)
+        infos.append(dict(r.info))
+        r.close()
+    return time.perf_counter() - t0, lat, infos
+
+
+def pct(x, q):
+    return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else None
+
+
+def run_served(ctx, models, w, rates=(50.0, 200.0, 2000.0), n_requests=24):
+    """Served form against the plain pipeline on the same Poisson arrivals."""
+    t0 = time.time()
+    reqs = synth.served_requests(
+        w, seed=5, n_requests=n_requests,
+        keep=lambda n_s, mi: n_s * len(w.models[mi][0]) <= SUBSET_MAX_PRODUCT)
+    log(f"served requests generated in {time.time() - t0:.1f}s")
+    n_targets = sum(len(ids) for _, ids in reqs)
+    desc = {"requests": len(reqs), "targets": n_targets, "scene_points": [len(s[0]) for s, _ in reqs],
+            "target_ids": [ids for _, ids in reqs],
+            "rule": f"served configuration (scenes 5k-100k, 1-8 targets) keeping targets with "
+                    f"n_scene * n_model <= {SUBSET_MAX_PRODUCT:.0e}"}
+
+    def plain_request(sc, ids):
+        scene = ppf_amd.Scene(sc, 0.0, w.rpdf, ctx=ctx)
+        for mi in ids:
+            r = models[mi].ppf_lookup(scene)
+            r.pose()
+            r.close()
+        scene.close()
+
+    plain_request(*reqs[0])  # warm-up
+    rng = np.random.default_rng(11)
+    out = []
+    srv = ppf_amd.Server(models, round_ms=20.0, ctx=ctx)
+    for rate in rates:
+        arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+        arr -= arr[0]
+        # served: a submitter thread releases requests at their arrival times
+        tickets = [None] * len(reqs)
+
+        def submitter():
+            t_start = time.perf_counter()
+            for k, (sc, ids) in enumerate(reqs):
+                d = t_start + arr[k] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[k] = srv.submit(sc, ids, w.rpdf, w.vote_count_threshold)
+
+        th = threading.Thread(target=submitter)
+        t0 = time.perf_counter()
+        th.start()
+        lat = []
+        for k in range(len(reqs)):
+            while tickets[k] is None:
+                time.sleep(0.0005)
+            _, _, info, _ = srv.wait(tickets[k], 600.0)
+            if info["rc"] != ppf_amd.PPF_OK:
+                raise RuntimeError(f"served request failed: {info}")
+            lat.append(info["latency_ms"])
+        th.join()
+        wall = time.perf_counter() - t0
+        # plain: one request at a time in arrival order, latency from arrival
+        plat = []
+        t_start = time.perf_counter()
+        for k, (sc, ids) in enumerate(reqs):
+            d = t_start + arr[k] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            plain_request(sc, ids)
+            plat.append((time.perf_counter() - t_start - arr[k]) * 1e3)
+        pwall = time.perf_counter() - t_start
+        out.append({"rate_req_per_s": rate, "registrations_per_s": n_targets / wall,
+                    "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                    "plain_registrations_per_s": n_targets / pwall,
+                    "plain_p50_ms": pct(plat, 50), "plain_p99_ms": pct(plat, 99)})
+        log(f"served at {rate} req/s: {wall:.2f}s (plain {pwall:.2f}s)")
+    st = srv.stats()
+    srv.close()
+    return {"workload": desc, "rates": out,
+            "server": {"rounds": st["rounds"], "max_requests_in_round": st["max_requests_in_round"],
+                       "peak_inflight_bytes": st["peak_inflight_bytes"]}}
 
 
 def run_large_hash(ctx, rpdf=100):
     """Large-hash stress workload: one 30,000-point superquadric with 8 bumps
     (9e8 model pairs) and a 200,000-point scene with three instances and 40%
     clutter.  The model build (keys, hash-array sort, vote order, records) is
-    timed in full, twice; voting runs on every 100th scene point (2,000
-    reference points, 4e8 lookups) rather than every 5th (40,000 reference
-    points, 8e9 lookups, about four minutes of voting), so that the bench
-    stays within minutes; throughput per lookup is what it reports."""
+    timed in full, twice; the lookup votes every rpdf-th scene point."""
     w = synth.large_hash()
     builds = []
     m = None
@@ -132,176 +236,108 @@ def run_large_hash(ctx, rpdf=100):
     r.close()
     scene.close()
     m.close()
-    log(f"large-hash sample lookup: {dt:.1f}s")
+    log(f"large-hash lookup (ref_point_df {rpdf}): {dt:.1f}s")
     return {"model_points": len(w.models[0][0]), "model_pairs": info["n_pairs"],
             "unique_keys": info["n_unique_keys"], "model_build_ms": builds[-1],
             "model_pairs_per_s": info["n_pairs"] / (builds[-1] * 1e-3),
             "scene_points": len(w.scenes[0][0]), "ref_point_df": rpdf,
             "lookups": i["lookups"], "votes": i["votes"], "vote_ms": i["vote_ms"],
             "lookups_per_s": i["lookups"] / (i["vote_ms"] * 1e-3),
-            "votes_per_s": i["votes"] / (i["vote_ms"] * 1e-3),
-            "n_kept": i["n_kept"], "cluster_ms": i["cluster_ms"], "lookup_s": dt}
+            "votes_per_s": i["votes"] / (i["vote_ms"] * 1e-3), "n_kept": i["n_kept"],
+            "cluster_ms": i["cluster_ms"], "lookup_s": dt}
 
 
-def run_forms(ctx, plain_models, plain_scene, w, log_every=True):
-    """Batched and served forms beside the plain pipeline.
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model or platform.processor(), "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
 
-    1. The dataset-like jobs (4 models x the 50,000-point scene) as one
-       ppf_batch_lookup call, against the plain step.
-    2. A request set shaped like the batched/served configurations but sized
-       to finish within seconds: 8 seeded superquadric models (1k-6k points)
-       built once, 48 scenes of 2k-20k points (log-uniform) each holding 1-4
-       of them.  Plain = each request's scene uploaded and looked up target by
-       target; batched = all (scene, model) jobs in one ppf_batch_lookup;
-       served = ppf_server with Poisson arrivals at 50, 200 and 2000 req/s.
-       The seed gives models with at least 0.45 unique keys per point, as in
-       the dataset-like models; flat-faced superquadrics with ~0.1 (huge
-       buckets) keep 10^7-10^8 votes per job and take minutes per job in
-       clustering, in the reference's algorithm as here.
-    3. The large-hash stress workload (run_large_hash).
-    """
-    out = {}
-    # ---- 1. dataset-like jobs batched
+
+def cpu_baseline(w, jobs, plain_infos, models, scenes, budget_s: float):
+    """The oracle on one job of the subset, stage by stage (steady clock):
+    model build, the OpenMP vote loop on a stride sample of reference rows,
+    poses of the kept votes, and the --cpu_clustering greedy clustering."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ppf_oracle as O
+    # the largest job with a model of at most 5,000 points and at most 1.5e5
+    # kept votes (the greedy clustering is O(kept x clusters))
+    cand = [(len(w.scenes[si][0]) * len(w.models[mi][0]), k) for k, (si, mi) in enumerate(jobs)
+            if len(w.models[mi][0]) <= 5000 and plain_infos[k]["n_kept"] <= 150000]
+    k = max(cand)[1]
+    si, mi = jobs[k]
+    m, s, d = w.models[mi], w.scenes[si], w.d_dists[mi]
+    n_s = len(s[0])
     t = time.perf_counter()
-    res, info = ppf_amd.batch_lookup(plain_models, [plain_scene] * len(plain_models),
-                                     w.vote_count_threshold, cpu_clustering=w.cpu_clustering, ctx=ctx)
-    for r in res:
-        r.pose()
-        r.close()
-    dt = time.perf_counter() - t
-    out["dataset_like_batched"] = {"jobs": len(plain_models), "s": dt,
-                                   "registrations_per_s": len(plain_models) / dt,
-                                   "lookups_per_s": info["lookups"] / dt,
-                                   "vote_launches": info["vote_launches"], "vote_ms": info["vote_ms"],
-                                   "finish_ms": info["finish_ms"]}
-    log(f"batched dataset-like: {dt:.2f}s")
-
-    # ---- 2. request set
-    rw = synth.batched(seed=11, n_models=8, n_scenes=48, model_range=(1000, 6000),
-                       scene_range=(2000, 20000), models_per_scene=(1, 4))
-    models = [ppf_amd.Model(m, d, rw.vote_count_threshold, ctx=ctx) for m, d in zip(rw.models, rw.d_dists)]
-    reqs = list(zip(rw.scenes, rw.scene_models))
-    n_targets = sum(len(ids) for _, ids in reqs)
-    desc = {"models": len(models), "model_points": [len(m[0]) for m in rw.models],
-            "requests": len(reqs), "targets": n_targets,
-            "scene_points": [len(s[0]) for s, _ in reqs], "ref_point_df": rw.rpdf,
-            "threshold": rw.vote_count_threshold, "clustering": "gpu"}
-
-    def plain_request(sc, ids):
-        t0 = time.perf_counter()
-        scene = ppf_amd.Scene(sc, 0.0, rw.rpdf, ctx=ctx)
-        L = 0
-        for mi in ids:
-            r = models[mi].ppf_lookup(scene)
-            r.pose()
-            L += r.info["lookups"]
-            r.close()
-        scene.close()
-        return time.perf_counter() - t0, L
-
-    plain_request(*reqs[0])  # warm-up
-    service, lookups = [], 0
-    t0 = time.perf_counter()
-    for sc, ids in reqs:
-        dt, L = plain_request(sc, ids)
-        service.append(dt)
-        lookups += L
-    plain_s = time.perf_counter() - t0
-    log(f"plain request set: {plain_s:.2f}s")
-
-    # batched: every (scene, model) job in one call (scenes resident)
-    scenes = [ppf_amd.Scene(sc, 0.0, rw.rpdf, ctx=ctx) for sc, _ in reqs]
-    jm = [models[mi] for _, ids in reqs for mi in ids]
-    js = [scenes[k] for k, (_, ids) in enumerate(reqs) for _ in ids]
-    res, _ = ppf_amd.batch_lookup(jm, js, rw.vote_count_threshold, ctx=ctx)  # warm-up
-    for r in res:
-        r.close()
-    t0 = time.perf_counter()
-    res, binfo = ppf_amd.batch_lookup(jm, js, rw.vote_count_threshold, ctx=ctx)
-    for r in res:
-        r.pose()
-        r.close()
-    batch_s = time.perf_counter() - t0
-    for sc in scenes:
-        sc.close()
-    log(f"batched request set: {batch_s:.2f}s")
-
-    # served: Poisson arrivals; the plain comparison is the same arrivals
-    # queued FIFO in front of the plain per-request service times above
-    served = []
-    srv = ppf_amd.Server(models, round_ms=20.0, ctx=ctx)
-    rng = np.random.default_rng(11)
-    for rate in (50.0, 200.0, 2000.0):
-        arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
-        arr -= arr[0]
-        tickets = [None] * len(reqs)
-
-        def submitter():
-            t_start = time.perf_counter()
-            for k, (sc, ids) in enumerate(reqs):
-                d = t_start + arr[k] - time.perf_counter()
-                if d > 0:
-                    time.sleep(d)
-                tickets[k] = srv.submit(sc, ids, rw.rpdf, rw.vote_count_threshold)
-
-        th = threading.Thread(target=submitter)
-        t0 = time.perf_counter()
-        th.start()
-        lat = []
-        for k in range(len(reqs)):
-            while tickets[k] is None:
-                time.sleep(0.0005)
-            _, st, info, _ = srv.wait(tickets[k], 600.0)
-            if info["rc"] != ppf_amd.PPF_OK:
-                raise RuntimeError(f"served request failed: {info}")
-            lat.append(info["latency_ms"])
-        th.join()
-        wall = time.perf_counter() - t0
-        fin, plat = 0.0, []
-        for a, sv in zip(arr, service):
-            fin = max(fin, a) + sv
-            plat.append((fin - a) * 1e3)
-        served.append({"rate_req_per_s": rate, "registrations_per_s": n_targets / wall,
-                       "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                       "plain_registrations_per_s": n_targets / fin,
-                       "plain_p50_ms": float(np.percentile(plat, 50)),
-                       "plain_p99_ms": float(np.percentile(plat, 99))})
-        log(f"served at {rate} req/s: {wall:.2f}s")
-    st = srv.stats()
-    srv.close()
-    for m in models:
-        m.close()
-    out["request_set"] = {
-        "workload": desc,
-        "plain": {"s": plain_s, "registrations_per_s": n_targets / plain_s, "lookups_per_s": lookups / plain_s},
-        "batched": {"s": batch_s, "registrations_per_s": n_targets / batch_s,
-                    "lookups_per_s": binfo["lookups"] / batch_s,
-                    "speedup_vs_plain": plain_s / batch_s, "vote_launches": binfo["vote_launches"],
-                    "vote_ms": binfo["vote_ms"], "finish_ms": binfo["finish_ms"]},
-        "served": served,
-        "server": {"rounds": st["rounds"], "max_requests_in_round": st["max_requests_in_round"],
-                   "peak_inflight_bytes": st["peak_inflight_bytes"]},
-    }
-    out["large_hash_sample"] = run_large_hash(ctx)
-    return out
+    om = O.OracleModel(m, d)
+    build_s = time.perf_counter() - t
+    refs_all = np.arange(0, n_s, w.rpdf, dtype=np.int64)
+    r = om.vote_throughput(s, refs_all[len(refs_all) // 2: len(refs_all) // 2 + 1])
+    left = budget_s - r["seconds"]
+    k2 = int(min(len(refs_all) - 1, max(0.0, left) / max(r["seconds"], 1e-6)))
+    if k2 >= 1:
+        more = refs_all[:: max(1, len(refs_all) // k2)][:k2]
+        more = more[more != refs_all[len(refs_all) // 2]]
+        r2 = om.vote_throughput(s, more)
+        r = {key: r[key] + r2[key] for key in r}
+        sampled = 1 + len(more)
+    else:
+        sampled = 1
+    del om
+    vote_full_s = r["seconds"] * len(refs_all) / sampled
+    res = models[mi].ppf_lookup(scenes[si])
+    codes, counts = res.votes()
+    res.close()
+    t = time.perf_counter()
+    T = O.poses(codes, m, s)
+    pose_s = time.perf_counter() - t
+    d_angle0 = np.float32(np.float32(2.0 * np.float32(3.141592654)) / np.float32(30.0))
+    t = time.perf_counter()
+    _, _, n_clusters = O.cluster_greedy(T, counts, d, d_angle0)
+    cluster_s = time.perf_counter() - t
+    total_s = build_s + vote_full_s + pose_s + cluster_s
+    return dict(value=r["lookups"] / r["seconds"], unit="pair-lookups/s", cores=O.num_threads(),
+                kind="port",
+                sample=f"oracle on the job (scene of {n_s} points, model of {len(m[0])} points): vote "
+                       f"loop (OpenMP, hoisted frames) on {sampled} of {len(refs_all)} reference rows, "
+                       f"{r['seconds']:.1f} s, extrapolated per lookup; model build, poses and greedy "
+                       f"clustering of the {len(codes)} kept votes timed in full",
+                votes_per_s=r["votes"] / r["seconds"],
+                stages_s={"model_build": build_s, "vote_extrapolated": vote_full_s, "poses": pose_s,
+                          "cpu_clustering": cluster_s, "registration_total": total_s},
+                registrations_per_s=1.0 / total_s, n_clusters=n_clusters, **cpu_info())
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--exact-votes", action="store_true")
-    ap.add_argument("--no-forms", action="store_true", help="skip the batched and served forms")
-    ap.add_argument("--forms-only", action="store_true",
-                    help="only the batched and served forms (no plain steps, no CPU baseline)")
-    ap.add_argument("--large-hash-only", action="store_true",
-                    help="only the large-hash stress workload (30,000-point model, 200,000-point scene)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed batched steps (for profiling runs)")
+    ap.add_argument("--max-product", type=float, default=SUBSET_MAX_PRODUCT,
+                    help="subset rule: jobs with n_scene * n_model <= this")
+    ap.add_argument("--large-hash-rpdf", type=int, default=100,
+                    help="reference-point stride of the large-hash lookup (5 = the full configuration)")
+    ap.add_argument("--large-hash-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(voting would shard by scene reference point)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         ctx = ppf_amd.Context(0)
         a = np.arange(1 << 26, dtype=np.uint32)
@@ -311,62 +347,57 @@ def main():
         log(f"copied {a.nbytes} bytes x3")
         return
     if args.large_hash_only:
-        print(json.dumps({"large_hash_sample": run_large_hash(ppf_amd.Context(0))}), flush=True)
+        print(json.dumps({"large_hash": run_large_hash(ppf_amd.Context(0), args.large_hash_rpdf)}),
+              flush=True)
         return
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(voting would shard by scene reference point)", file=sys.stderr)
-        sys.exit(2)
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the oracle's OpenMP runtime loads
 
     t0 = time.time()
-    w = synth.dataset_like()
-    log(f"workload generated in {time.time() - t0:.1f}s: models "
-        f"{[len(m[0]) for m in w.models]}, scene {len(w.scenes[0][0])}")
+    w = synth.batched()
+    jobs = subset(w, args.max_product)
+    wide = subset(w, WIDE_MAX_PRODUCT)
+    log(f"batched workload generated in {time.time() - t0:.1f}s: {len(w.models)} models, "
+        f"{len(w.scenes)} scenes; subset {len(jobs)} jobs, wide subset {len(wide)} jobs")
     ctx = ppf_amd.Context(0)
     t0 = time.time()
-    models = [ppf_amd.Model(m, d, w.vote_count_threshold, cpu_clustering=w.cpu_clustering, ctx=ctx)
-              for m, d in zip(w.models, w.d_dists)]
-    log(f"models built in {time.time() - t0:.2f}s")
-    scene = ppf_amd.Scene(w.scenes[0], 0.0, w.rpdf, ctx=ctx)
-
-    if args.forms_only:
-        print(json.dumps({"forms": run_forms(ctx, models, scene, w)}), flush=True)
-        return
-
-    def step(record):
-        for mdl in models:
-            t = time.perf_counter()
-            res = mdl.ppf_lookup(scene, exact_votes=args.exact_votes)
-            res.pose()
-            dt = time.perf_counter() - t
-            if record is not None:
-                record.append((dt, dict(res.info)))
-            res.close()
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    build_s = time.time() - t0
+    log(f"16 models built in {build_s:.2f}s")
+    used = sorted({si for si, _ in wide})
+    scenes = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in used}
 
     for i in range(args.warmup):
-        step(None)
+        run_batched(ctx, models, scenes, jobs, w.vote_count_threshold)
         log(f"warmup {i + 1}/{args.warmup} done")
-    recs = []
+    step_s, step_info, job_infos = [], [], []
     t_start = time.perf_counter()
     for i in range(args.steps):
-        step(recs)
-        log(f"step {i + 1}/{args.steps} done")
+        dt, info, infos = run_batched(ctx, models, scenes, jobs, w.vote_count_threshold)
+        step_s.append(dt)
+        step_info.append(info)
+        job_infos.extend(infos)
+        log(f"step {i + 1}/{args.steps} done ({dt:.2f}s)")
     elapsed = time.perf_counter() - t_start
-
-    lookups = sum(r["lookups"] for _, r in recs)
-    votes = sum(r["votes"] for _, r in recs)
-    vote_ms = sum(r["vote_ms"] for _, r in recs)
-    lat = np.array([dt for dt, _ in recs]) * 1e3
-    n_launch = len(recs)
-    alg_bytes = 8.0 * lookups + 4.0 * votes
-    achieved = alg_bytes / n_launch / (vote_ms / n_launch * 1e-3) / 1e9
-    vote_traffic, traffic_src = hbm_traffic("k_vote<false>" if not args.exact_votes else "k_vote<true>")
-    valu = valu_insts("k_vote<false>") if not args.exact_votes else None
+    regs = len(jobs) * args.steps
+    lookups = sum(i["lookups"] for i in step_info)
+    votes = sum(i["votes"] for i in step_info)
+    vote_ms = sum(i["vote_ms"] for i in step_info)
+    vote_launches = sum(i["vote_launches"] for i in step_info)
+    cl_ms = sum(i["cluster_kernel_ms"] for i in job_infos)
+    cl_launches = sum(1 for i in job_infos if i["cluster_kernel_ms"] > 0)
+    kernels = {
+        "k_vote": kernel_roofline("k_vote<false>", vote_ms / vote_launches,
+                                  (8.0 * lookups + 4.0 * votes) / vote_launches),
+        "k_cluster_tiled": kernel_roofline("k_cluster_tiled<0, 0>", cl_ms / max(cl_launches, 1)),
+    }
+    dominant = "k_vote" if vote_ms >= cl_ms else "k_cluster_tiled"
+    roofline = dict(kernels[dominant])
+    roofline["device_ms_per_step"] = {"k_vote": vote_ms / args.steps, "k_cluster_tiled": cl_ms / args.steps}
+    roofline["kernels"] = kernels
     out = {
         "metric": METRIC,
-        "value": lookups / elapsed,
-        "unit": "pair-lookups/s",
+        "value": regs / elapsed,
+        "unit": "registrations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -377,48 +408,62 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "dataset_like: 4 seeded superquadric models (3k-8k points), one 50,000-point "
-                        "scene with all four on a ground plane, ref_point_df 5, threshold 0.3, "
-                        "GPU spatial-hash clustering, plain per-scene pipeline",
-            "model_points": [len(m[0]) for m in w.models],
-            "scene_points": len(w.scenes[0][0]),
+            "workload": f"batched configuration (16 resident superquadric models of 1k-20k points, 64 scenes "
+                        f"of 5k-120k points with 1-4 models each, ref_point_df 5, threshold 0.3, GPU "
+                        f"spatial-hash clustering), subset of the jobs with n_scene * n_model <= "
+                        f"{args.max_product:.0e}: {len(jobs)} of 148 jobs per step, one ppf_batch_lookup",
+            "form": "batched",
+            "jobs": len(jobs),
+            "scene_points": [len(w.scenes[si][0]) for si, _ in jobs],
+            "model_points": [len(w.models[mi][0]) for _, mi in jobs],
             "ref_point_df": w.rpdf,
-            "d_dist": w.d_dists,
-            "form": "plain",
-            "exact_votes": bool(args.exact_votes),
+            "model_build_s_all_16": build_s,
         },
-        "registrations_per_s": n_launch / elapsed,
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "pair_lookups_per_s": lookups / elapsed,
         "votes_per_s": votes / elapsed,
-        "lookups_per_step": lookups / args.steps,
-        "votes_per_step": votes / args.steps,
-        "vote_kernel_ms_per_launch": vote_ms / n_launch,
-        "roofline": {
-            "kernel": "k_vote",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": vote_traffic,
-            "alg_bytes_per_launch": alg_bytes / n_launch,
-            "traffic_source": traffic_src,
-            # the counters' bytes over the same launch time: what HBM really moved
-            "measured_gbs": (vote_traffic / (vote_ms / n_launch * 1e-3) / 1e9) if vote_traffic else None,
-            "measured_frac": (vote_traffic / (vote_ms / n_launch * 1e-3) / 1e9 / HBM_PEAK_GBS)
-                             if vote_traffic else None,
-            # what bounds the kernel: its share of the chip's VALU issue slots
-            # (recorded instruction count x 4 cycles per wave64 instruction
-            # over 1,024 SIMDs at 2.4 GHz for the launch time measured here)
-            "valu_issue_share": (valu * 4.0 / (1024 * 2.4e9 * vote_ms / n_launch * 1e-3)) if valu else None,
-            "valu_source": "profiles/dataset_like_sq_counters.md" if valu else None,
-        },
+        "latency_ms": {"p50": pct(step_s, 50) * 1e3, "p99": pct(step_s, 99) * 1e3},
+        "vote_kernel_ms_per_step": vote_ms / args.steps,
+        "roofline": roofline,
     }
-    if not args.no_forms:
-        out["forms"] = run_forms(ctx, models, scene, w)
+    if args.headline_only:
+        print(json.dumps(out), flush=True)
+        return
+
+    # ---- plain pipeline on the same jobs
+    run_plain(models, scenes, jobs[:2])  # warm-up
+    p_s, p_lat, p_infos = run_plain(models, scenes, jobs)
+    log(f"plain subset: {p_s:.2f}s")
+    out["plain"] = {"s": p_s, "registrations_per_s": len(jobs) / p_s,
+                    "pair_lookups_per_s": sum(i["lookups"] for i in p_infos) / p_s,
+                    "latency_ms": {"p50": pct(p_lat, 50) * 1e3, "p99": pct(p_lat, 99) * 1e3},
+                    "vote_ms": sum(i["vote_ms"] for i in p_infos),
+                    "cluster_kernel_ms": sum(i["cluster_kernel_ms"] for i in p_infos)}
+    out["speedup_batched_vs_plain"] = out["value"] / out["plain"]["registrations_per_s"]
+    # bit-identical results are tested in tests/; here the kept-vote counts
+    # of the two forms are compared as a sanity check
+    assert [i["n_kept"] for i in job_infos[-len(jobs):]] == [i["n_kept"] for i in p_infos]
+
+    # ---- wider subset, one pass each
+    b_s, b_info, _ = run_batched(ctx, models, scenes, wide, w.vote_count_threshold)
+    pw_s, pw_lat, _ = run_plain(models, scenes, wide)
+    log(f"wide subset: batched {b_s:.2f}s, plain {pw_s:.2f}s")
+    out["wide_subset"] = {"rule": f"n_scene * n_model <= {WIDE_MAX_PRODUCT:.0e}", "jobs": len(wide),
+                          "batched_registrations_per_s": len(wide) / b_s,
+                          "plain_registrations_per_s": len(wide) / pw_s,
+                          "speedup_batched_vs_plain": pw_s / b_s,
+                          "plain_latency_ms": {"p50": pct(pw_lat, 50) * 1e3, "p99": pct(pw_lat, 99) * 1e3},
+                          "vote_ms": b_info["vote_ms"], "finish_ms": b_info["finish_ms"]}
+    for s in scenes.values():
+        s.close()
+
+    out["served"] = run_served(ctx, models, w)
+    out["large_hash"] = run_large_hash(ctx, args.large_hash_rpdf)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        sc = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in sorted({si for si, _ in jobs})}
+        out["cpu_baseline"] = cpu_baseline(w, jobs, p_infos, models, sc, args.cpu_seconds)
+        for s in sc.values():
+            s.close()
     print(json.dumps(out), flush=True)
 
 

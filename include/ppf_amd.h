@@ -152,6 +152,7 @@ typedef struct {
                                    buckets than the vote kernel groups, voted hit by hit */
   int64_t cluster_mixed_chunks; /* GPU clustering: pose chunks spanning several cells that
                                    share a spatial hash (FNV collision) */
+  double cluster_kernel_ms;     /* GPU clustering: device time of the pair-test kernel alone */
 } ppf_result_info_t;
 int ppf_result_info(ppf_result result, ppf_result_info_t *info);
 

@@ -473,7 +473,8 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
                                 r->counts.as<uint32_t>(), m->weights.as<float>(), P, m->d_dist,
                                 p->use_l1_norm != 0, p->use_averaged_clusters != 0, r->vc.as<float>(),
                                 r->trans.as<float>(), r->quats.as<float>(), &mi,
-                                &r->info.cluster_tests, &r->info.cluster_mixed_chunks));
+                                &r->info.cluster_tests, &r->info.cluster_mixed_chunks,
+                                &r->info.cluster_kernel_ms));
       r->info.max_idx = mi;
       float t3[3];
       HIPC(hipMemcpyAsync(r->pose, r->T.as<float>() + mi * 16, 64, hipMemcpyDeviceToHost, st));
@@ -1214,9 +1215,10 @@ int ppf_debug_cluster(ppf_ctx ctx, const float *T, const uint64_t *codes, const 
     HIPC(hipMemcpy(dw.ptr, weights, (size_t)n_weights * 4, hipMemcpyHostToDevice));
   }
   int64_t mi = -1, tests = 0, mixed = 0;
+  double kms = 0.0;
   TRY(ppfk::cluster_spatial(ctx, dT.as<float>(), dc.as<unsigned long long>(), dn.as<uint32_t>(),
                             weights ? dw.as<float>() : nullptr, P, d_dist, use_l1 != 0, use_avg != 0,
-                            dvc.as<float>(), dtr.as<float>(), dq.as<float>(), &mi, &tests, &mixed));
+                            dvc.as<float>(), dtr.as<float>(), dq.as<float>(), &mi, &tests, &mixed, &kms));
   HIPC(hipMemcpyAsync(vc_out, dvc.ptr, (size_t)P * 4, hipMemcpyDeviceToHost, st));
   HIPC(hipMemcpyAsync(trans_out, dtr.ptr, (size_t)P * 12, hipMemcpyDeviceToHost, st));
   HIPC(hipMemcpyAsync(quats_out, dq.ptr, (size_t)P * 16, hipMemcpyDeviceToHost, st));

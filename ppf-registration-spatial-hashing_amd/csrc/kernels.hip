@@ -1342,36 +1342,40 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   }
 }
 
-// first index of the maximum (thrust::max_element), one block
-__global__ __launch_bounds__(1024) void k_argmax(const float *__restrict__ v, int64_t P,
-                                                 long long *__restrict__ out) {
-  __shared__ float sv[1024];
-  __shared__ long long si[1024];
-  float bv = 0.0f;
-  long long bi = -1;
-  for (int64_t i = threadIdx.x; i < P; i += 1024) {
-    const float x = v[i];
-    if (bi < 0 || bv < x) {
-      bv = x;
-      bi = i;
-    }
+// First index of the maximum (thrust::max_element, model.cu:293-295) as one
+// 64-bit atomicMax over the grid: key = order-preserving bits of the value
+// (-0 counted as +0, NaN below every number) << 32 | ~index, so the largest
+// value wins and, among equal values, the smallest index.
+__device__ __forceinline__ unsigned long long argmax_key(float x, uint32_t i) {
+  uint32_t b = ppfm::fbits(x);
+  if ((b & 0x7fffffffu) == 0u) b = 0u;
+  uint32_t k;
+  if ((b & 0x7fffffffu) > 0x7f800000u)
+    k = 0u;  // NaN
+  else
+    k = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  return ((unsigned long long)k << 32) | (unsigned long long)(0xffffffffu - i);
+}
+
+__global__ __launch_bounds__(256) void k_argmax(const float *__restrict__ v, int64_t P,
+                                                unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long sk[256 / WAVE];
+  unsigned long long best = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < P; i += (int64_t)gridDim.x * 256) {
+    const unsigned long long k = argmax_key(v[i], (uint32_t)i);
+    best = k > best ? k : best;
   }
-  sv[threadIdx.x] = bv;
-  si[threadIdx.x] = bi;
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(best, o, WAVE);
+    best = w > best ? w : best;
+  }
+  if (threadIdx.x % WAVE == 0) sk[threadIdx.x / WAVE] = best;
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const float ov = sv[threadIdx.x + s];
-      const long long oi = si[threadIdx.x + s];
-      const long long mi = si[threadIdx.x];
-      if (oi >= 0 && (mi < 0 || sv[threadIdx.x] < ov || (!(ov < sv[threadIdx.x]) && oi < mi))) {
-        sv[threadIdx.x] = ov;
-        si[threadIdx.x] = oi;
-      }
-    }
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 256 / WAVE; ++k) best = sk[k] > best ? sk[k] : best;
+    atomicMax(out, best);
   }
-  if (threadIdx.x == 0) *out = si[0];
 }
 
 // ---------------------------------------------------------- diagnostics
@@ -1610,7 +1614,6 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
   PPF_HIP(hipMemcpyAsync(ctx->tmp_a.ptr, h_jobs, sizeof(VoteJob) * n_jobs, hipMemcpyHostToDevice,
                          ctx->stream));
   PPF_HIP(hipMemsetAsync(ctx->queue.ptr, 0, 16, ctx->stream));
-  PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   // heaviest rows first, when there are more rows than blocks
   const uint64_t *order = nullptr;
   if (n_items > (uint64_t)G && n_items < ((uint64_t)1 << 32)) {
@@ -1622,6 +1625,8 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
     PPF_TRY(radix_sort_u64(keys, nullptr, (int64_t)n_items, 32, 64, ctx->sort, ctx->stream));
     order = keys;
   }
+  // vote_ms times the vote kernel alone
+  PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream,
                      ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items, order,
                      ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4,
@@ -1705,10 +1710,11 @@ int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const
 int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
-                    int64_t *max_idx, int64_t *pair_tests, int64_t *mixed_chunks) {
+                    int64_t *max_idx, int64_t *pair_tests, int64_t *mixed_chunks, double *kernel_ms) {
   *max_idx = -1;
   *pair_tests = 0;
   *mixed_chunks = 0;
+  *kernel_ms = 0.0;
   if (P <= 0) return 0;
   // cq, ct (2 x float4 P) words, okeys (2 x u64 P) trans (3P) w (P) adj (27P)
   // ukeys/ufirst/ucount (3P) members, centers (2P) argmax (8) test counter (8)
@@ -1742,7 +1748,8 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   long long *am = reinterpret_cast<long long *>(((uintptr_t)p + 7) & ~(uintptr_t)7);
   unsigned long long *ntest = reinterpret_cast<unsigned long long *>(am + 1);
   unsigned long long *nmixed = ntest + 1;
-  PPF_HIP(hipMemsetAsync(ntest, 0, 16, ctx->stream));
+  unsigned long long *amax = ntest + 2;
+  PPF_HIP(hipMemsetAsync(ntest, 0, 24, ctx->stream));
   hipLaunchKernelGGL(k_cluster_prep, dim3(nblk(P, 128)), dim3(128), 0, ctx->stream, T, codes, counts,
                      weights, P, d_dist, trans, quats, w, words, adj);
   PPF_HIP(hipGetLastError());
@@ -1778,18 +1785,26 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
     PPF_TRY(radix_sort_u64(order, nullptr, n_items, 32, 64, ctx->sort, ctx->stream));
     auto kern = use_avg ? (use_l1 ? k_cluster_tiled<1, 1> : k_cluster_tiled<0, 1>)
                         : (use_l1 ? k_cluster_tiled<1, 0> : k_cluster_tiled<0, 0>);
+    PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(kern, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, cq, ct, adj,
                        ukeys, ufirst, ucount, U, cstart, centers, order, sqrt_below_max(d_dist), d_dist,
                        vc, trans_out, ntest, nmixed);
+    PPF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   }
-  hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, ctx->stream, vc, P, am);
+  hipLaunchKernelGGL(k_argmax, dim3((unsigned)std::min<int64_t>(nblk(P, 256), 1024)), dim3(256), 0,
+                     ctx->stream, vc, P, amax);
   PPF_HIP(hipGetLastError());
-  long long mi_t[3] = {-1, 0, 0};
-  PPF_HIP(hipMemcpyAsync(mi_t, am, 24, hipMemcpyDeviceToHost, ctx->stream));
+  unsigned long long mi_t[3] = {0, 0, 0};
+  PPF_HIP(hipMemcpyAsync(mi_t, ntest, 24, hipMemcpyDeviceToHost, ctx->stream));
   PPF_HIP(hipStreamSynchronize(ctx->stream));
-  *max_idx = mi_t[0];
-  *pair_tests = mi_t[1];
-  *mixed_chunks = mi_t[2];
+  *max_idx = (int64_t)(0xffffffffu - (uint32_t)(mi_t[2] & 0xffffffffull));
+  *pair_tests = (int64_t)mi_t[0];
+  *mixed_chunks = (int64_t)mi_t[1];
+  if (n_items > 0) {
+    float ms = 0.0f;
+    PPF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *kernel_ms = ms;
+  }
   return 0;
 }
 

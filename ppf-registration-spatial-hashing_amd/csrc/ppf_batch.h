@@ -29,8 +29,11 @@ struct BatchJob {
   int64_t index = 0;     // position in the caller's list
 };
 
+// Same initial capacity as a plain lookup: a job whose candidates overflow
+// is voted again (rerun launch), which costs far more than the 12 B per
+// candidate of a larger buffer.
 inline uint32_t batch_cap(int64_t R) {
-  return round_cap((uint64_t)std::max<int64_t>(1 << 20, R * (int64_t)CAND_PER_REF));
+  return round_cap((uint64_t)std::max<int64_t>(CAND_CAP_MIN, R * (int64_t)CAND_PER_REF));
 }
 
 // Vote every job in one persistent launch (more only for jobs whose candidate

@@ -230,7 +230,7 @@ int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const
 int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
-                    int64_t *max_idx, int64_t *pair_tests, int64_t *mixed_chunks);
+                    int64_t *max_idx, int64_t *pair_tests, int64_t *mixed_chunks, double *kernel_ms);
 int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *out, int64_t n);
 int debug_copy(ppf_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n);  // n multiple of 4
 

@@ -13,6 +13,8 @@
 //             ranks its 64-element slots with 8 ballots (lanes holding the same
 //             digit find each other), which keeps the sort stable; per-wave
 //             digit totals in LDS turn the ranks into global positions.
+#include <algorithm>
+
 #include "ppf_internal.h"
 
 namespace ppfi {
@@ -213,6 +215,165 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_down(const uint32_t *__restri
   }
 }
 
+// ---- one-sweep LSD radix sort: one histogram pass over the keys for every
+// digit, then one kernel per digit that ranks a tile, finds the tile's
+// global offsets by decoupled look-back over the preceding tiles' digit
+// counts, and writes the tile in digit order through LDS so that each digit's
+// run leaves the block as contiguous stores.
+constexpr int OS_BLOCK = 256;
+constexpr int OS_WAVES = OS_BLOCK / WAVE;
+constexpr int OS_ITEMS = 16;
+constexpr int OS_TILE = OS_BLOCK * OS_ITEMS;  // 4096 keys
+constexpr int OS_MAX_PASSES = 8;
+// look-back status word per (tile, digit): flag in bits 30-31, count below
+constexpr uint32_t OS_AGG = 1u << 30;      // the tile's own count of the digit
+constexpr uint32_t OS_INC = 2u << 30;      // inclusive prefix through this tile
+constexpr uint32_t OS_COUNT = (1u << 30) - 1u;
+// a wait longer than this many status reads (about a second) is reported as
+// an error instead of waiting on; every other waiting tile then stops too
+constexpr uint32_t OS_SPIN_LIMIT = 1u << 20;
+
+__global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict__ keys, int64_t n,
+                                                      int lo, int npass, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[OS_MAX_PASSES][256];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < OS_MAX_PASSES * 256; i += OS_BLOCK) (&h[0][0])[i] = 0u;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * OS_BLOCK + tid; i < n; i += (int64_t)gridDim.x * OS_BLOCK) {
+    const uint64_t k = keys[i];
+    for (int p = 0; p < npass; ++p) atomicAdd(&h[p][(uint32_t)(k >> (lo + 8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int i = tid; i < npass * 256; i += OS_BLOCK) {
+    const uint32_t c = (&h[0][0])[i];
+    if (c) atomicAdd(&hist[i], c);
+  }
+}
+
+// hist [npass][256] -> exclusive digit starts, in place; one block
+__global__ __launch_bounds__(256) void k_os_starts(uint32_t *__restrict__ hist, int npass) {
+  __shared__ uint32_t s[256 / WAVE + 1];
+  for (int p = 0; p < npass; ++p) {
+    uint32_t tot;
+    const uint32_t v = hist[p * 256 + threadIdx.x];
+    const uint32_t r = block_excl_scan(v, s, &tot);
+    hist[p * 256 + threadIdx.x] = r;
+  }
+}
+
+template <bool VALS>
+__global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
+    const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+    uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ starts,
+    uint32_t *__restrict__ status, uint32_t *__restrict__ counters) {
+  __shared__ uint64_t s_key[OS_TILE];
+  __shared__ uint32_t s_val[VALS ? OS_TILE : 1];
+  __shared__ uint32_t s_cnt[OS_WAVES][256];
+  __shared__ uint32_t s_off[256];   // tile-local start of each digit
+  __shared__ uint32_t s_glob[256];  // global position of tile-local position 0 of each digit
+  __shared__ uint32_t s_wsum[OS_WAVES + 1];
+  __shared__ uint32_t s_tile;
+  const int tid = threadIdx.x, w = tid / WAVE, l = tid % WAVE;
+  if (tid == 0) s_tile = atomicAdd(&counters[0], 1u);  // tiles in acquisition order
+#pragma unroll
+  for (int q = 0; q < OS_WAVES; ++q) s_cnt[q][tid] = 0u;
+  __syncthreads();
+  const uint32_t t = s_tile;
+  const int64_t base = (int64_t)t * OS_TILE;
+  const int64_t base_w = base + (int64_t)w * WAVE * OS_ITEMS;
+  const uint64_t lt = lanemask_lt();
+  uint64_t k[OS_ITEMS];
+  uint32_t v[OS_ITEMS];
+  uint32_t rank[OS_ITEMS];
+#pragma unroll
+  for (int i = 0; i < OS_ITEMS; ++i) {
+    const int64_t idx = min(base_w + (int64_t)i * WAVE + l, n - 1);
+    k[i] = kin[idx];
+    if (VALS) v[i] = vin[idx];
+  }
+  // stable rank inside the wave: slots in order, lanes in order
+#pragma unroll
+  for (int i = 0; i < OS_ITEMS; ++i) {
+    const bool valid = base_w + (int64_t)i * WAVE + l < n;
+    const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t before = (uint32_t)__popcll(peers & lt);
+    uint32_t prev = 0;
+    if (valid) prev = s_cnt[w][d];
+    rank[i] = prev + before;
+    if (valid && before == 0) s_cnt[w][d] = prev + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // thread d: the digit's count in the tile, its waves' offsets, its tile-local start
+  const int d = tid;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int q = 0; q < OS_WAVES; ++q) {
+    const uint32_t c = s_cnt[q][d];
+    s_cnt[q][d] = cnt;
+    cnt += c;
+  }
+  // publish the tile's count at once, so that later tiles can look back
+  uint32_t *st_row = status + (size_t)t * 256;
+  __hip_atomic_store(&st_row[d], (t == 0 ? OS_INC : OS_AGG) | cnt, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t tot;
+  const uint32_t off = block_excl_scan(cnt, s_wsum, &tot);
+  s_off[d] = off;
+  // decoupled look-back over the preceding tiles' counts of digit d
+  uint32_t excl = 0;
+  if (t > 0) {
+    int64_t j = (int64_t)t - 1;
+    uint32_t spins = 0;
+    while (j >= 0) {
+      const uint32_t sv = __hip_atomic_load(&status[(size_t)j * 256 + d], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t flag = sv & ~OS_COUNT;
+      if (flag == 0u) {
+        ++spins;
+        if (spins > OS_SPIN_LIMIT) {  // never expected: report instead of waiting forever
+          atomicOr(&counters[1], 1u);
+          break;
+        }
+        if ((spins & 1023u) == 0u &&
+            __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+          break;
+        continue;
+      }
+      excl += sv & OS_COUNT;
+      if (flag == OS_INC) break;
+      --j;
+    }
+    __hip_atomic_store(&st_row[d], OS_INC | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  s_glob[d] = starts[d] + excl - off;
+  __syncthreads();
+  // the tile in digit order, in LDS
+#pragma unroll
+  for (int i = 0; i < OS_ITEMS; ++i) {
+    if (base_w + (int64_t)i * WAVE + l < n) {
+      const uint32_t dd = (uint32_t)(k[i] >> shift) & 255u;
+      const uint32_t lp = s_off[dd] + s_cnt[w][dd] + rank[i];
+      s_key[lp] = k[i];
+      if (VALS) s_val[lp] = v[i];
+    }
+  }
+  __syncthreads();
+  const int nt = (int)min((int64_t)OS_TILE, n - base);
+  for (int i = tid; i < nt; i += OS_BLOCK) {
+    const uint64_t kk = s_key[i];
+    const uint32_t pos = s_glob[(uint32_t)(kk >> shift) & 255u] + (uint32_t)i;
+    kout[pos] = kk;
+    if (VALS) vout[pos] = s_val[i];
+  }
+}
+
 }  // namespace
 
 // Scan of n <= SC_CHUNK^2 elements using `sums` (>= nb entries) as scratch.
@@ -249,6 +410,55 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp
   return scan_level(in, out, n, tmp.as<uint32_t>(), st, total_out_dev);
 }
 
+// One-sweep form (n < 2^30): a histogram kernel, a start kernel, then one
+// kernel per 8-bit digit.
+static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
+                               SortScratch &s, hipStream_t st) {
+  const int npass = (bit_hi - bit_lo + 7) / 8;
+  const int64_t n_tiles = (n + OS_TILE - 1) / OS_TILE;
+  // [hist/starts: npass x 256][counters: 2 per pass][status: n_tiles x 256]
+  const size_t hist_words = (size_t)OS_MAX_PASSES * 256, ctr_words = 2 * OS_MAX_PASSES;
+  PPF_TRY(s.counts.ensure((hist_words + ctr_words + (size_t)n_tiles * 256) * 4));
+  PPF_TRY(s.tmp_keys.ensure((size_t)n * 8));
+  if (vals) PPF_TRY(s.tmp_vals.ensure((size_t)n * 4));
+  uint32_t *hist = s.counts.as<uint32_t>();
+  uint32_t *ctr = hist + hist_words;
+  uint32_t *status = ctr + ctr_words;
+  PPF_HIP(hipMemsetAsync(hist, 0, (hist_words + ctr_words) * 4, st));
+  const unsigned hist_blocks = (unsigned)std::min<int64_t>((n + OS_BLOCK * 16 - 1) / (OS_BLOCK * 16), 2048);
+  hipLaunchKernelGGL(k_os_hist, dim3(hist_blocks), dim3(OS_BLOCK), 0, st, keys, n, bit_lo, npass, hist);
+  hipLaunchKernelGGL(k_os_starts, dim3(1), dim3(256), 0, st, hist, npass);
+  uint64_t *ka = keys, *kb = s.tmp_keys.as<uint64_t>();
+  uint32_t *va = vals, *vb = vals ? s.tmp_vals.as<uint32_t>() : nullptr;
+  for (int p = 0; p < npass; ++p) {
+    PPF_HIP(hipMemsetAsync(status, 0, (size_t)n_tiles * 256 * 4, st));
+    const int shift = bit_lo + 8 * p;
+    if (vals)
+      hipLaunchKernelGGL(k_os_pass<true>, dim3((unsigned)n_tiles), dim3(OS_BLOCK), 0, st, ka, va, kb, vb, n,
+                         shift, hist + 256 * p, status, ctr + 2 * p);
+    else
+      hipLaunchKernelGGL(k_os_pass<false>, dim3((unsigned)n_tiles), dim3(OS_BLOCK), 0, st, ka, va, kb, vb,
+                         n, shift, hist + 256 * p, status, ctr + 2 * p);
+    PPF_HIP(hipGetLastError());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (npass & 1) {
+    PPF_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+    if (vals) PPF_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  // look-back time-out flags (never expected)
+  uint32_t h_ctr[2 * OS_MAX_PASSES];
+  PPF_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof h_ctr, hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipStreamSynchronize(st));
+  for (int p = 0; p < npass; ++p)
+    if (h_ctr[2 * p + 1]) {
+      set_error("radix_sort_u64: look-back wait exceeded its spin limit");
+      return 2;
+    }
+  return 0;
+}
+
 int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
                    SortScratch &s, hipStream_t st) {
   if (n <= 1 || bit_hi <= bit_lo) return 0;
@@ -256,6 +466,7 @@ int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bi
     set_error("radix_sort_u64: more than 2^32 elements");
     return 1;
   }
+  if (n < ((int64_t)1 << 30) && bit_hi - bit_lo <= 8 * OS_MAX_PASSES) return radix_sort_onesweep(keys, vals, n, bit_lo, bit_hi, s, st);
   const int64_t n_tiles = (n + RS_TILE - 1) / RS_TILE;
   const int G = (int)std::min<int64_t>(n_tiles, RS_MAX_BLOCKS);
   const int64_t tpb = (n_tiles + G - 1) / G;

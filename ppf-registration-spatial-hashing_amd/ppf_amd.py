@@ -60,7 +60,7 @@ class ResultInfo(C.Structure):
                 ("n_clusters", C.c_int64), ("vote_ms", C.c_double), ("total_ms", C.c_double),
                 ("select_ms", C.c_double), ("pose_ms", C.c_double), ("cluster_ms", C.c_double),
                 ("cluster_tests", C.c_int64), ("vote_rows_ungrouped", C.c_int64),
-                ("cluster_mixed_chunks", C.c_int64)]
+                ("cluster_mixed_chunks", C.c_int64), ("cluster_kernel_ms", C.c_double)]
 
 
 class BatchInfo(C.Structure):

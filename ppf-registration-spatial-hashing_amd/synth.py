@@ -244,6 +244,7 @@ class Workload:
     truths: List[List[np.ndarray]]  # [scene][model] 4x4 ground truth (identity if absent)
     scene_models: List[List[int]]   # model ids present in each scene
     seed: int
+    shapes: list | None = None      # the models' superquadrics, when generated from them
 
 
 def plain_small(seed: int = 1, n_model: int = 2000, n_clutter: int = 3000) -> Workload:
@@ -373,7 +374,32 @@ def batched(seed: int = 4, n_models: int = 16, n_scenes: int = 64,
         scenes.append(sc)
         truths.append(tr)
         scene_models.append(ids)
-    return Workload("batched", models, scenes, d_dists, 5, 0.3, False, truths, scene_models, seed)
+    return Workload("batched", models, scenes, d_dists, 5, 0.3, False, truths, scene_models, seed,
+                    shapes=sqs)
+
+
+def served_requests(w: Workload, seed: int = 5, n_requests: int = 32, scene_range=(5000, 100000),
+                    targets=(1, 8), keep=None):
+    """Requests of the served configuration over the resident models of `w`
+    (a `batched` workload): scenes of log-uniform size in scene_range, each
+    built like the dataset-like scene (the target models on a ground plane,
+    30% occluded, 15% plane points, noise 0.003 D) with 1-8 target model ids.
+    keep(n_scene, model_id) -> bool, when given, drops the targets it rejects
+    (a request left without targets is drawn again).  Returns [(scene cloud,
+    target ids)]."""
+    rng = _rng(seed)
+    out = []
+    while len(out) < n_requests:
+        n_pts = _log_uniform_int(rng, *scene_range, 1)[0]
+        k = int(rng.integers(targets[0], targets[1] + 1))
+        ids = [int(x) for x in rng.choice(len(w.models), size=k, replace=False)]
+        if keep is not None:
+            ids = [i for i in ids if keep(n_pts, i)]
+            if not ids:
+                continue
+        sc, _ = _place_on_plane(w.shapes, w.models, n_pts, 0.15, 0.003, 0.3, rng, ids)
+        out.append((sc, ids))
+    return out
 
 
 WORKLOADS = {

@@ -405,6 +405,19 @@ def test_device_radix_sort_is_stable(ctx, case):
         assert np.array_equal(sv, v[order]) and np.array_equal(sk, k[order])
 
 
+@pytest.mark.parametrize("n", [2, 4095, 4096, 4097, 1 << 21])
+def test_device_radix_sort_sizes_keys_only(ctx, n):
+    """Tile edges of the one-sweep sort (4,096 keys per tile), keys alone."""
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 2**40, n, dtype=np.uint64)
+    k[::3] = k[0]  # many equal keys
+    sk, _ = ctx.debug_sort(k, None, 0, 40)
+    assert np.array_equal(sk, np.sort(k, kind="stable"))
+    sk, _ = ctx.debug_sort(k, None, 16, 32)
+    digit = (k >> np.uint64(16)) & np.uint64(0xffff)
+    assert np.array_equal(sk, k[np.argsort(digit, kind="stable")])
+
+
 def test_device_copy_kernel(ctx):
     a = np.random.default_rng(0).integers(0, 2**32, 1 << 20, dtype=np.uint64).astype(np.uint32)
     assert np.array_equal(ctx.debug_copy(a), a)
