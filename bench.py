@@ -228,10 +228,15 @@ def run_large_hash(ctx, rpdf=100):
     info = m.info()
     log(f"large-hash model built: {builds[-1]:.0f} ms")
     scene = ppf_amd.Scene(w.scenes[0], 0.0, rpdf, ctx=ctx)
+    if rpdf < 50:  # minutes of voting: report each launch of 4,000 rows
+        t_prog = time.perf_counter()
+        ctx.set_progress(lambda st, done, tot: log(f"large-hash {st}: {done}/{tot} "
+                                                   f"({time.perf_counter() - t_prog:.0f}s)"), 4000)
     t = time.perf_counter()
     r = m.ppf_lookup(scene)
     r.pose()
     dt = time.perf_counter() - t
+    ctx.set_progress(None)
     i = dict(r.info)
     r.close()
     scene.close()
@@ -244,7 +249,9 @@ def run_large_hash(ctx, rpdf=100):
             "lookups": i["lookups"], "votes": i["votes"], "vote_ms": i["vote_ms"],
             "lookups_per_s": i["lookups"] / (i["vote_ms"] * 1e-3),
             "votes_per_s": i["votes"] / (i["vote_ms"] * 1e-3), "n_kept": i["n_kept"],
-            "cluster_ms": i["cluster_ms"], "lookup_s": dt}
+            "select_ms": i["select_ms"], "pose_ms": i["pose_ms"], "cluster_ms": i["cluster_ms"],
+            "cluster_kernel_ms": i["cluster_kernel_ms"], "lookup_s": dt,
+            "registrations_per_s": 1.0 / dt, "max_idx": i["max_idx"]}
 
 
 def cpu_info():

@@ -54,6 +54,17 @@ int ppf_ctx_destroy(ppf_ctx ctx);
 /* Bytes of device memory currently held by objects of this context. */
 uint64_t ppf_ctx_device_bytes(ppf_ctx ctx);
 
+/* Progress reports of long lookups (the reference's DEBUG stage timers,
+ * src/cuda/scene.cu:72-98 and model.cu:271-305, as a callback).  With a
+ * callback set, ppf_lookup on this context votes in launches of
+ * rows_per_launch reference rows (0: all rows in one launch) and calls
+ * fn(user, stage, done, total) after each launch ("vote": rows voted of the
+ * scene's reference rows) and after each later stage ("select", "poses",
+ * "cluster": done = total = kept votes).  Results do not depend on it.  fn
+ * runs on the calling thread; NULL turns reports off. */
+typedef void (*ppf_progress_fn)(void *user, const char *stage, int64_t done, int64_t total);
+int ppf_ctx_set_progress(ppf_ctx ctx, ppf_progress_fn fn, void *user, int64_t rows_per_launch);
+
 /* -------------------------------------------------------------------- model
  * Model::Model(cloud, d_dist, ...) (include/model.h:17-19, src/cuda/model.cu:43-82):
  * PPF of every ordered model pair, quantized and FNV-hashed, then the sort-based

@@ -444,6 +444,9 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
   TRY(ppfk::select_and_sort(ctx, ccode, ccount, n_cand, max_count, p->vote_count_threshold,
                             s->cloud.n, m->cloud.n, scode, scount, &P));
   HIPC(hipEventRecord(ctx->ev5, st));
+  // progress reports of plain lookups only (lanes finish batched jobs concurrently)
+  const ppf_progress_fn prog = ctx == ctx->root ? ctx->root->progress : nullptr;
+  if (prog) prog(ctx->root->progress_user, "select", P, P);
   r->P = P;
   r->info.n_kept = P;
   if (P > 0) {
@@ -455,6 +458,10 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
     TRY(ppfk::compute_poses(ctx, r->codes.as<unsigned long long>(), P, m->frames.as<float>(),
                             s->frames.as<float>(), s->rpdf, r->T.as<float>()));
     HIPC(hipEventRecord(ctx->ev6, st));
+    if (prog) {
+      HIPC(hipEventSynchronize(ctx->ev6));
+      prog(ctx->root->progress_user, "poses", P, P);
+    }
     if (p->cpu_clustering) {
       std::vector<float> T((size_t)P * 16);
       std::vector<uint32_t> counts((size_t)P);
@@ -486,6 +493,7 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
       r->pose[11] = t3[2];
     }
     r->info.status = PPF_OK;
+    if (prog) prog(ctx->root->progress_user, "cluster", P, P);
   } else {
     r->info.status = PPF_ERR_NO_VOTES;
   }
@@ -755,6 +763,15 @@ int ppf_ctx_destroy(ppf_ctx ctx) {
 
 uint64_t ppf_ctx_device_bytes(ppf_ctx ctx) { return ctx ? ctx->root->bytes.load() : 0; }
 
+int ppf_ctx_set_progress(ppf_ctx ctx, ppf_progress_fn fn, void *user, int64_t rows_per_launch) {
+  ARG_CHECK(ctx, "ppf_ctx_set_progress: NULL context");
+  ARG_CHECK(rows_per_launch >= 0, "ppf_ctx_set_progress: negative rows_per_launch");
+  ctx->root->progress = fn;
+  ctx->root->progress_user = user;
+  ctx->root->progress_rows = rows_per_launch;
+  return PPF_OK;
+}
+
 // ------------------------------------------------------------------ model
 int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int64_t n,
                     float d_dist, ppf_model *out) {
@@ -943,10 +960,19 @@ int lookup_core(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *
     ppfk::VoteJob J = ppfi::vote_job(m, s, p->vote_count_threshold, cc, cn, meta, cap);
     J.per_ref = p->per_reference_argmax != 0;
     J.ungrouped = meta + 4;
-    float ms = 0.0f;
-    TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)s->R, m->TM, s->cloud.n, p->exact_votes != 0, &ms,
-                        max_blocks));
-    vote_ms += ms;
+    // with progress reports, the rows in launches of progress_rows (the
+    // running maximum and the candidates carry over between launches)
+    const ppf_ctx_s *root = ctx->root;
+    const int64_t step = root->progress && root->progress_rows > 0 ? root->progress_rows : s->R;
+    for (int64_t r0 = 0; r0 < s->R; r0 += step) {
+      const int64_t nr = std::min<int64_t>(step, s->R - r0);
+      J.r_begin = (int32_t)r0;
+      float ms = 0.0f;
+      TRY(ppfk::run_votes(ctx, &J, 1, (uint64_t)nr, m->TM, s->cloud.n, p->exact_votes != 0, &ms,
+                          max_blocks));
+      vote_ms += ms;
+      if (root->progress) root->progress(root->progress_user, "vote", r0 + nr, s->R);
+    }
     uint32_t mh[8];
     HIPC(hipMemcpyAsync(mh, meta, 32, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));

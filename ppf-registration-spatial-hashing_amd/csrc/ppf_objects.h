@@ -48,6 +48,10 @@ struct ppf_ctx_s {
   // reference row may group before it is voted hit by hit (0 = the LDS
   // grouping table's size)
   uint32_t dbg_group_cap = 0;
+  // root only: progress reports of plain lookups (ppf_ctx_set_progress)
+  ppf_progress_fn progress = nullptr;
+  void *progress_user = nullptr;
+  int64_t progress_rows = 0;
 
   // Account n bytes against the root's budget (false: over budget).
   bool reserve(uint64_t n) {

@@ -94,6 +94,9 @@ _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 
 
+_PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p, C.c_int64, C.c_int64)
+
+
 def lib():
     """Load libppf_amd.so (raises if it has not been built)."""
     global _lib
@@ -107,6 +110,7 @@ def lib():
     L.ppf_ctx_destroy.argtypes = [_vp]
     L.ppf_ctx_device_bytes.argtypes = [_vp]
     L.ppf_ctx_device_bytes.restype = C.c_uint64
+    L.ppf_ctx_set_progress.argtypes = [_vp, _PROGRESS_FN, C.c_void_p, C.c_int64]
     L.ppf_model_build.argtypes = [_vp, _f32p, _f32p, C.c_int64, C.c_float, C.POINTER(_vp)]
     L.ppf_model_destroy.argtypes = [_vp]
     L.ppf_model_info.argtypes = [_vp, C.POINTER(ModelInfo)]
@@ -258,6 +262,18 @@ class Context:
         out = np.zeros_like(x)
         _check(lib().ppf_debug_primitive(self.handle, 2, x.ctypes.data, out.ctypes.data, len(x), 0, 0))
         return out
+
+    def set_progress(self, fn=None, rows_per_launch: int = 0):
+        """ppf_ctx_set_progress: fn(stage, done, total) after each vote launch
+        of rows_per_launch reference rows and after each later stage of a
+        plain lookup; None turns reports off."""
+        if fn is None:
+            self._progress = None
+            _check(lib().ppf_ctx_set_progress(self.handle, _PROGRESS_FN(), None, 0))
+            return
+        cb = _PROGRESS_FN(lambda _u, stage, done, total: fn(stage.decode(), int(done), int(total)))
+        self._progress = cb  # keep the C callback alive
+        _check(lib().ppf_ctx_set_progress(self.handle, cb, None, int(rows_per_launch)))
 
     def debug_set(self, knob: int, value: int):
         """ppf_debug_set: knob 0 = distinct buckets a reference row may group

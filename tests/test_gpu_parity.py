@@ -460,3 +460,29 @@ def test_candidate_overflow_reruns_give_same_result(ctx):
         srv.close()
     for o in (small, ref, m, s):
         o.close()
+
+
+def test_progress_reports_and_sliced_votes_give_same_result():
+    """ppf_ctx_set_progress: voting in launches of 37 reference rows (running
+    maximum and candidates carried over) gives the one-launch result bit for
+    bit, with a report per launch and per later stage."""
+    w = synth.plain_small(seed=15, n_model=500, n_clutter=900)
+    c = ppf_amd.Context(0)
+    try:
+        m = ppf_amd.Model(w.models[0], w.d_dists[0], 0.3, ctx=c)
+        s = ppf_amd.Scene(w.scenes[0], 0.0, w.rpdf, ctx=c)
+        ref = m.ppf_lookup(s)
+        rc, rn = ref.votes()
+        seen = []
+        c.set_progress(lambda stage, done, total: seen.append((stage, done, total)), rows_per_launch=37)
+        res = m.ppf_lookup(s)
+        c.set_progress(None)
+        codes, counts = res.votes()
+        assert np.array_equal(codes, rc) and np.array_equal(counts, rn)
+        assert res.pose().tobytes() == ref.pose().tobytes()
+        R = (len(w.scenes[0][0]) + w.rpdf - 1) // w.rpdf
+        votes = [d for st, d, t in seen if st == "vote"]
+        assert votes == list(range(37, R, 37)) + [R]
+        assert [st for st, _, _ in seen if st != "vote"] == ["select", "poses", "cluster"]
+    finally:
+        c.close()
