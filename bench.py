@@ -373,6 +373,10 @@ def main():
     used = sorted({si for si, _ in wide})
     scenes = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in used}
 
+    # the project's autotuner picks the batched form's launch shapes on this batch
+    tuned = ppf_amd.batch_tune([models[mi] for _, mi in jobs], [scenes[si] for si, _ in jobs],
+                               w.vote_count_threshold, ctx=ctx)
+    log(f"autotuned in {tuned['tune_ms'] / 1e3:.1f}s: {tuned}")
     for i in range(args.warmup):
         run_batched(ctx, models, scenes, jobs, w.vote_count_threshold)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -425,11 +429,14 @@ def main():
             "model_points": [len(w.models[mi][0]) for _, mi in jobs],
             "ref_point_df": w.rpdf,
             "model_build_s_all_16": build_s,
+            "autotune": tuned,
         },
         "pair_lookups_per_s": lookups / elapsed,
         "votes_per_s": votes / elapsed,
         "latency_ms": {"p50": pct(step_s, 50) * 1e3, "p99": pct(step_s, 99) * 1e3},
         "vote_kernel_ms_per_step": vote_ms / args.steps,
+        "batched_split_ms_per_step": {"vote": vote_ms / args.steps,
+                                      "finish_wall": sum(i["finish_ms"] for i in step_info) / args.steps},
         "roofline": roofline,
     }
     if args.headline_only:

@@ -274,6 +274,27 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
                      const ppf_lookup_params *params, ppf_result *results_out,
                      ppf_batch_info_t *info);
 
+/* Autotuner of the batched form (no counterpart in the reference, whose
+ * launch shapes are constants, include/kernel.h:9-10).  Times candidate
+ * launch shapes on the given batch -- vote blocks per CU, finishing lanes,
+ * one-sweep radix-sort tile -- and keeps the fastest on the context for
+ * every later ppf_batch_lookup and server round.  Costs about three vote
+ * passes and three finishing passes of the batch; results of every call are
+ * identical whatever the shapes.  ppf_tune_reset restores the defaults. */
+typedef struct {
+  int32_t vote_blocks_per_cu; /* persistent vote blocks per compute unit */
+  int32_t finish_lanes;       /* host threads + streams finishing jobs */
+  int32_t sort_tile_keys;     /* keys per one-sweep radix-sort tile */
+  int32_t pad;
+  double best_vote_ms;        /* vote pass of the batch with the chosen shape */
+  double best_finish_ms;      /* finishing of the batch with the chosen lanes */
+  double best_sort_ms;        /* the tile's timed sorts */
+  double tune_ms;             /* wall time of the call */
+} ppf_tune_info_t;
+int ppf_batch_tune(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const ppf_scene *scenes,
+                   const ppf_lookup_params *params, ppf_tune_info_t *info);
+int ppf_tune_reset(ppf_ctx ctx);
+
 /* ------------------------------------------------------------------ served
  * A request loop over resident models (no counterpart in the reference, whose
  * driver registers one pair at a time, src/cuda/ppf.cu:63-98).  A request is
@@ -331,9 +352,9 @@ int ppf_server_stats(ppf_server server, ppf_server_stats_t *stats);
 
 /* Sizes of the ABI structs, in this order: ppf_model_info_t,
  * ppf_lookup_params, ppf_result_info_t, ppf_batch_info_t, ppf_server_config,
- * ppf_request_info_t, ppf_server_stats_t.  Lets a binding check that it was
+ * ppf_request_info_t, ppf_server_stats_t, ppf_tune_info_t.  Lets a binding check that it was
  * written against the same header. */
-#define PPF_ABI_N_STRUCTS 7
+#define PPF_ABI_N_STRUCTS 8
 void ppf_abi_sizes(uint32_t sizes_out[PPF_ABI_N_STRUCTS]);
 
 /* ------------------------------------------------------------- diagnostics
@@ -357,8 +378,9 @@ int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_i
 /* Diagnostic knobs of a context (for tests).  knob 0: the number of distinct
  * model buckets a reference row may group before the vote kernel votes that
  * row hit by hit (the path a row takes when its hits span more buckets than
- * the kernel's grouping table holds); 0 restores the table's size.  Results
- * do not depend on it. */
+ * the kernel's grouping table holds); 0 restores the table's size.  knob 1:
+ * keys per thread of the one-sweep radix sort's tiles (8 or 16; the
+ * autotuner's choice otherwise).  Results do not depend on either. */
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value);
 
 /* GPU spatial-hash clustering (Model::ClusterTransformations + max_element,

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libppf_amd.so")
-SOURCES = ["api.hip", "batch.hip", "server.hip", "preprocess.hip", "host_io.hip", "kernels.hip", "radix_sort.hip"]
+SOURCES = ["api.hip", "batch.hip", "server.hip", "preprocess.hip", "host_io.hip", "kernels.hip", "radix_sort.hip", "tune.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -161,6 +161,7 @@ int ctx_lanes(ppf_ctx_s *root, int n, std::vector<ppf_ctx_s *> &out) {
       delete l;
       return rc;
     }
+    l->sort.tile_items = root->sort.tile_items;
     root->lanes.push_back(l);
   }
   out.assign(root->lanes.begin(), root->lanes.begin() + n);
@@ -732,6 +733,7 @@ void ppf_abi_sizes(uint32_t sizes_out[PPF_ABI_N_STRUCTS]) {
   sizes_out[4] = (uint32_t)sizeof(ppf_server_config);
   sizes_out[5] = (uint32_t)sizeof(ppf_request_info_t);
   sizes_out[6] = (uint32_t)sizeof(ppf_server_stats_t);
+  sizes_out[7] = (uint32_t)sizeof(ppf_tune_info_t);
 }
 
 int ppf_ctx_create(int device, uint64_t hbm_budget_bytes, ppf_ctx *out) {
@@ -1207,7 +1209,13 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
 
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value) {
   ARG_CHECK(ctx, "ppf_debug_set: NULL context");
-  ARG_CHECK(knob == 0, "ppf_debug_set: unknown knob");
+  ARG_CHECK(knob == 0 || knob == 1, "ppf_debug_set: unknown knob");
+  if (knob == 1) {
+    ARG_CHECK(value == 8 || value == 16, "ppf_debug_set: sort tile keys per thread must be 8 or 16");
+    ctx->root->sort.tile_items = (int)value;
+    for (ppf_ctx_s *l : ctx->root->lanes) l->sort.tile_items = (int)value;
+    return PPF_OK;
+  }
   ARG_CHECK(value >= 0 && value <= 0xffffffffll, "ppf_debug_set: value out of range");
   ctx->root->dbg_group_cap = (uint32_t)value;
   return PPF_OK;

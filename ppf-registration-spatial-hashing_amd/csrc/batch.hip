@@ -94,6 +94,31 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
   return 0;
 }
 
+void make_batch_jobs(int32_t n_jobs, const ppf_model *models, const ppf_scene *scenes,
+                     const ppf_lookup_params *params, std::vector<size_t> &order, std::vector<BatchJob> &q) {
+  order.resize((size_t)n_jobs);
+  std::iota(order.begin(), order.end(), (size_t)0);
+  // longest jobs first in the item queue (work ~ R * n_s * n_m)
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    auto w = [&](size_t j) {
+      return (double)scenes[j]->R * (double)scenes[j]->cloud.n * (double)models[j]->cloud.n;
+    };
+    return w(a) > w(b);
+  });
+  q.clear();
+  q.reserve((size_t)n_jobs);
+  for (size_t j : order) {
+    BatchJob b;
+    b.m = models[j];
+    b.s = scenes[j];
+    b.thr = params->vote_count_threshold;
+    b.per_ref = params->per_reference_argmax != 0;
+    b.cap = params->candidate_capacity ? round_cap(params->candidate_capacity) : batch_cap(scenes[j]->R);
+    b.index = (int64_t)j;
+    q.push_back(std::move(b));
+  }
+}
+
 int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,
                  std::vector<std::unique_ptr<ppf_result_s>> &results) {
   const size_t nj = jobs.size();
@@ -170,33 +195,16 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
   }
   PPF_HIP(hipSetDevice(root->device));
   const auto t0 = std::chrono::steady_clock::now();
-  std::vector<size_t> order((size_t)n_jobs);
-  std::iota(order.begin(), order.end(), (size_t)0);
-  // longest jobs first in the item queue (work ~ R * n_s * n_m)
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    auto w = [&](size_t j) {
-      return (double)scenes[j]->R * (double)scenes[j]->cloud.n * (double)models[j]->cloud.n;
-    };
-    return w(a) > w(b);
-  });
+  std::vector<size_t> order;
   std::vector<BatchJob> q;
-  q.reserve((size_t)n_jobs);
-  for (size_t j : order) {
-    BatchJob b;
-    b.m = models[j];
-    b.s = scenes[j];
-    b.thr = params->vote_count_threshold;
-    b.per_ref = params->per_reference_argmax != 0;
-    b.cap = params->candidate_capacity ? round_cap(params->candidate_capacity) : batch_cap(scenes[j]->R);
-    b.index = (int64_t)j;
-    q.push_back(std::move(b));
-  }
+  make_batch_jobs(n_jobs, models, scenes, params, order, q);
   float vote_ms = 0.0f;
   int launches = 0;
   PPF_TRY(batch_votes(root, q, params->exact_votes != 0, &vote_ms, &launches));
   std::vector<std::unique_ptr<ppf_result_s>> res;
   const auto t1 = std::chrono::steady_clock::now();
-  PPF_TRY(batch_finish(root, q, params, PPF_BATCH_LANES, res));
+  const int lanes = root->tune_lanes > 0 ? root->tune_lanes : PPF_BATCH_LANES;
+  PPF_TRY(batch_finish(root, q, params, lanes, res));
   const auto t2 = std::chrono::steady_clock::now();
   int64_t L = 0, V = 0;
   for (size_t k = 0; k < order.size(); ++k) {
@@ -211,7 +219,7 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
     info->lookups = L;
     info->votes = V;
     info->vote_launches = launches;
-    info->lanes = std::min<int>(PPF_BATCH_LANES, n_jobs);
+    info->lanes = std::min<int>(lanes, n_jobs);
     info->vote_ms = vote_ms;
     info->finish_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     info->total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();

@@ -1604,7 +1604,9 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
   PPF_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;
   PPF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, VOTE_BLOCK, lds));
-  int G = ctx->n_cu * std::max(1, per_cu);
+  per_cu = std::max(1, per_cu);
+  if (ctx->root->tune_vote_per_cu > 0) per_cu = std::min(per_cu, ctx->root->tune_vote_per_cu);
+  int G = ctx->n_cu * per_cu;
   if (max_blocks > 0) G = std::min(G, max_blocks);
   PPF_TRY(ctx->tmp_a.ensure(sizeof(VoteJob) * n_jobs));
   PPF_TRY(ctx->queue.ensure(16));

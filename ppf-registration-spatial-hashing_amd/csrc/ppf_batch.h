@@ -36,6 +36,9 @@ inline uint32_t batch_cap(int64_t R) {
   return round_cap((uint64_t)std::max<int64_t>(CAND_CAP_MIN, R * (int64_t)CAND_PER_REF));
 }
 
+// The jobs of a batch, longest first (order[k] = caller's index of q[k]).
+void make_batch_jobs(int32_t n_jobs, const ppf_model *models, const ppf_scene *scenes,
+                     const ppf_lookup_params *params, std::vector<size_t> &order, std::vector<BatchJob> &q);
 // Vote every job in one persistent launch (more only for jobs whose candidate
 // buffer overflowed, rerun with their final maximum preset).
 int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vote_ms, int *launches);

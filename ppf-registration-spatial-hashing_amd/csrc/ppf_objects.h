@@ -48,6 +48,10 @@ struct ppf_ctx_s {
   // reference row may group before it is voted hit by hit (0 = the LDS
   // grouping table's size)
   uint32_t dbg_group_cap = 0;
+  // root only: the batched form's tuned launch shape (ppf_batch_tune; 0 =
+  // default): vote blocks per CU, finishing lanes
+  int32_t tune_vote_per_cu = 0;
+  int32_t tune_lanes = 0;
   // root only: progress reports of plain lookups (ppf_ctx_set_progress)
   ppf_progress_fn progress = nullptr;
   void *progress_user = nullptr;

@@ -222,8 +222,6 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_down(const uint32_t *__restri
 // run leaves the block as contiguous stores.
 constexpr int OS_BLOCK = 256;
 constexpr int OS_WAVES = OS_BLOCK / WAVE;
-constexpr int OS_ITEMS = 16;
-constexpr int OS_TILE = OS_BLOCK * OS_ITEMS;  // 4096 keys
 constexpr int OS_MAX_PASSES = 8;
 // look-back status word per (tile, digit): flag in bits 30-31, count below
 constexpr uint32_t OS_AGG = 1u << 30;      // the tile's own count of the digit
@@ -261,13 +259,15 @@ __global__ __launch_bounds__(256) void k_os_starts(uint32_t *__restrict__ hist, 
   }
 }
 
-template <bool VALS>
+// ITEMS keys per thread: a tile of 256 * ITEMS keys (the tuner picks 8 or 16)
+template <bool VALS, int ITEMS>
 __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ starts,
     uint32_t *__restrict__ status, uint32_t *__restrict__ counters) {
-  __shared__ uint64_t s_key[OS_TILE];
-  __shared__ uint32_t s_val[VALS ? OS_TILE : 1];
+  constexpr int OS_TILE_T = OS_BLOCK * ITEMS;
+  __shared__ uint64_t s_key[OS_TILE_T];
+  __shared__ uint32_t s_val[VALS ? OS_TILE_T : 1];
   __shared__ uint32_t s_cnt[OS_WAVES][256];
   __shared__ uint32_t s_off[256];   // tile-local start of each digit
   __shared__ uint32_t s_glob[256];  // global position of tile-local position 0 of each digit
@@ -279,21 +279,21 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
   for (int q = 0; q < OS_WAVES; ++q) s_cnt[q][tid] = 0u;
   __syncthreads();
   const uint32_t t = s_tile;
-  const int64_t base = (int64_t)t * OS_TILE;
-  const int64_t base_w = base + (int64_t)w * WAVE * OS_ITEMS;
+  const int64_t base = (int64_t)t * OS_TILE_T;
+  const int64_t base_w = base + (int64_t)w * WAVE * ITEMS;
   const uint64_t lt = lanemask_lt();
-  uint64_t k[OS_ITEMS];
-  uint32_t v[OS_ITEMS];
-  uint32_t rank[OS_ITEMS];
+  uint64_t k[ITEMS];
+  uint32_t v[ITEMS];
+  uint32_t rank[ITEMS];
 #pragma unroll
-  for (int i = 0; i < OS_ITEMS; ++i) {
+  for (int i = 0; i < ITEMS; ++i) {
     const int64_t idx = min(base_w + (int64_t)i * WAVE + l, n - 1);
     k[i] = kin[idx];
     if (VALS) v[i] = vin[idx];
   }
   // stable rank inside the wave: slots in order, lanes in order
 #pragma unroll
-  for (int i = 0; i < OS_ITEMS; ++i) {
+  for (int i = 0; i < ITEMS; ++i) {
     const bool valid = base_w + (int64_t)i * WAVE + l < n;
     const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
     uint64_t peers = __ballot(valid);
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
   __syncthreads();
   // the tile in digit order, in LDS
 #pragma unroll
-  for (int i = 0; i < OS_ITEMS; ++i) {
+  for (int i = 0; i < ITEMS; ++i) {
     if (base_w + (int64_t)i * WAVE + l < n) {
       const uint32_t dd = (uint32_t)(k[i] >> shift) & 255u;
       const uint32_t lp = s_off[dd] + s_cnt[w][dd] + rank[i];
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
     }
   }
   __syncthreads();
-  const int nt = (int)min((int64_t)OS_TILE, n - base);
+  const int nt = (int)min((int64_t)OS_TILE_T, n - base);
   for (int i = tid; i < nt; i += OS_BLOCK) {
     const uint64_t kk = s_key[i];
     const uint32_t pos = s_glob[(uint32_t)(kk >> shift) & 255u] + (uint32_t)i;
@@ -415,7 +415,9 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp
 static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
                                SortScratch &s, hipStream_t st) {
   const int npass = (bit_hi - bit_lo + 7) / 8;
-  const int64_t n_tiles = (n + OS_TILE - 1) / OS_TILE;
+  const int items = s.tile_items == 8 ? 8 : 16;
+  const int64_t tile = (int64_t)OS_BLOCK * items;
+  const int64_t n_tiles = (n + tile - 1) / tile;
   // [hist/starts: npass x 256][counters: 2 per pass][status: n_tiles x 256]
   const size_t hist_words = (size_t)OS_MAX_PASSES * 256, ctr_words = 2 * OS_MAX_PASSES;
   PPF_TRY(s.counts.ensure((hist_words + ctr_words + (size_t)n_tiles * 256) * 4));
@@ -433,12 +435,10 @@ static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bi
   for (int p = 0; p < npass; ++p) {
     PPF_HIP(hipMemsetAsync(status, 0, (size_t)n_tiles * 256 * 4, st));
     const int shift = bit_lo + 8 * p;
-    if (vals)
-      hipLaunchKernelGGL(k_os_pass<true>, dim3((unsigned)n_tiles), dim3(OS_BLOCK), 0, st, ka, va, kb, vb, n,
-                         shift, hist + 256 * p, status, ctr + 2 * p);
-    else
-      hipLaunchKernelGGL(k_os_pass<false>, dim3((unsigned)n_tiles), dim3(OS_BLOCK), 0, st, ka, va, kb, vb,
-                         n, shift, hist + 256 * p, status, ctr + 2 * p);
+    auto kern = vals ? (items == 8 ? k_os_pass<true, 8> : k_os_pass<true, 16>)
+                     : (items == 8 ? k_os_pass<false, 8> : k_os_pass<false, 16>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)n_tiles), dim3(OS_BLOCK), 0, st, ka, va, kb, vb, n, shift,
+                       hist + 256 * p, status, ctr + 2 * p);
     PPF_HIP(hipGetLastError());
     std::swap(ka, kb);
     std::swap(va, vb);

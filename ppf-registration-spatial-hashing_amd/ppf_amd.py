@@ -69,6 +69,12 @@ class BatchInfo(C.Structure):
                 ("finish_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class TuneInfo(C.Structure):
+    _fields_ = [("vote_blocks_per_cu", C.c_int32), ("finish_lanes", C.c_int32),
+                ("sort_tile_keys", C.c_int32), ("pad", C.c_int32), ("best_vote_ms", C.c_double),
+                ("best_finish_ms", C.c_double), ("best_sort_ms", C.c_double), ("tune_ms", C.c_double)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_int32), ("lanes", C.c_int32),
                 ("round_ms", C.c_double)]
@@ -142,6 +148,8 @@ def lib():
     L.ppf_ht_dist.argtypes = [_f32p, _f32p, _f32p]
     L.ppf_batch_lookup.argtypes = [_vp, C.c_int32, _vp, _vp, C.POINTER(LookupParams), _vp,
                                    C.POINTER(BatchInfo)]
+    L.ppf_batch_tune.argtypes = [_vp, C.c_int32, _vp, _vp, C.POINTER(LookupParams), C.POINTER(TuneInfo)]
+    L.ppf_tune_reset.argtypes = [_vp]
     L.ppf_server_create.argtypes = [_vp, C.c_int32, _vp, C.POINTER(ServerConfig), C.POINTER(_vp)]
     L.ppf_server_destroy.argtypes = [_vp]
     L.ppf_server_submit.argtypes = [_vp, _f32p, _f32p, C.c_int64, C.c_uint32, C.c_int32, _vp,
@@ -155,10 +163,10 @@ def lib():
     L.ppf_debug_set.argtypes = [_vp, C.c_int32, C.c_int64]
     L.ppf_debug_cluster.argtypes = [_vp, _f32p, _vp, _vp, _vp, C.c_int64, C.c_int64, C.c_float,
                                     C.c_int32, C.c_int32, _f32p, _f32p, _f32p, _vp]
-    sizes = (C.c_uint32 * 7)()
+    sizes = (C.c_uint32 * 8)()
     L.ppf_abi_sizes(sizes)
     want = tuple(C.sizeof(t) for t in (ModelInfo, LookupParams, ResultInfo, BatchInfo, ServerConfig,
-                                       RequestInfo, ServerStats))
+                                       RequestInfo, ServerStats, TuneInfo))
     if tuple(sizes) != want:
         raise PPFError(PPF_ERR_STATE, f"{LIB_PATH} was built from a different include/ppf_amd.h "
                                       f"(struct sizes {tuple(sizes)} != {want}): rebuild it")
@@ -587,6 +595,24 @@ def batch_lookup(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_
     _check(lib().ppf_batch_lookup(ctx.handle, n, mh, sh, C.byref(prm), out, C.byref(info)))
     res = [LookupResult(_vp(out[j]), bool(cpu_clustering), ctx) for j in range(n)]
     return res, {k: getattr(info, k) for k, _ in BatchInfo._fields_}
+
+
+def batch_tune(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_threshold: float,
+               ctx: Optional[Context] = None) -> dict:
+    """Autotuner of the batched form (ppf_batch_tune): times launch shapes on
+    this batch and keeps the fastest on the context; returns the choice."""
+    ctx = ctx or models[0].ctx
+    n = len(models)
+    mh = (C.c_void_p * n)(*[m.handle.value for m in models])
+    sh = (C.c_void_p * n)(*[s.handle.value for s in scenes])
+    prm = LookupParams(float(vote_count_threshold), 0, 0, 0, 0, 0, 0)
+    info = TuneInfo()
+    _check(lib().ppf_batch_tune(ctx.handle, n, mh, sh, C.byref(prm), C.byref(info)))
+    return {k: getattr(info, k) for k, _ in TuneInfo._fields_ if k != "pad"}
+
+
+def tune_reset(ctx: Context):
+    _check(lib().ppf_tune_reset(ctx.handle))
 
 
 NOALLOC_MAX_SCENE_POINTS = 8192

@@ -121,3 +121,34 @@ def test_batch_rejects_foreign_objects(ctx):
         m.close()
     finally:
         other.close()
+
+
+def test_autotuner_choice_keeps_results(ctx):
+    """ppf_batch_tune picks launch shapes by timing them on the batch; the
+    batch's results afterwards equal the default shapes' and the plain
+    lookups', bit for bit."""
+    w = synth.batched(seed=21, n_models=3, n_scenes=4, model_range=(300, 900),
+                      scene_range=(1500, 4000), models_per_scene=(1, 2))
+    c = ppf_amd.Context(0)
+    try:
+        models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=c) for m, d in zip(w.models, w.d_dists)]
+        scenes = [ppf_amd.Scene(s, 0.0, w.rpdf, ctx=c) for s in w.scenes]
+        jm = [models[mi] for ids in w.scene_models for mi in ids]
+        js = [scenes[k] for k, ids in enumerate(w.scene_models) for _ in ids]
+        base, _ = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
+        choice = ppf_amd.batch_tune(jm, js, w.vote_count_threshold, ctx=c)
+        print("tuned:", choice, flush=True)
+        assert choice["vote_blocks_per_cu"] in (1, 2) and choice["finish_lanes"] in (1, 2, 4)
+        assert choice["sort_tile_keys"] in (2048, 4096) and choice["tune_ms"] > 0
+        tuned, info = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
+        assert info["lanes"] == min(choice["finish_lanes"], len(jm))
+        for a, b, m, s in zip(base, tuned, jm, js):
+            ca, na = a.votes()
+            cb, nb = b.votes()
+            assert np.array_equal(ca, cb) and np.array_equal(na, nb)
+            assert a.pose().tobytes() == b.pose().tobytes()
+            p = m.ppf_lookup(s)
+            assert p.pose().tobytes() == b.pose().tobytes()
+        ppf_amd.tune_reset(c)
+    finally:
+        c.close()

@@ -405,17 +405,25 @@ def test_device_radix_sort_is_stable(ctx, case):
         assert np.array_equal(sv, v[order]) and np.array_equal(sk, k[order])
 
 
-@pytest.mark.parametrize("n", [2, 4095, 4096, 4097, 1 << 21])
-def test_device_radix_sort_sizes_keys_only(ctx, n):
-    """Tile edges of the one-sweep sort (4,096 keys per tile), keys alone."""
-    rng = np.random.default_rng(n)
-    k = rng.integers(0, 2**40, n, dtype=np.uint64)
-    k[::3] = k[0]  # many equal keys
-    sk, _ = ctx.debug_sort(k, None, 0, 40)
-    assert np.array_equal(sk, np.sort(k, kind="stable"))
-    sk, _ = ctx.debug_sort(k, None, 16, 32)
-    digit = (k >> np.uint64(16)) & np.uint64(0xffff)
-    assert np.array_equal(sk, k[np.argsort(digit, kind="stable")])
+@pytest.mark.parametrize("items", [16, 8])
+@pytest.mark.parametrize("n", [2, 2047, 2048, 2049, 4095, 4096, 4097, 1 << 21])
+def test_device_radix_sort_sizes(n, items):
+    """Tile edges of the one-sweep sort (256 x 16 or 256 x 8 keys per tile),
+    keys alone and with values."""
+    c = ppf_amd.Context(0)
+    try:
+        c.debug_set(1, items)
+        rng = np.random.default_rng(n)
+        k = rng.integers(0, 2**40, n, dtype=np.uint64)
+        k[::3] = k[0]  # many equal keys
+        sk, _ = c.debug_sort(k, None, 0, 40)
+        assert np.array_equal(sk, np.sort(k, kind="stable"))
+        v = np.arange(n, dtype=np.uint32)
+        sk, sv = c.debug_sort(k, v, 16, 32)
+        order = np.argsort((k >> np.uint64(16)) & np.uint64(0xffff), kind="stable")
+        assert np.array_equal(sk, k[order]) and np.array_equal(sv, v[order])
+    finally:
+        c.close()
 
 
 def test_device_copy_kernel(ctx):
