@@ -445,6 +445,36 @@ __global__ __launch_bounds__(256) void k_row_cost(const VoteJob *__restrict__ jo
   if (lane == 0) keys[item] = ((uint64_t)(0xffffffffu - (uint32_t)min(c, (uint64_t)0xffffffffu)) << 32) | item;
 }
 
+// The float pipeline for one vote: record j (in vote order) against a hit's
+// v = (vy, vz).  moved_approx = 0x100 | bin when the fast loop already added
+// the vote to `bin`: then it is moved only if the exact bin differs.  Out of
+// line: it runs for a few votes in 10^5, and inlined its registers would
+// count against the fast loop's.
+__device__ __noinline__ void exact_vote_fn(const VoteJob *__restrict__ J, uint32_t *acc, uint32_t pitch,
+                                           uint32_t tile_base, uint32_t j, float vy, float vz,
+                                           uint32_t moved_approx) {
+  const uint32_t n_m = (uint32_t)J->n_m;
+  const uint32_t pr = gld(J->pair + j);
+  const uint32_t m_r = pr / n_m, m_i = pr - m_r * n_m;
+  const float *Fm = J->mframe + (size_t)m_r * 16;
+  float r1[4], r2[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    r1[kk] = gld(Fm + 4 + kk);
+    r2[kk] = gld(Fm + 8 + kk);
+  }
+  float uy, uz;
+  ppfg::frame_apply_yz(r1, r2, gld(J->mx + m_i), gld(J->my + m_i), gld(J->mz + m_i), uy, uz);
+  const uint32_t bin = ppfg::alpha_bin(uy, uz, vy, vz);
+  const uint32_t mloc = m_r - tile_base;
+  const bool moved = (moved_approx & 0x100u) != 0u;
+  const uint32_t approx = moved_approx & 0xffu;
+  if (!moved || bin != approx) {
+    if (moved) atomicSub(&acc[approx * pitch + mloc], 1u);
+    atomicAdd(&acc[bin * pitch + mloc], 1u);
+  }
+}
+
 template <bool EXACT>
 __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restrict__ jobs, int n_jobs,
                                                      uint64_t n_items, const uint64_t *__restrict__ order,
@@ -662,23 +692,7 @@ __global__ __launch_bounds__(VOTE_BLOCK, 8) void k_vote(const VoteJob *__restric
       // vz); a vote the fast loop added to bin `approx` (moved) is moved
       // when its exact bin differs
       auto exact_vote = [&](uint32_t j, float vy, float vz, bool moved, uint32_t approx) {
-        const uint32_t pr = gld(J.pair + j);
-        const uint32_t m_r = pr / (uint32_t)n_m, m_i = pr - m_r * (uint32_t)n_m;
-        const float *Fm = J.mframe + (size_t)m_r * 16;
-        float r1[4], r2[4];
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-          r1[kk] = gld(Fm + 4 + kk);
-          r2[kk] = gld(Fm + 8 + kk);
-        }
-        float uy, uz;
-        ppfg::frame_apply_yz(r1, r2, gld(J.mx + m_i), gld(J.my + m_i), gld(J.mz + m_i), uy, uz);
-        const uint32_t bin = ppfg::alpha_bin(uy, uz, vy, vz);
-        const uint32_t mloc = m_r - (uint32_t)tile_base;
-        if (!moved || bin != approx) {
-          if (moved) atomicSub(&acc[approx * pitch + mloc], 1u);
-          atomicAdd(&acc[bin * pitch + mloc], 1u);
-        }
+        exact_vote_fn(&J, acc, pitch, (uint32_t)tile_base, j, vy, vz, moved ? (0x100u | approx) : 0u);
       };
       for (int32_t i = tid; i < nb; i += VOTE_BLOCK) acc[i] = 0u;
       if (tid == 0) sh.next = 0;
