@@ -285,7 +285,7 @@ typedef struct {
   int32_t vote_blocks_per_cu; /* persistent vote blocks per compute unit */
   int32_t finish_lanes;       /* host threads + streams finishing jobs */
   int32_t sort_tile_keys;     /* keys per one-sweep radix-sort tile */
-  int32_t pad;
+  int32_t sort_tile_threads;  /* threads per one-sweep radix-sort tile */
   double best_vote_ms;        /* vote pass of the batch with the chosen shape */
   double best_finish_ms;      /* finishing of the batch with the chosen lanes */
   double best_sort_ms;        /* the tile's timed sorts */
@@ -379,8 +379,9 @@ int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_i
  * model buckets a reference row may group before the vote kernel votes that
  * row hit by hit (the path a row takes when its hits span more buckets than
  * the kernel's grouping table holds); 0 restores the table's size.  knob 1:
- * keys per thread of the one-sweep radix sort's tiles (8 or 16; the
- * autotuner's choice otherwise).  Results do not depend on either. */
+ * keys per thread of the one-sweep radix sort's tiles (8 or 16), knob 2:
+ * threads per tile (256 or 512) (the autotuner's choice otherwise).
+ * Results do not depend on any of them. */
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value);
 
 /* GPU spatial-hash clustering (Model::ClusterTransformations + max_element,

@@ -162,6 +162,7 @@ int ctx_lanes(ppf_ctx_s *root, int n, std::vector<ppf_ctx_s *> &out) {
       return rc;
     }
     l->sort.tile_items = root->sort.tile_items;
+    l->sort.tile_block = root->sort.tile_block;
     root->lanes.push_back(l);
   }
   out.assign(root->lanes.begin(), root->lanes.begin() + n);
@@ -1209,11 +1210,17 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
 
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value) {
   ARG_CHECK(ctx, "ppf_debug_set: NULL context");
-  ARG_CHECK(knob == 0 || knob == 1, "ppf_debug_set: unknown knob");
+  ARG_CHECK(knob >= 0 && knob <= 2, "ppf_debug_set: unknown knob");
   if (knob == 1) {
     ARG_CHECK(value == 8 || value == 16, "ppf_debug_set: sort tile keys per thread must be 8 or 16");
     ctx->root->sort.tile_items = (int)value;
     for (ppf_ctx_s *l : ctx->root->lanes) l->sort.tile_items = (int)value;
+    return PPF_OK;
+  }
+  if (knob == 2) {
+    ARG_CHECK(value == 256 || value == 512, "ppf_debug_set: sort tile threads must be 256 or 512");
+    ctx->root->sort.tile_block = (int)value;
+    for (ppf_ctx_s *l : ctx->root->lanes) l->sort.tile_block = (int)value;
     return PPF_OK;
   }
   ARG_CHECK(value >= 0 && value <= 0xffffffffll, "ppf_debug_set: value out of range");

@@ -66,7 +66,8 @@ struct DevBuf {
 // buffers); tmp_keys/tmp_vals are ping-pong space of the same length.
 struct SortScratch {
   DevBuf counts, scan, tmp_keys, tmp_vals;
-  int tile_items = 16;  // keys per thread of a one-sweep tile (8 or 16; tuned)
+  int tile_items = 16;   // keys per thread of a one-sweep tile (8 or 16; tuned)
+  int tile_block = 256;  // threads per one-sweep tile (256 or 512; tuned)
 };
 int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
                    SortScratch &s, hipStream_t st);

@@ -237,9 +237,34 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict
   const int tid = threadIdx.x;
   for (int i = tid; i < OS_MAX_PASSES * 256; i += OS_BLOCK) (&h[0][0])[i] = 0u;
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * OS_BLOCK + tid; i < n; i += (int64_t)gridDim.x * OS_BLOCK) {
-    const uint64_t k = keys[i];
-    for (int p = 0; p < npass; ++p) atomicAdd(&h[p][(uint32_t)(k >> (lo + 8 * p)) & 255u], 1u);
+  // each thread counts OS_HSEG keys, OS_BLOCK apart (coalesced loads), and
+  // adds one count per run of equal digits: keys with long runs of equal
+  // digits (vote-order words) would otherwise send 64 lanes' atomics to one
+  // LDS word at once
+  constexpr int OS_HSEG = 16;
+  for (int64_t base = (int64_t)blockIdx.x * OS_BLOCK * OS_HSEG + tid; base < n;
+       base += (int64_t)gridDim.x * OS_BLOCK * OS_HSEG) {
+    const int cnt_k = (int)min((int64_t)OS_HSEG, (n - base + OS_BLOCK - 1) / OS_BLOCK);
+    uint64_t k[OS_HSEG];
+#pragma unroll
+    for (int i = 0; i < OS_HSEG; ++i) k[i] = keys[base + (int64_t)min(i, cnt_k - 1) * OS_BLOCK];
+    for (int p = 0; p < npass; ++p) {
+      const int sh = lo + 8 * p;
+      uint32_t cur = (uint32_t)(k[0] >> sh) & 255u, run = 1u;
+#pragma unroll
+      for (int i = 1; i < OS_HSEG; ++i) {
+        if (i >= cnt_k) break;
+        const uint32_t d = (uint32_t)(k[i] >> sh) & 255u;
+        if (d == cur) {
+          ++run;
+        } else {
+          atomicAdd(&h[p][cur], run);
+          cur = d;
+          run = 1u;
+        }
+      }
+      atomicAdd(&h[p][cur], run);
+    }
   }
   __syncthreads();
   for (int i = tid; i < npass * 256; i += OS_BLOCK) {
@@ -259,27 +284,28 @@ __global__ __launch_bounds__(256) void k_os_starts(uint32_t *__restrict__ hist, 
   }
 }
 
-// ITEMS keys per thread: a tile of 256 * ITEMS keys (the tuner picks 8 or 16)
-template <bool VALS, int ITEMS>
-__global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
+// BLOCK threads x ITEMS keys per thread per tile (the tuner picks the shape);
+// threads 0..255 own one digit each in the count and look-back steps.
+template <bool VALS, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_os_pass(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ starts,
     uint32_t *__restrict__ status, uint32_t *__restrict__ counters) {
-  constexpr int OS_TILE_T = OS_BLOCK * ITEMS;
-  __shared__ uint64_t s_key[OS_TILE_T];
-  __shared__ uint32_t s_val[VALS ? OS_TILE_T : 1];
-  __shared__ uint32_t s_cnt[OS_WAVES][256];
+  constexpr int TILE = BLOCK * ITEMS;
+  constexpr int WV = BLOCK / WAVE;
+  __shared__ uint64_t s_key[TILE];
+  __shared__ uint32_t s_val[VALS ? TILE : 1];
+  __shared__ uint32_t s_cnt[WV][256];
   __shared__ uint32_t s_off[256];   // tile-local start of each digit
   __shared__ uint32_t s_glob[256];  // global position of tile-local position 0 of each digit
-  __shared__ uint32_t s_wsum[OS_WAVES + 1];
+  __shared__ uint32_t s_wsum[WV + 1];
   __shared__ uint32_t s_tile;
   const int tid = threadIdx.x, w = tid / WAVE, l = tid % WAVE;
   if (tid == 0) s_tile = atomicAdd(&counters[0], 1u);  // tiles in acquisition order
-#pragma unroll
-  for (int q = 0; q < OS_WAVES; ++q) s_cnt[q][tid] = 0u;
+  for (int i = tid; i < WV * 256; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
   __syncthreads();
   const uint32_t t = s_tile;
-  const int64_t base = (int64_t)t * OS_TILE_T;
+  const int64_t base = (int64_t)t * TILE;
   const int64_t base_w = base + (int64_t)w * WAVE * ITEMS;
   const uint64_t lt = lanemask_lt();
   uint64_t k[ITEMS];
@@ -310,49 +336,55 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
     if (valid && before == 0) s_cnt[w][d] = prev + (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  // thread d: the digit's count in the tile, its waves' offsets, its tile-local start
+  // thread d < 256: the digit's count in the tile, its waves' offsets
   const int d = tid;
+  const bool owner = d < 256;
   uint32_t cnt = 0;
+  if (owner) {
 #pragma unroll
-  for (int q = 0; q < OS_WAVES; ++q) {
-    const uint32_t c = s_cnt[q][d];
-    s_cnt[q][d] = cnt;
-    cnt += c;
+    for (int q = 0; q < WV; ++q) {
+      const uint32_t c = s_cnt[q][d];
+      s_cnt[q][d] = cnt;
+      cnt += c;
+    }
   }
   // publish the tile's count at once, so that later tiles can look back
   uint32_t *st_row = status + (size_t)t * 256;
-  __hip_atomic_store(&st_row[d], (t == 0 ? OS_INC : OS_AGG) | cnt, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (owner)
+    __hip_atomic_store(&st_row[d], (t == 0 ? OS_INC : OS_AGG) | cnt, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   uint32_t tot;
-  const uint32_t off = block_excl_scan(cnt, s_wsum, &tot);
-  s_off[d] = off;
-  // decoupled look-back over the preceding tiles' counts of digit d
-  uint32_t excl = 0;
-  if (t > 0) {
-    int64_t j = (int64_t)t - 1;
-    uint32_t spins = 0;
-    while (j >= 0) {
-      const uint32_t sv = __hip_atomic_load(&status[(size_t)j * 256 + d], __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t flag = sv & ~OS_COUNT;
-      if (flag == 0u) {
-        ++spins;
-        if (spins > OS_SPIN_LIMIT) {  // never expected: report instead of waiting forever
-          atomicOr(&counters[1], 1u);
-          break;
+  const uint32_t off = block_excl_scan(cnt, s_wsum, &tot);  // tile-local start of digit d
+  if (owner) {
+    s_off[d] = off;
+    // decoupled look-back over the preceding tiles' counts of digit d
+    uint32_t excl = 0;
+    if (t > 0) {
+      int64_t j = (int64_t)t - 1;
+      uint32_t spins = 0;
+      while (j >= 0) {
+        const uint32_t sv = __hip_atomic_load(&status[(size_t)j * 256 + d], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t flag = sv & ~OS_COUNT;
+        if (flag == 0u) {
+          ++spins;
+          if (spins > OS_SPIN_LIMIT) {  // never expected: report instead of waiting forever
+            atomicOr(&counters[1], 1u);
+            break;
+          }
+          if ((spins & 1023u) == 0u &&
+              __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            break;
+          continue;
         }
-        if ((spins & 1023u) == 0u &&
-            __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
-          break;
-        continue;
+        excl += sv & OS_COUNT;
+        if (flag == OS_INC) break;
+        --j;
       }
-      excl += sv & OS_COUNT;
-      if (flag == OS_INC) break;
-      --j;
+      __hip_atomic_store(&st_row[d], OS_INC | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __hip_atomic_store(&st_row[d], OS_INC | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_glob[d] = starts[d] + excl - off;
   }
-  s_glob[d] = starts[d] + excl - off;
   __syncthreads();
   // the tile in digit order, in LDS
 #pragma unroll
@@ -365,8 +397,8 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(
     }
   }
   __syncthreads();
-  const int nt = (int)min((int64_t)OS_TILE_T, n - base);
-  for (int i = tid; i < nt; i += OS_BLOCK) {
+  const int nt = (int)min((int64_t)TILE, n - base);
+  for (int i = tid; i < nt; i += BLOCK) {
     const uint64_t kk = s_key[i];
     const uint32_t pos = s_glob[(uint32_t)(kk >> shift) & 255u] + (uint32_t)i;
     kout[pos] = kk;
@@ -415,8 +447,12 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp
 static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
                                SortScratch &s, hipStream_t st) {
   const int npass = (bit_hi - bit_lo + 7) / 8;
-  const int items = s.tile_items == 8 ? 8 : 16;
-  const int64_t tile = (int64_t)OS_BLOCK * items;
+  // tiles of 512 x 16 keys for sorts of 2^24 keys and more (the model
+  // build's, measured fastest there); the tuned shape below that
+  const bool big = n >= ((int64_t)1 << 24);
+  const int items = big ? 16 : (s.tile_items == 8 ? 8 : 16);
+  const int block = big ? 512 : (s.tile_block == 512 ? 512 : 256);
+  const int64_t tile = (int64_t)block * items;
   const int64_t n_tiles = (n + tile - 1) / tile;
   // [hist/starts: npass x 256][counters: 2 per pass][status: n_tiles x 256]
   const size_t hist_words = (size_t)OS_MAX_PASSES * 256, ctr_words = 2 * OS_MAX_PASSES;
@@ -427,7 +463,7 @@ static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bi
   uint32_t *ctr = hist + hist_words;
   uint32_t *status = ctr + ctr_words;
   PPF_HIP(hipMemsetAsync(hist, 0, (hist_words + ctr_words) * 4, st));
-  const unsigned hist_blocks = (unsigned)std::min<int64_t>((n + OS_BLOCK * 16 - 1) / (OS_BLOCK * 16), 2048);
+  const unsigned hist_blocks = (unsigned)std::min<int64_t>((n + OS_BLOCK * 64 - 1) / (OS_BLOCK * 64), 2048);
   hipLaunchKernelGGL(k_os_hist, dim3(hist_blocks), dim3(OS_BLOCK), 0, st, keys, n, bit_lo, npass, hist);
   hipLaunchKernelGGL(k_os_starts, dim3(1), dim3(256), 0, st, hist, npass);
   uint64_t *ka = keys, *kb = s.tmp_keys.as<uint64_t>();
@@ -435,9 +471,14 @@ static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bi
   for (int p = 0; p < npass; ++p) {
     PPF_HIP(hipMemsetAsync(status, 0, (size_t)n_tiles * 256 * 4, st));
     const int shift = bit_lo + 8 * p;
-    auto kern = vals ? (items == 8 ? k_os_pass<true, 8> : k_os_pass<true, 16>)
-                     : (items == 8 ? k_os_pass<false, 8> : k_os_pass<false, 16>);
-    hipLaunchKernelGGL(kern, dim3((unsigned)n_tiles), dim3(OS_BLOCK), 0, st, ka, va, kb, vb, n, shift,
+    decltype(&k_os_pass<true, 256, 16>) kern;
+    if (block == 512)
+      kern = vals ? (items == 8 ? k_os_pass<true, 512, 8> : k_os_pass<true, 512, 16>)
+                  : (items == 8 ? k_os_pass<false, 512, 8> : k_os_pass<false, 512, 16>);
+    else
+      kern = vals ? (items == 8 ? k_os_pass<true, 256, 8> : k_os_pass<true, 256, 16>)
+                  : (items == 8 ? k_os_pass<false, 256, 8> : k_os_pass<false, 256, 16>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)n_tiles), dim3(block), 0, st, ka, va, kb, vb, n, shift,
                        hist + 256 * p, status, ctr + 2 * p);
     PPF_HIP(hipGetLastError());
     std::swap(ka, kb);

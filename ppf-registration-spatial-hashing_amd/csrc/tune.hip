@@ -8,7 +8,7 @@
 //   * vote blocks per CU (1 or the most the LDS allows, 2): two blocks hide
 //     latency, one leaves the whole CU to a block;
 //   * finishing lanes (1, 2 or 4 host threads with their own streams);
-//   * one-sweep radix-sort tile (256 threads x 8 or x 16 keys).
+//   * one-sweep radix-sort tile (256 or 512 threads x 8 or 16 keys).
 // None of them changes a result: vote counts are integer atomics, every job
 // is finished by the same stage sequence on whatever lane, and the sort is
 // stable for any tile.
@@ -98,8 +98,10 @@ int ppf_batch_tune(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const p
     PPF_TRY(vals.ensure((size_t)n_sort * 4));
     PPF_HIP(hipMemcpy(keys0.ptr, hk.data(), (size_t)n_sort * 8, hipMemcpyHostToDevice));
     double best_sort = 1e300;
-    int best_items = 16;
-    for (int items : {16, 8}) {
+    int best_items = 16, best_block = 256;
+    for (const auto shape : {std::pair<int, int>{256, 16}, {512, 16}, {512, 8}, {256, 8}}) {
+      const int block = shape.first, items = shape.second;
+      root->sort.tile_block = block;
       root->sort.tile_items = items;
       double t = 1e300;
       for (int rep = 0; rep < 3; ++rep) {
@@ -115,11 +117,17 @@ int ppf_batch_tune(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const p
       if (t < best_sort) {
         best_sort = t;
         best_items = items;
+        best_block = block;
       }
     }
     root->sort.tile_items = best_items;
-    for (ppf_ctx_s *l : root->lanes) l->sort.tile_items = best_items;
-    info->sort_tile_keys = 256 * best_items;
+    root->sort.tile_block = best_block;
+    for (ppf_ctx_s *l : root->lanes) {
+      l->sort.tile_items = best_items;
+      l->sort.tile_block = best_block;
+    }
+    info->sort_tile_keys = best_block * best_items;
+    info->sort_tile_threads = best_block;
     info->best_sort_ms = best_sort;
   }
 
@@ -155,7 +163,11 @@ int ppf_tune_reset(ppf_ctx ctx) {
   root->tune_vote_per_cu = 0;
   root->tune_lanes = 0;
   root->sort.tile_items = 16;
-  for (ppf_ctx_s *l : root->lanes) l->sort.tile_items = 16;
+  root->sort.tile_block = 256;
+  for (ppf_ctx_s *l : root->lanes) {
+    l->sort.tile_items = 16;
+    l->sort.tile_block = 256;
+  }
   return PPF_OK;
 }
 

@@ -71,7 +71,7 @@ class BatchInfo(C.Structure):
 
 class TuneInfo(C.Structure):
     _fields_ = [("vote_blocks_per_cu", C.c_int32), ("finish_lanes", C.c_int32),
-                ("sort_tile_keys", C.c_int32), ("pad", C.c_int32), ("best_vote_ms", C.c_double),
+                ("sort_tile_keys", C.c_int32), ("sort_tile_threads", C.c_int32), ("best_vote_ms", C.c_double),
                 ("best_finish_ms", C.c_double), ("best_sort_ms", C.c_double), ("tune_ms", C.c_double)]
 
 
@@ -608,7 +608,7 @@ def batch_tune(models: Sequence["Model"], scenes: Sequence[Scene], vote_count_th
     prm = LookupParams(float(vote_count_threshold), 0, 0, 0, 0, 0, 0)
     info = TuneInfo()
     _check(lib().ppf_batch_tune(ctx.handle, n, mh, sh, C.byref(prm), C.byref(info)))
-    return {k: getattr(info, k) for k, _ in TuneInfo._fields_ if k != "pad"}
+    return {k: getattr(info, k) for k, _ in TuneInfo._fields_}
 
 
 def tune_reset(ctx: Context):

@@ -139,7 +139,8 @@ def test_autotuner_choice_keeps_results(ctx):
         choice = ppf_amd.batch_tune(jm, js, w.vote_count_threshold, ctx=c)
         print("tuned:", choice, flush=True)
         assert choice["vote_blocks_per_cu"] in (1, 2) and choice["finish_lanes"] in (1, 2, 4)
-        assert choice["sort_tile_keys"] in (2048, 4096) and choice["tune_ms"] > 0
+        assert choice["sort_tile_keys"] in (2048, 4096) and choice["sort_tile_threads"] in (256, 512)
+        assert choice["tune_ms"] > 0
         tuned, info = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
         assert info["lanes"] == min(choice["finish_lanes"], len(jm))
         for a, b, m, s in zip(base, tuned, jm, js):

@@ -405,14 +405,15 @@ def test_device_radix_sort_is_stable(ctx, case):
         assert np.array_equal(sv, v[order]) and np.array_equal(sk, k[order])
 
 
-@pytest.mark.parametrize("items", [16, 8])
-@pytest.mark.parametrize("n", [2, 2047, 2048, 2049, 4095, 4096, 4097, 1 << 21])
-def test_device_radix_sort_sizes(n, items):
-    """Tile edges of the one-sweep sort (256 x 16 or 256 x 8 keys per tile),
+@pytest.mark.parametrize("shape", [(256, 16), (256, 8), (512, 8), (512, 16)])
+@pytest.mark.parametrize("n", [2, 2047, 2048, 2049, 4095, 4096, 4097, 8193, 1 << 21])
+def test_device_radix_sort_sizes(n, shape):
+    """Tile edges of the one-sweep sort (threads x keys per thread per tile),
     keys alone and with values."""
     c = ppf_amd.Context(0)
     try:
-        c.debug_set(1, items)
+        c.debug_set(2, shape[0])
+        c.debug_set(1, shape[1])
         rng = np.random.default_rng(n)
         k = rng.integers(0, 2**40, n, dtype=np.uint64)
         k[::3] = k[0]  # many equal keys
@@ -424,6 +425,18 @@ def test_device_radix_sort_sizes(n, items):
         assert np.array_equal(sk, k[order]) and np.array_equal(sv, v[order])
     finally:
         c.close()
+
+
+def test_device_radix_sort_big_tiles(ctx):
+    """Sorts of 2^24 keys and more use 512 x 16-key tiles; a partial last tile."""
+    n = (1 << 24) + 12345
+    rng = np.random.default_rng(24)
+    k = rng.integers(0, 2**48, n, dtype=np.uint64)
+    k[: n // 2] &= np.uint64(0xFFFF0000FFFF)  # runs of equal digits in the middle bytes
+    v = np.arange(n, dtype=np.uint32)
+    sk, sv = ctx.debug_sort(k, v, 8, 40)
+    order = np.argsort((k >> np.uint64(8)) & np.uint64(0xFFFFFFFF), kind="stable")
+    assert np.array_equal(sv, v[order]) and np.array_equal(sk, k[order])
 
 
 def test_device_copy_kernel(ctx):
