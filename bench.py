@@ -332,6 +332,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed batched steps (for profiling runs)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (default launch shapes; for counter runs)")
     ap.add_argument("--max-product", type=float, default=SUBSET_MAX_PRODUCT,
                     help="subset rule: jobs with n_scene * n_model <= this")
     ap.add_argument("--large-hash-rpdf", type=int, default=100,
@@ -358,6 +360,7 @@ def main():
               flush=True)
         return
     os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the oracle's OpenMP runtime loads
+    host = cpu_info()  # before the oracle's OpenMP runtime binds this thread to one core
 
     t0 = time.time()
     w = synth.batched()
@@ -374,9 +377,11 @@ def main():
     scenes = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in used}
 
     # the project's autotuner picks the batched form's launch shapes on this batch
-    tuned = ppf_amd.batch_tune([models[mi] for _, mi in jobs], [scenes[si] for si, _ in jobs],
-                               w.vote_count_threshold, ctx=ctx)
-    log(f"autotuned in {tuned['tune_ms'] / 1e3:.1f}s: {tuned}")
+    tuned = None
+    if not args.no_tune:
+        tuned = ppf_amd.batch_tune([models[mi] for _, mi in jobs], [scenes[si] for si, _ in jobs],
+                                   w.vote_count_threshold, ctx=ctx)
+        log(f"autotuned in {tuned['tune_ms'] / 1e3:.1f}s: {tuned}")
     for i in range(args.warmup):
         run_batched(ctx, models, scenes, jobs, w.vote_count_threshold)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -476,6 +481,7 @@ def main():
         log("cpu baseline ...")
         sc = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in sorted({si for si, _ in jobs})}
         out["cpu_baseline"] = cpu_baseline(w, jobs, p_infos, models, sc, args.cpu_seconds)
+        out["cpu_baseline"].update(host)
         for s in sc.values():
             s.close()
     print(json.dumps(out), flush=True)
