@@ -13,8 +13,8 @@
 //
 // Independence from the product: FNV-1a is written byte-wise through a signed
 // char pointer as in the reference, quant_downf uses glibc's fmodf, the 4x4
-// matrix code is the reference's literal sequence, the hash array uses
-// std::stable_sort / std::lower_bound.  Only acosf/atan2f/sinf/cosf come from
+// matrix code is the reference's literal sequence, the hash array is a
+// stable sort by key (ascending (key, index) words) and std::lower_bound.  Only acosf/atan2f/sinf/cosf come from
 // the build's ppf_math.h: the reference's GPU libm bits are not reproducible,
 // so both sides must share one implementation (pinned against double-precision
 // libm in tests/test_numerics.py).
@@ -30,6 +30,7 @@
 #include "ppf_oracle.h"
 
 #include <omp.h>
+#include <parallel/algorithm>
 
 #include <algorithm>
 #include <chrono>
@@ -343,9 +344,20 @@ struct HashArray {
 HashArray build_hash_array(const uint32_t *data, int64_t n) {
   HashArray h;
   h.key2data.resize(n);
-  std::iota(h.key2data.begin(), h.key2data.end(), (uint64_t)0);
-  std::stable_sort(h.key2data.begin(), h.key2data.end(),
-                   [&](uint64_t a, uint64_t b) { return data[a] < data[b]; });
+  if (n < ((int64_t)1 << 32)) {
+    // a stable sort by key is the ascending order of (key, index): sort the
+    // packed words (parallel; the order is unique, so equal to std::stable_sort's)
+    std::vector<uint64_t> wd((size_t)n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) wd[i] = ((uint64_t)data[i] << 32) | (uint64_t)i;
+    __gnu_parallel::sort(wd.begin(), wd.end());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) h.key2data[i] = wd[i] & 0xffffffffull;
+  } else {
+    std::iota(h.key2data.begin(), h.key2data.end(), (uint64_t)0);
+    std::stable_sort(h.key2data.begin(), h.key2data.end(),
+                     [&](uint64_t a, uint64_t b) { return data[a] < data[b]; });
+  }
   for (int64_t i = 0; i < n; ++i) {
     uint32_t k = data[h.key2data[i]];
     if (i == 0 || k != h.hashkeys.back()) {
