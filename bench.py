@@ -399,8 +399,13 @@ def main():
     votes = sum(i["votes"] for i in step_info)
     vote_ms = sum(i["vote_ms"] for i in step_info)
     vote_launches = sum(i["vote_launches"] for i in step_info)
-    cl_ms = sum(i["cluster_kernel_ms"] for i in job_infos)
-    cl_launches = sum(1 for i in job_infos if i["cluster_kernel_ms"] > 0)
+    fused = all(i["fused"] for i in step_info)
+    if fused:  # one clustering launch per step, over every job's pose chunks
+        cl_ms = sum(i["cluster_kernel_ms"] for i in step_info)
+        cl_launches = sum(1 for i in step_info if i["cluster_kernel_ms"] > 0)
+    else:  # one clustering launch per job, on the finishing lanes
+        cl_ms = sum(i["cluster_kernel_ms"] for i in job_infos)
+        cl_launches = sum(1 for i in job_infos if i["cluster_kernel_ms"] > 0)
     kernels = {
         "k_vote": kernel_roofline("k_vote<false>", vote_ms / vote_launches,
                                   (8.0 * lookups + 4.0 * votes) / vote_launches),
@@ -441,7 +446,12 @@ def main():
         "latency_ms": {"p50": pct(step_s, 50) * 1e3, "p99": pct(step_s, 99) * 1e3},
         "vote_kernel_ms_per_step": vote_ms / args.steps,
         "batched_split_ms_per_step": {"vote": vote_ms / args.steps,
-                                      "finish_wall": sum(i["finish_ms"] for i in step_info) / args.steps},
+                                      "finish_wall": sum(i["finish_ms"] for i in step_info) / args.steps,
+                                      "finish_together": fused,
+                                      "select": sum(i["select_ms"] for i in step_info) / args.steps,
+                                      "poses": sum(i["pose_ms"] for i in step_info) / args.steps,
+                                      "cluster": sum(i["cluster_ms"] for i in step_info) / args.steps,
+                                      "cluster_kernel": cl_ms / args.steps},
         "roofline": roofline,
     }
     if args.headline_only:

@@ -255,9 +255,14 @@ int ppf_ht_dist(const float a16[16], const float b16[16], float out2[2]);
  * Many (model, scene) pairs in one call: the batched form of the loop in
  * ppf_registration (src/cuda/ppf.cu:63-98) over device-resident models.
  * All jobs are voted in one persistent launch (work items are (job, reference
- * point), longest jobs first); the stages after voting run concurrently on
- * the context's lanes.  results_out[j] is the result of (models[j],
- * scenes[j]) and is bit-identical to ppf_lookup(models[j], scenes[j], params).
+ * point), longest jobs first).  The stages after voting then run once for
+ * all jobs -- one threshold/order sort, one pose launch, one clustering
+ * launch over every job's pose chunks (heaviest first), one argmax -- or,
+ * with CPU clustering or when the batch's sort keys would exceed 64 bits,
+ * job by job concurrently on the context's lanes.  results_out[j] is the
+ * result of (models[j], scenes[j]) and is bit-identical to
+ * ppf_lookup(models[j], scenes[j], params); the results of a batch finished
+ * together share one device allocation, freed with the last of them.
  * A model or scene may appear in several jobs.  info may be NULL. */
 #define PPF_MAX_BATCH_JOBS 65536
 typedef struct {
@@ -265,10 +270,16 @@ typedef struct {
   int64_t lookups;       /* sum over jobs */
   int64_t votes;         /* sum over jobs */
   int32_t vote_launches; /* 1, or more when a job's candidate buffer overflowed */
-  int32_t lanes;         /* lanes that finished the jobs */
+  int32_t lanes;         /* lanes that finished the jobs (0: finished together) */
   double vote_ms;        /* device time of the vote launches */
   double finish_ms;      /* wall time of threshold/order, poses and clustering */
   double total_ms;       /* wall time of the call */
+  int32_t fused;         /* 1: the stages after voting ran once for all jobs */
+  int32_t reserved;
+  double select_ms;      /* fused: device time of threshold/order of all jobs */
+  double pose_ms;        /* fused: device time of the poses of all jobs */
+  double cluster_ms;     /* fused: device time of clustering and final poses */
+  double cluster_kernel_ms; /* fused: the pair-test kernel alone */
 } ppf_batch_info_t;
 int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const ppf_scene *scenes,
                      const ppf_lookup_params *params, ppf_result *results_out,
@@ -283,7 +294,7 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
  * identical whatever the shapes.  ppf_tune_reset restores the defaults. */
 typedef struct {
   int32_t vote_blocks_per_cu; /* persistent vote blocks per compute unit */
-  int32_t finish_lanes;       /* host threads + streams finishing jobs */
+  int32_t finish_lanes;       /* host threads + streams finishing jobs (0: all jobs together) */
   int32_t sort_tile_keys;     /* keys per one-sweep radix-sort tile */
   int32_t sort_tile_threads;  /* threads per one-sweep radix-sort tile */
   double best_vote_ms;        /* vote pass of the batch with the chosen shape */
@@ -380,8 +391,9 @@ int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_i
  * row hit by hit (the path a row takes when its hits span more buckets than
  * the kernel's grouping table holds); 0 restores the table's size.  knob 1:
  * keys per thread of the one-sweep radix sort's tiles (8 or 16), knob 2:
- * threads per tile (256 or 512) (the autotuner's choice otherwise).
- * Results do not depend on any of them. */
+ * threads per tile (256 or 512) (the autotuner's choice otherwise).  knob 3:
+ * how ppf_batch_lookup finishes its jobs: 0 together when it can (default),
+ * 1 job by job on lanes.  Results do not depend on any of them. */
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value);
 
 /* GPU spatial-hash clustering (Model::ClusterTransformations + max_element,

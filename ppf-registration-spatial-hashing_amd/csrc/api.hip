@@ -71,6 +71,7 @@ void DevBuf::release() {
 
 int TBuf::alloc(ppf_ctx c, size_t n) {
   release();
+  view = false;
   ctx = c->root;
   size_t want = std::max<size_t>(n, 256);
   if (current_arena()) {
@@ -96,13 +97,14 @@ int TBuf::alloc(ppf_ctx c, size_t n) {
   return 0;
 }
 void TBuf::release() {
-  if (ptr && !arena) {
+  if (ptr && !arena && !view) {
     (void)hipFree(ptr);
     if (ctx) ctx->unreserve(bytes);
   }
   ptr = nullptr;
   bytes = 0;
   arena = false;
+  view = false;
 }
 
 namespace ppfi {
@@ -141,7 +143,9 @@ void ctx_release(ppf_ctx_s *c) {
   c->sort.scan.release();
   c->sort.tmp_keys.release();
   c->sort.tmp_vals.release();
-  for (DevBuf *b : {&c->tmp_a, &c->tmp_b, &c->tmp_c, &c->tmp_d, &c->hits, &c->queue, &c->sel}) b->release();
+  for (DevBuf *b : {&c->tmp_a, &c->tmp_b, &c->tmp_c, &c->tmp_d, &c->hits, &c->queue, &c->sel, &c->fin_meta,
+                    &c->fin_flags, &c->fin_keys, &c->fin_cl, &c->fin_chunk, &c->fin_order, &c->fin_scan})
+    b->release();
   for (hipEvent_t *ev : {&c->ev0, &c->ev1, &c->ev2, &c->ev3, &c->ev4, &c->ev5, &c->ev6})
     if (*ev) {
       (void)hipEventDestroy(*ev);
@@ -1210,7 +1214,12 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
 
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value) {
   ARG_CHECK(ctx, "ppf_debug_set: NULL context");
-  ARG_CHECK(knob >= 0 && knob <= 2, "ppf_debug_set: unknown knob");
+  ARG_CHECK(knob >= 0 && knob <= 3, "ppf_debug_set: unknown knob");
+  if (knob == 3) {
+    ARG_CHECK(value == PPF_FINISH_AUTO || value == PPF_FINISH_LANES, "ppf_debug_set: finish mode must be 0 or 1");
+    ctx->root->finish_mode = (int32_t)value;
+    return PPF_OK;
+  }
   if (knob == 1) {
     ARG_CHECK(value == 8 || value == 16, "ppf_debug_set: sort tile keys per thread must be 8 or 16");
     ctx->root->sort.tile_items = (int)value;

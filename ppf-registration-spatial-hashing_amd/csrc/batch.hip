@@ -1,7 +1,8 @@
 // batch.hip -- batched registration: many (model, scene) pairs of uneven
 // sizes voted in one persistent launch over device-resident model hash
-// arrays, then finished (threshold/order, poses, clustering) concurrently on
-// the context's lanes.
+// arrays, then finished (threshold/order, poses, clustering) for all jobs
+// together, one launch per stage (finish_batch), or job by job concurrently
+// on the context's lanes.
 //
 // The reference registers one pair at a time and rebuilds the model for every
 // scene (src/cuda/ppf.cu:63-98); each pair's result here is bit-identical to
@@ -10,6 +11,7 @@
 // maximum.
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -119,6 +121,120 @@ void make_batch_jobs(int32_t n_jobs, const ppf_model *models, const ppf_scene *s
   }
 }
 
+namespace {
+// The kept votes and clustering arrays of a batch finished together; each
+// job's result views its range and shares the ownership.
+struct BatchStore {
+  TBuf codes, counts, T, vc, trans, quats;
+};
+}  // namespace
+
+bool batch_fused_ok(const std::vector<BatchJob> &jobs, const ppf_lookup_params *p) {
+  if (p->cpu_clustering || jobs.empty()) return false;
+  uint64_t C = 0;
+  uint32_t mx = 0;
+  int64_t ns = 0, nm = 0;
+  for (const BatchJob &b : jobs) {
+    C += b.n_cand;
+    mx = std::max(mx, b.max_count);
+    ns = std::max(ns, b.s->cloud.n);
+    nm = std::max(nm, b.m->cloud.n);
+  }
+  return ppfk::finish_batch_fits((int)jobs.size(), C, mx, ns, nm);
+}
+
+int batch_finish_fused(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p,
+                       std::vector<std::unique_ptr<ppf_result_s>> &results, ppf_batch_info_t *stages) {
+  ppf_ctx root = ctx->root;
+  const size_t nj = jobs.size();
+  std::vector<ppfk::FinJob> fj(nj);
+  std::vector<uint32_t> n_cand(nj);
+  uint32_t mx = 0;
+  int64_t ns = 0, nm = 0;
+  for (size_t j = 0; j < nj; ++j) {
+    const BatchJob &b = jobs[j];
+    auto *cc = b.cand.as<unsigned long long>();
+    fj[j].cand_code = cc;
+    fj[j].cand_count = reinterpret_cast<const uint32_t *>(cc + b.cap);
+    fj[j].thr_f = b.thr * (float)b.max_count;  // model.cu:164, as select_and_sort
+    fj[j].rpdf = b.s->rpdf;
+    fj[j].mframe = b.m->frames.as<float>();
+    fj[j].sframe = b.s->frames.as<float>();
+    fj[j].weights = b.m->weights.as<float>();
+    fj[j].d_dist = b.m->d_dist;
+    n_cand[j] = b.n_cand;
+    mx = std::max(mx, b.max_count);
+    ns = std::max(ns, b.s->cloud.n);
+    nm = std::max(nm, b.m->cloud.n);
+  }
+  auto store = std::make_shared<BatchStore>();
+  auto alloc = [&](int64_t P, ppfk::FinishArrays &A) -> int {
+    const size_t n = (size_t)std::max<int64_t>(P, 1);
+    PPF_TRY(store->codes.alloc(root, n * 8));
+    PPF_TRY(store->counts.alloc(root, n * 4));
+    PPF_TRY(store->T.alloc(root, n * 64));
+    PPF_TRY(store->vc.alloc(root, n * 4));
+    PPF_TRY(store->trans.alloc(root, n * 12));
+    PPF_TRY(store->quats.alloc(root, n * 16));
+    A.codes = store->codes.as<unsigned long long>();
+    A.counts = store->counts.as<uint32_t>();
+    A.T = store->T.as<float>();
+    A.vc = store->vc.as<float>();
+    A.trans = store->trans.as<float>();
+    A.quats = store->quats.as<float>();
+    return 0;
+  };
+  ppfk::FinishOut out;
+  PPF_TRY(ppfk::finish_batch(root, fj, n_cand, mx, ns, nm, p->use_l1_norm != 0, p->use_averaged_clusters != 0,
+                             alloc, out));
+  results.clear();
+  results.resize(nj);
+  for (size_t j = 0; j < nj; ++j) {
+    const BatchJob &b = jobs[j];
+    std::unique_ptr<ppf_result_s> r(new ppf_result_s());
+    r->ctx = root;
+    r->n_m = b.m->cloud.n;
+    r->cpu = false;
+    r->info.lookups = (int64_t)b.s->R * (b.s->cloud.n - 1);
+    r->info.votes = b.votes;
+    r->info.max_count = b.max_count;
+    r->info.n_candidates = b.n_cand;
+    const int64_t k0 = (int64_t)out.kbeg[j], P = (int64_t)out.kbeg[j + 1] - k0;
+    r->P = P;
+    r->info.n_kept = P;
+    r->info.max_idx = out.max_idx[j];
+    // the batch's stage times: every job's finishing ran in the same launches
+    r->info.select_ms = out.select_ms;
+    r->info.pose_ms = out.pose_ms;
+    r->info.cluster_ms = out.cluster_ms;
+    r->info.cluster_kernel_ms = out.cluster_kernel_ms;
+    r->info.cluster_tests = out.tests[j];
+    r->info.cluster_mixed_chunks = out.mixed[j];
+    if (P > 0) {
+      r->codes.set_view(store->codes.as<char>() + (size_t)k0 * 8, (size_t)P * 8);
+      r->counts.set_view(store->counts.as<char>() + (size_t)k0 * 4, (size_t)P * 4);
+      r->T.set_view(store->T.as<char>() + (size_t)k0 * 64, (size_t)P * 64);
+      r->vc.set_view(store->vc.as<char>() + (size_t)k0 * 4, (size_t)P * 4);
+      r->trans.set_view(store->trans.as<char>() + (size_t)k0 * 12, (size_t)P * 12);
+      r->quats.set_view(store->quats.as<char>() + (size_t)k0 * 16, (size_t)P * 16);
+      r->store = store;
+      std::memcpy(r->pose, out.pose.data() + 16 * j, 64);
+      r->info.status = PPF_OK;
+    } else {
+      r->info.status = PPF_ERR_NO_VOTES;
+    }
+    results[j] = std::move(r);
+  }
+  if (stages) {
+    stages->fused = 1;
+    stages->select_ms = out.select_ms;
+    stages->pose_ms = out.pose_ms;
+    stages->cluster_ms = out.cluster_ms;
+    stages->cluster_kernel_ms = out.cluster_kernel_ms;
+  }
+  return 0;
+}
+
 int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,
                  std::vector<std::unique_ptr<ppf_result_s>> &results) {
   const size_t nj = jobs.size();
@@ -204,7 +320,14 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
   std::vector<std::unique_ptr<ppf_result_s>> res;
   const auto t1 = std::chrono::steady_clock::now();
   const int lanes = root->tune_lanes > 0 ? root->tune_lanes : PPF_BATCH_LANES;
-  PPF_TRY(batch_finish(root, q, params, lanes, res));
+  ppf_batch_info_t stages{};
+  // one launch per finishing stage over all jobs, unless the context asks for
+  // lanes (tuner or diagnostics) or the batch's sort keys do not fit
+  const bool fused = root->finish_mode != PPF_FINISH_LANES && batch_fused_ok(q, params);
+  if (fused)
+    PPF_TRY(batch_finish_fused(root, q, params, res, &stages));
+  else
+    PPF_TRY(batch_finish(root, q, params, lanes, res));
   const auto t2 = std::chrono::steady_clock::now();
   int64_t L = 0, V = 0;
   for (size_t k = 0; k < order.size(); ++k) {
@@ -219,7 +342,12 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
     info->lookups = L;
     info->votes = V;
     info->vote_launches = launches;
-    info->lanes = std::min<int>(lanes, n_jobs);
+    info->lanes = fused ? 0 : std::min<int>(lanes, n_jobs);
+    info->fused = fused ? 1 : 0;
+    info->select_ms = stages.select_ms;
+    info->pose_ms = stages.pose_ms;
+    info->cluster_ms = stages.cluster_ms;
+    info->cluster_kernel_ms = stages.cluster_kernel_ms;
     info->vote_ms = vote_ms;
     info->finish_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     info->total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();

@@ -21,6 +21,7 @@
 //                       (kernel.cu:647-763, model.cu:202-244,293-295)
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "ppf_cluster.h"
 #include "ppf_geom.h"
@@ -116,16 +117,20 @@ __global__ __launch_bounds__(B256) void k_pair_keys(const float *__restrict__ so
 // ------------------------------------------------------------ unique
 constexpr int UQ_MAX_BLOCKS = 1024;
 
-__device__ __forceinline__ bool is_head(const uint64_t *__restrict__ w, int64_t j) {
-  return j == 0 || (uint32_t)(w[j] >> 32) != (uint32_t)(w[j - 1] >> 32);
+// A run of equal keys starts at j; the key of a word is w >> shift (32: the
+// high half; the batched clustering's words carry a (job, cell) key above a
+// pose index of fewer bits).
+__device__ __forceinline__ bool is_head(const uint64_t *__restrict__ w, int64_t j, int shift) {
+  return j == 0 || (w[j] >> shift) != (w[j - 1] >> shift);
 }
 
 __global__ __launch_bounds__(B256) void k_unique_count(const uint64_t *__restrict__ w, int64_t n,
-                                                       int64_t per, uint32_t *__restrict__ counts) {
+                                                       int64_t per, int shift,
+                                                       uint32_t *__restrict__ counts) {
   __shared__ uint32_t s[B256 / WAVE];
   const int64_t beg = (int64_t)blockIdx.x * per, end = min(beg + per, n);
   uint32_t c = 0;
-  for (int64_t j = beg + threadIdx.x; j < end; j += B256) c += is_head(w, j) ? 1u : 0u;
+  for (int64_t j = beg + threadIdx.x; j < end; j += B256) c += is_head(w, j, shift) ? 1u : 0u;
   c = wave_sum_u32(c);
   if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = c;
   __syncthreads();
@@ -136,12 +141,15 @@ __global__ __launch_bounds__(B256) void k_unique_count(const uint64_t *__restric
   }
 }
 
+// hkeys[u]: the low 32 bits of run u's key; hjob[u] (may be NULL): the bits
+// above them; low[j]: the bits of word j below the key.
 __global__ __launch_bounds__(B256) void k_unique_emit(const uint64_t *__restrict__ w, int64_t n,
-                                                      int64_t per,
+                                                      int64_t per, int shift,
                                                       const uint32_t *__restrict__ offsets,
                                                       uint32_t *__restrict__ hkeys,
                                                       uint32_t *__restrict__ hfirst,
-                                                      uint32_t *__restrict__ low) {
+                                                      uint32_t *__restrict__ low,
+                                                      uint32_t *__restrict__ hjob) {
   __shared__ uint32_t s_w[B256 / WAVE];
   __shared__ uint32_t s_run;
   const int tid = threadIdx.x, wv = tid / WAVE, l = tid % WAVE;
@@ -153,8 +161,8 @@ __global__ __launch_bounds__(B256) void k_unique_emit(const uint64_t *__restrict
     const int64_t j = c0 + tid;
     const bool valid = j < end;
     uint64_t wj = valid ? w[j] : 0ull;
-    const bool h = valid && is_head(w, j);
-    if (valid && low) low[j] = (uint32_t)wj;
+    const bool h = valid && is_head(w, j, shift);
+    if (valid && low) low[j] = (uint32_t)(wj & ((1ull << shift) - 1ull));
     const uint64_t m = __ballot(h);
     if (l == 0) s_w[wv] = (uint32_t)__popcll(m);
     __syncthreads();
@@ -162,8 +170,9 @@ __global__ __launch_bounds__(B256) void k_unique_emit(const uint64_t *__restrict
     for (int k = 0; k < wv; ++k) base += s_w[k];
     if (h) {
       const uint32_t u = base + (uint32_t)__popcll(m & lt);
-      hkeys[u] = (uint32_t)(wj >> 32);
+      hkeys[u] = (uint32_t)(wj >> shift);
       hfirst[u] = (uint32_t)j;
+      if (hjob) hjob[u] = (uint32_t)(wj >> shift >> 32);
     }
     __syncthreads();
     if (tid == 0) {
@@ -969,12 +978,10 @@ __global__ void k_gather_pairs(const uint64_t *__restrict__ codes, const uint32_
 }
 
 // ----------------------------------------------------------------- poses
-__global__ void k_poses(const unsigned long long *__restrict__ codes, int64_t P,
-                        const float *__restrict__ mframe, const float *__restrict__ sframe,
-                        uint32_t rpdf, float *__restrict__ T) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P) return;
-  const unsigned long long c = codes[i];
+// The pose of kept vote code c into T[0..15].
+__device__ __forceinline__ void pose_of_code(unsigned long long c, const float *__restrict__ mframe,
+                                             const float *__restrict__ sframe, uint32_t rpdf,
+                                             float *__restrict__ T) {
   const uint32_t s_r = (uint32_t)(c >> 32);
   const uint32_t mc = (uint32_t)c;
   const uint32_t m_r = mc >> 6, a = mc & 63u;
@@ -993,17 +1000,26 @@ __global__ void k_poses(const unsigned long long *__restrict__ codes, int64_t P,
     ppfg::pose_from_frames(Tm, Ts, a, out);
   }
 #pragma unroll
-  for (int k = 0; k < 16; ++k) T[i * 16 + k] = out[k];
+  for (int k = 0; k < 16; ++k) T[k] = out[k];
+}
+
+__global__ void k_poses(const unsigned long long *__restrict__ codes, int64_t P,
+                        const float *__restrict__ mframe, const float *__restrict__ sframe,
+                        uint32_t rpdf, float *__restrict__ T) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  pose_of_code(codes[i], mframe, sframe, rpdf, T + i * 16);
 }
 
 // ------------------------------------------------------------ clustering
-__global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long long *__restrict__ codes,
-                               const uint32_t *__restrict__ cnt, const float *__restrict__ weights,
-                               int64_t P, float d_dist, float *__restrict__ trans,
-                               float *__restrict__ quats, float *__restrict__ w,
-                               uint64_t *__restrict__ words, uint32_t *__restrict__ adj) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P) return;
+// Translation, quaternion, weighted count and the 26 neighbour-cell hashes of
+// pose i (mat2transquat + trans2idx, kernel.cu:647-722); returns its cell hash.
+__device__ __forceinline__ uint32_t cluster_prep_one(int64_t i, const float *__restrict__ T,
+                                                     const unsigned long long *__restrict__ codes,
+                                                     const uint32_t *__restrict__ cnt,
+                                                     const float *__restrict__ weights, float d_dist,
+                                                     float *__restrict__ trans, float *__restrict__ quats,
+                                                     float *__restrict__ w, uint32_t *__restrict__ adj) {
   float M[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) M[k] = T[i * 16 + k];
@@ -1018,13 +1034,24 @@ __global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long 
   w[i] = (weights ? weights[m_r] : 1.0f) * (float)cnt[i];
   int32_t c[3];
   ppfg::cell_of(M[3], M[7], M[11], d_dist, c);
-  words[i] = ((uint64_t)ppfm::cell_hash(c[0], c[1], c[2]) << 32) | (uint64_t)i;
   int cc = 0;
   for (int a = -1; a < 2; ++a)
     for (int b = -1; b < 2; ++b)
       for (int k = -1; k < 2; ++k, ++cc)
         adj[27 * i + cc] = (a == 0 && b == 0 && k == 0) ? 0u
                                                          : ppfm::cell_hash(c[0] + a, c[1] + b, c[2] + k);
+  return ppfm::cell_hash(c[0], c[1], c[2]);
+}
+
+__global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long long *__restrict__ codes,
+                               const uint32_t *__restrict__ cnt, const float *__restrict__ weights,
+                               int64_t P, float d_dist, float *__restrict__ trans,
+                               float *__restrict__ quats, float *__restrict__ w,
+                               uint64_t *__restrict__ words, uint32_t *__restrict__ adj) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const uint32_t h = cluster_prep_one(i, T, codes, cnt, weights, d_dist, trans, quats, w, adj);
+  words[i] = ((uint64_t)h << 32) | (uint64_t)i;
 }
 
 // Clustering blocks: CL_PER poses per thread, CL_CHUNK poses per block.  One
@@ -1055,13 +1082,7 @@ __device__ __forceinline__ uint32_t spread4(uint32_t v) {  // 8 bits -> every 4t
   return v;
 }
 
-__global__ void k_cluster_order_keys(const float *__restrict__ quats, const uint64_t *__restrict__ words,
-                                     int64_t P, uint64_t *__restrict__ okeys,
-                                     uint32_t *__restrict__ ovals) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P) return;
-  const uint64_t wd = words[k];  // sorted (cell hash, pose)
-  const uint32_t i = (uint32_t)wd;
+__device__ __forceinline__ uint32_t quat_morton(const float *__restrict__ quats, uint32_t i) {
   float q[4], n2 = 0.0f;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -1077,17 +1098,46 @@ __global__ void k_cluster_order_keys(const float *__restrict__ quats, const uint
     const uint32_t b = (uint32_t)((x + 1.0f) * 127.5f);
     m |= spread4(b) << c;
   }
-  okeys[k] = (wd & 0xffffffff00000000ull) | (uint64_t)m;
+  return m;
+}
+
+__global__ void k_cluster_order_keys(const float *__restrict__ quats, const uint64_t *__restrict__ words,
+                                     int64_t P, uint64_t *__restrict__ okeys,
+                                     uint32_t *__restrict__ ovals) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  const uint64_t wd = words[k];  // sorted (cell hash, pose)
+  const uint32_t i = (uint32_t)wd;
+  okeys[k] = (wd & 0xffffffff00000000ull) | (uint64_t)quat_morton(quats, i);
   ovals[k] = i;
 }
 
 // Estimated cost of every clustering chunk: its poses times the members of
 // its cell's 26 neighbour buckets.  Writes (~cost << 32 | chunk), so that an
 // ascending sort launches the heaviest chunks first.
+// Cells of a batch of jobs: ukeys holds each job's cell hashes, ascending,
+// in the range [job_ub[j], job_ub[j+1]) of job j = ujob[u]; a pose's
+// neighbour cells are looked up in its own job's range.  Without a batch
+// (ujob NULL) the range is the whole array.
+__device__ __forceinline__ void cell_range(const uint32_t *__restrict__ ujob,
+                                           const uint32_t *__restrict__ job_ub, int64_t U, int64_t u,
+                                           int64_t &lo, int64_t &hi, uint32_t &job) {
+  if (ujob) {
+    job = ujob[u];
+    lo = job_ub[job];
+    hi = job_ub[job + 1];
+  } else {
+    job = 0;
+    lo = 0;
+    hi = U;
+  }
+}
+
 __global__ void k_cluster_chunk_cost(const uint32_t *__restrict__ chunk_start, int64_t U,
                                      uint32_t n_chunks, const uint32_t *__restrict__ ukeys,
                                      const uint32_t *__restrict__ ufirst, const uint32_t *__restrict__ ucount,
                                      const uint32_t *__restrict__ centers, const uint32_t *__restrict__ adj,
+                                     const uint32_t *__restrict__ ujob, const uint32_t *__restrict__ job_ub,
                                      uint64_t *__restrict__ keys) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n_chunks) return;
@@ -1102,11 +1152,14 @@ __global__ void k_cluster_chunk_cost(const uint32_t *__restrict__ chunk_start, i
   const uint32_t c = b - chunk_start[lo], n = ucount[lo];
   const uint32_t poses = min((uint32_t)CL_CHUNK, n - c * (uint32_t)CL_CHUNK);
   const uint32_t i0 = centers[ufirst[lo]];
+  int64_t r0, r1;
+  uint32_t job;
+  cell_range(ujob, job_ub, U, lo, r0, r1, job);
   uint64_t nb = 0;
   for (int q = 0; q < 27; ++q) {
     const uint32_t h = adj[27 * (size_t)i0 + q];
     if (h == 0u) continue;
-    int64_t a = 0, e = U;
+    int64_t a = r0, e = r1;
     while (a < e) {
       const int64_t mid = (a + e) >> 1;
       if (ukeys[mid] < h)
@@ -1114,7 +1167,7 @@ __global__ void k_cluster_chunk_cost(const uint32_t *__restrict__ chunk_start, i
       else
         e = mid;
     }
-    if (a < U && ukeys[a] == h) nb += ucount[a];
+    if (a < r1 && ukeys[a] == h) nb += ucount[a];
   }
   const uint64_t cost = min((uint64_t)poses * nb, (uint64_t)0xffffffffu);
   keys[b] = ((uint64_t)(0xffffffffu - (uint32_t)cost) << 32) | b;
@@ -1161,9 +1214,10 @@ __global__ void k_cluster_gather(const uint32_t *__restrict__ members, int64_t P
 using ppfc::ClPose;
 using ppfc::ClWave;
 
-__device__ __forceinline__ int64_t cell_lower_bound(const uint32_t *__restrict__ ukeys, int64_t U,
-                                                    uint32_t h) {
-  int64_t lo = 0, hi = U;
+// The cell of hash h among ukeys[r0, r1) (ascending), or -1.
+__device__ __forceinline__ int64_t cell_lower_bound(const uint32_t *__restrict__ ukeys, int64_t r0,
+                                                    int64_t r1, uint32_t h) {
+  int64_t lo = r0, hi = r1;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
     if (ukeys[mid] < h)
@@ -1171,7 +1225,7 @@ __device__ __forceinline__ int64_t cell_lower_bound(const uint32_t *__restrict__
     else
       hi = mid;
   }
-  return (lo < U && ukeys[lo] == h) ? lo : -1;
+  return (lo < r1 && ukeys[lo] == h) ? lo : -1;
 }
 
 __device__ __forceinline__ void load_pose(ClPose &P, const float *__restrict__ trans,
@@ -1233,7 +1287,9 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
     const uint32_t *__restrict__ ucount, int64_t U, const uint32_t *__restrict__ chunk_start,
     const uint32_t *__restrict__ centers, const uint64_t *__restrict__ order, float s_max, float d_dist,
     float *__restrict__ vc_out, float *__restrict__ trans_out,
-    unsigned long long *__restrict__ n_tests, unsigned long long *__restrict__ n_mixed) {
+    unsigned long long *__restrict__ n_tests, unsigned long long *__restrict__ n_mixed,
+    const uint32_t *__restrict__ ujob, const uint32_t *__restrict__ job_ub,
+    const float2 *__restrict__ job_dist) {
   __shared__ float4 s_q[CL_BLOCK];
   __shared__ float4 s_t[CL_BLOCK];
   __shared__ int s_u, s_c, s_uniform;
@@ -1257,6 +1313,18 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   }
   __syncthreads();
   const int u = s_u;
+  // a batch of jobs: this chunk's job -- its cells' range, d_dist and s_max
+  // (job_dist[j] = (s_max, d_dist)), and its own statistics counters
+  int64_t r0, r1;
+  uint32_t job;
+  cell_range(ujob, job_ub, U, u, r0, r1, job);
+  if (ujob) {
+    const float2 jd = job_dist[job];
+    s_max = jd.x;
+    d_dist = jd.y;
+    n_tests += job;
+    n_mixed += job;
+  }
   const uint32_t f = ufirst[u], n = ucount[u];
   const uint32_t base = (uint32_t)s_c * CL_CHUNK;
   const uint32_t i0 = centers[f + base];
@@ -1288,7 +1356,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
     for (int blk = 0; blk < 27; ++blk) {
       const uint32_t h = s_adj[blk];
       if (h == 0u) continue;
-      if (tid == 0) s_bucket = cell_lower_bound(ukeys, U, h);
+      if (tid == 0) s_bucket = cell_lower_bound(ukeys, r0, r1, h);
       __syncthreads();
       const int64_t bu = s_bucket;
       __syncthreads();
@@ -1334,7 +1402,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
       for (int blk = 0; blk < 27; ++blk) {
         const uint32_t h = adj[27 * (size_t)idx[r] + blk];
         if (h == 0u) continue;
-        const int64_t bu = cell_lower_bound(ukeys, U, h);
+        const int64_t bu = cell_lower_bound(ukeys, r0, r1, h);
         if (bu < 0) continue;
         const uint32_t bf = ufirst[bu], bn = ucount[bu];
         tests += bn;
@@ -1390,6 +1458,180 @@ __global__ __launch_bounds__(256) void k_argmax(const float *__restrict__ v, int
     for (int k = 1; k < 256 / WAVE; ++k) best = sk[k] > best ? sk[k] : best;
     atomicMax(out, best);
   }
+}
+
+// ------------------------------------------------- batched finishing
+// Threshold/order, poses and clustering of every job of a batch, one launch
+// per stage.  Job j's elements are a contiguous range [begin[j], begin[j+1])
+// of each stage's arrays (candidates, then kept votes), jobs in order, so
+// every per-job result is the sub-range the plain path computes alone.
+
+// The job of element i: the last j with begin[j] <= i (begin ascending, nj
+// jobs; an empty job shares its begin with the next one and is skipped).
+__device__ __forceinline__ uint32_t job_of(const uint64_t *__restrict__ begin, int nj, uint64_t i) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return (uint32_t)lo;
+}
+
+// select flags: candidate count above the job's fl(thr * fl(max)) (model.cu:155-170)
+__global__ void k_bsel_flags(const FinJob *__restrict__ jobs, const uint64_t *__restrict__ cbeg, int nj,
+                             uint64_t C, uint32_t *__restrict__ flags) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C) return;
+  const uint32_t j = job_of(cbeg, nj, i);
+  flags[i] = ((float)jobs[j].cand_count[i - cbeg[j]] > jobs[j].thr_f) ? 1u : 0u;
+}
+
+// kbeg[j] = the first kept vote of job j (j = 0..nj)
+__global__ void k_bsel_begins(const uint64_t *__restrict__ cbeg, int nj, uint64_t C,
+                              const uint32_t *__restrict__ pos, const uint32_t *__restrict__ total,
+                              uint64_t *__restrict__ kbeg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > nj) return;
+  kbeg[j] = cbeg[j] < C ? pos[cbeg[j]] : *total;
+}
+
+// kept candidates as one sort key: job, then count descending, then code
+// ascending (the plain path's composite key below the job bits)
+__global__ void k_bsel_scatter(const FinJob *__restrict__ jobs, const uint64_t *__restrict__ cbeg, int nj,
+                               uint64_t C, const uint32_t *__restrict__ flags,
+                               const uint32_t *__restrict__ pos, uint32_t max_count, int sh_m, int sh_c,
+                               int sh_j, uint64_t *__restrict__ keys) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C || !flags[i]) return;
+  const uint32_t j = job_of(cbeg, nj, i);
+  const uint64_t l = i - cbeg[j];
+  const unsigned long long c = jobs[j].cand_code[l];
+  const uint64_t s_r = c >> 32, mc = c & 0xffffffffull;
+  keys[pos[i]] = ((uint64_t)j << sh_j) | ((uint64_t)(max_count - jobs[j].cand_count[l]) << sh_c) |
+                 (s_r << sh_m) | mc;
+}
+
+__global__ void k_bsel_decode(const uint64_t *__restrict__ keys, int64_t P, uint32_t max_count, int sh_m,
+                              int sh_c, int bc, unsigned long long *__restrict__ out_code,
+                              uint32_t *__restrict__ out_cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const uint64_t k = keys[i];
+  const uint64_t mc = k & ((1ull << sh_m) - 1ull);
+  const uint64_t s_r = (k & ((1ull << sh_c) - 1ull)) >> sh_m;
+  out_code[i] = (s_r << 32) | mc;
+  out_cnt[i] = max_count - (uint32_t)((k >> sh_c) & ((1ull << bc) - 1ull));
+}
+
+__global__ void k_bposes(const unsigned long long *__restrict__ codes, int64_t P,
+                         const uint64_t *__restrict__ kbeg, int nj, const FinJob *__restrict__ jobs,
+                         float *__restrict__ T) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const FinJob &J = jobs[job_of(kbeg, nj, (uint64_t)i)];
+  pose_of_code(codes[i], J.mframe, J.sframe, J.rpdf, T + i * 16);
+}
+
+// words: (job, cell hash) above the pose index (bg bits)
+__global__ void k_bcluster_prep(const float *__restrict__ T, const unsigned long long *__restrict__ codes,
+                                const uint32_t *__restrict__ cnt, int64_t P, const uint64_t *__restrict__ kbeg,
+                                int nj, const FinJob *__restrict__ jobs, int bg, float *__restrict__ trans,
+                                float *__restrict__ quats, float *__restrict__ w,
+                                uint64_t *__restrict__ words, uint32_t *__restrict__ adj) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const uint32_t j = job_of(kbeg, nj, (uint64_t)i);
+  const uint32_t h = cluster_prep_one(i, T, codes, cnt, jobs[j].weights, jobs[j].d_dist, trans, quats, w, adj);
+  words[i] = (((uint64_t)j << 32 | (uint64_t)h) << bg) | (uint64_t)i;
+}
+
+// job_ub[j] = the first cell of job j among the unique cells (j = 0..nj)
+__global__ void k_bcell_begins(const uint32_t *__restrict__ ujob, int64_t U, int nj,
+                               uint32_t *__restrict__ job_ub) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > nj) return;
+  int64_t lo = 0, hi = U;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ujob[mid] < (uint32_t)j)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  job_ub[j] = (uint32_t)lo;
+}
+
+// chunk order inside each cell: (cell index u, rotation Morton code)
+__global__ void k_bcluster_order_keys(const float *__restrict__ quats, const uint32_t *__restrict__ members,
+                                      int64_t P, const uint32_t *__restrict__ ufirst, int64_t U,
+                                      uint64_t *__restrict__ okeys, uint32_t *__restrict__ ovals) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  int64_t lo = 0, hi = U - 1;  // last u with ufirst[u] <= k
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (ufirst[mid] <= (uint32_t)k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const uint32_t i = members[k];
+  okeys[k] = ((uint64_t)lo << 32) | (uint64_t)quat_morton(quats, i);
+  ovals[k] = i;
+}
+
+// first maximum of each job's vote counts (index within the job)
+__global__ __launch_bounds__(256) void k_bargmax(const float *__restrict__ v, int64_t P,
+                                                 const uint64_t *__restrict__ kbeg, int nj,
+                                                 unsigned long long *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < P;
+  const uint32_t j = valid ? job_of(kbeg, nj, (uint64_t)i) : 0xffffffffu;
+  unsigned long long key = valid ? argmax_key(v[i], (uint32_t)((uint64_t)i - kbeg[j])) : 0ull;
+  const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)j);
+  if (__ballot(valid && j != j0) == 0ull) {
+    // the wave's elements belong to one job: one atomic
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+      const unsigned long long x = __shfl_xor(key, o, WAVE);
+      key = x > key ? x : key;
+    }
+    if (lane_id() == 0 && j0 != 0xffffffffu) atomicMax(out + j0, key);
+  } else if (valid) {
+    atomicMax(out + j, key);
+  }
+}
+
+// per job: final pose (T of the first maximum with the clustering's
+// translation, ppf.cu:78-93), its index and the clustering statistics
+__global__ void k_bfinal(const uint64_t *__restrict__ kbeg, int nj, const unsigned long long *__restrict__ amax,
+                         const float *__restrict__ T, const float *__restrict__ trans_out,
+                         const unsigned long long *__restrict__ jtests,
+                         const unsigned long long *__restrict__ jmixed, uint32_t *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  uint32_t *o = out + 24 * (size_t)j;
+  const uint64_t b = kbeg[j], P = kbeg[j + 1] - b;
+  uint32_t mi = 0xffffffffu;
+  if (P > 0) {
+    mi = 0xffffffffu - (uint32_t)(amax[j] & 0xffffffffull);
+    const float *t = T + (b + mi) * 16;
+    for (int k = 0; k < 16; ++k) o[k] = ppfm::fbits(t[k]);
+    o[3] = ppfm::fbits(trans_out[(b + mi) * 3]);
+    o[7] = ppfm::fbits(trans_out[(b + mi) * 3 + 1]);
+    o[11] = ppfm::fbits(trans_out[(b + mi) * 3 + 2]);
+  } else {
+    for (int k = 0; k < 16; ++k) o[k] = 0u;
+  }
+  o[16] = mi;
+  o[17] = 0u;
+  o[18] = (uint32_t)jtests[j];
+  o[19] = (uint32_t)(jtests[j] >> 32);
+  o[20] = (uint32_t)jmixed[j];
+  o[21] = (uint32_t)(jmixed[j] >> 32);
 }
 
 // ---------------------------------------------------------- diagnostics
@@ -1477,7 +1719,7 @@ int cloud_pair_keys(ppf_ctx ctx, const CloudDev &c, float d_dist, uint32_t rpdf,
   return 0;
 }
 
-int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out) {
+int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out, int shift) {
   *U_out = 0;
   if (n <= 0) return 0;
   const int64_t chunks = (n + B256 - 1) / B256;
@@ -1487,7 +1729,7 @@ int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out) 
   uint32_t *counts = ctx->tmp_c.as<uint32_t>();
   uint32_t *offs = counts + G;
   uint32_t *total = offs + G;
-  hipLaunchKernelGGL(k_unique_count, dim3(G), dim3(B256), 0, ctx->stream, words, n, per, counts);
+  hipLaunchKernelGGL(k_unique_count, dim3(G), dim3(B256), 0, ctx->stream, words, n, per, shift, counts);
   PPF_TRY(exclusive_scan_u32(counts, offs, G, ctx->tmp_d, ctx->stream, total));
   PPF_HIP(hipGetLastError());
   uint32_t U = 0;
@@ -1498,14 +1740,14 @@ int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out) 
 }
 
 int unique_emit(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t U, uint32_t *hkeys,
-                uint32_t *hfirst, uint32_t *hcount, uint32_t *low) {
+                uint32_t *hfirst, uint32_t *hcount, uint32_t *low, int shift, uint32_t *hjob) {
   if (n <= 0) return 0;
   const int64_t chunks = (n + B256 - 1) / B256;
   const int G = (int)std::min<int64_t>(chunks, UQ_MAX_BLOCKS);
   const int64_t per = (chunks + G - 1) / G * B256;
   const uint32_t *offs = ctx->tmp_c.as<uint32_t>() + G;  // left by unique_count
-  hipLaunchKernelGGL(k_unique_emit, dim3(G), dim3(B256), 0, ctx->stream, words, n, per, offs, hkeys,
-                     hfirst, low);
+  hipLaunchKernelGGL(k_unique_emit, dim3(G), dim3(B256), 0, ctx->stream, words, n, per, shift, offs,
+                     hkeys, hfirst, low, hjob);
   if (U > 0)
     hipLaunchKernelGGL(k_unique_counts, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, hfirst, U, n,
                        hcount);
@@ -1796,7 +2038,7 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
     PPF_TRY(ctx->tmp_c.ensure((size_t)n_items * 8));
     uint64_t *order = ctx->tmp_c.as<uint64_t>();
     hipLaunchKernelGGL(k_cluster_chunk_cost, dim3(nblk(n_items, 256)), dim3(256), 0, ctx->stream, cstart, U,
-                       n_items, ukeys, ufirst, ucount, centers, adj, order);
+                       n_items, ukeys, ufirst, ucount, centers, adj, nullptr, nullptr, order);
     PPF_HIP(hipGetLastError());
     PPF_TRY(radix_sort_u64(order, nullptr, n_items, 32, 64, ctx->sort, ctx->stream));
     auto kern = use_avg ? (use_l1 ? k_cluster_tiled<1, 1> : k_cluster_tiled<0, 1>)
@@ -1804,7 +2046,7 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
     PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(kern, dim3(n_items), dim3(CL_BLOCK), 0, ctx->stream, trans, quats, cq, ct, adj,
                        ukeys, ufirst, ucount, U, cstart, centers, order, sqrt_below_max(d_dist), d_dist,
-                       vc, trans_out, ntest, nmixed);
+                       vc, trans_out, ntest, nmixed, nullptr, nullptr, nullptr);
     PPF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   }
   hipLaunchKernelGGL(k_argmax, dim3((unsigned)std::min<int64_t>(nblk(P, 256), 1024)), dim3(256), 0,
@@ -1820,6 +2062,213 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
     float ms = 0.0f;
     PPF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *kernel_ms = ms;
+  }
+  return 0;
+}
+
+bool finish_batch_fits(int nj, uint64_t total_cand, uint32_t max_count, int64_t max_ns, int64_t max_nm) {
+  if (nj < 1 || total_cand >= ((uint64_t)1 << 31)) return false;
+  const int bj = std::max(1, bitlen((uint64_t)(nj - 1)));
+  const int bs = std::max(1, bitlen((uint64_t)(max_ns > 0 ? max_ns - 1 : 0)));
+  const int bm = bitlen((uint64_t)(max_nm > 0 ? max_nm - 1 : 0));
+  const int bc = std::max(1, bitlen(max_count));
+  const int bg = std::max(1, bitlen(total_cand));
+  return bj + bc + bs + bm + 6 <= 64 && bj + 32 + bg <= 64;
+}
+
+int finish_batch(ppf_ctx ctx, const std::vector<FinJob> &jobs, const std::vector<uint32_t> &n_cand,
+                 uint32_t max_count, int64_t max_ns, int64_t max_nm, bool use_l1, bool use_avg,
+                 const std::function<int(int64_t, FinishArrays &)> &alloc, FinishOut &out) {
+  hipStream_t st = ctx->stream;
+  const int nj = (int)jobs.size();
+  std::vector<uint64_t> cbeg((size_t)nj + 1, 0);
+  for (int j = 0; j < nj; ++j) cbeg[j + 1] = cbeg[j] + n_cand[j];
+  const uint64_t C = cbeg[nj];
+  if (!finish_batch_fits(nj, C, max_count, max_ns, max_nm)) {
+    set_error("finish_batch: the batch's sort keys exceed 64 bits");
+    return PPF_ERR_ARG;
+  }
+  const int bj = std::max(1, bitlen((uint64_t)(nj - 1)));
+  const int bs = std::max(1, bitlen((uint64_t)(max_ns > 0 ? max_ns - 1 : 0)));
+  const int bm = bitlen((uint64_t)(max_nm > 0 ? max_nm - 1 : 0));
+  const int bc = std::max(1, bitlen(max_count));
+  const int bg = std::max(1, bitlen(C));
+  const int sh_m = 6 + bm, sh_c = 6 + bm + bs, sh_j = 6 + bm + bs + bc;
+
+  // job table, candidate begins and per-job constants, then device-written
+  // kept begins, cell begins, argmax keys, statistics and final poses
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_cbeg = al(sizeof(FinJob) * nj), o_dist = o_cbeg + al(8 * ((size_t)nj + 1));
+  const size_t o_kbeg = o_dist + al(8 * (size_t)nj), o_jub = o_kbeg + al(8 * ((size_t)nj + 1));
+  const size_t o_amax = o_jub + al(4 * ((size_t)nj + 1)), o_jt = o_amax + al(8 * (size_t)nj);
+  const size_t o_jm = o_jt + al(8 * (size_t)nj), o_out = o_jm + al(8 * (size_t)nj);
+  const size_t meta_bytes = o_out + al(96 * (size_t)nj);
+  PPF_TRY(ctx->fin_meta.ensure(meta_bytes));
+  char *meta = ctx->fin_meta.as<char>();
+  const FinJob *d_jobs = reinterpret_cast<const FinJob *>(meta);
+  const uint64_t *d_cbeg = reinterpret_cast<const uint64_t *>(meta + o_cbeg);
+  const float2 *d_dist = reinterpret_cast<const float2 *>(meta + o_dist);
+  uint64_t *d_kbeg = reinterpret_cast<uint64_t *>(meta + o_kbeg);
+  uint32_t *d_jub = reinterpret_cast<uint32_t *>(meta + o_jub);
+  unsigned long long *d_amax = reinterpret_cast<unsigned long long *>(meta + o_amax);
+  unsigned long long *d_jt = reinterpret_cast<unsigned long long *>(meta + o_jt);
+  unsigned long long *d_jm = reinterpret_cast<unsigned long long *>(meta + o_jm);
+  uint32_t *d_out = reinterpret_cast<uint32_t *>(meta + o_out);
+  {
+    std::vector<char> h(o_kbeg, 0);
+    std::memcpy(h.data(), jobs.data(), sizeof(FinJob) * nj);
+    std::memcpy(h.data() + o_cbeg, cbeg.data(), 8 * ((size_t)nj + 1));
+    float2 *dd = reinterpret_cast<float2 *>(h.data() + o_dist);
+    for (int j = 0; j < nj; ++j) dd[j] = make_float2(sqrt_below_max(jobs[j].d_dist), jobs[j].d_dist);
+    PPF_HIP(hipMemcpyAsync(meta, h.data(), o_kbeg, hipMemcpyHostToDevice, st));
+    PPF_HIP(hipMemsetAsync(meta + o_amax, 0, o_out - o_amax, st));
+  }
+  PPF_HIP(hipEventRecord(ctx->ev4, st));
+
+  // ---- threshold and order (model.cu:148-170)
+  PPF_TRY(ctx->fin_flags.ensure((size_t)C * 8 + 16));
+  uint32_t *flags = ctx->fin_flags.as<uint32_t>();
+  uint32_t *pos = flags + C;
+  uint32_t *total = pos + C;
+  if (C > 0) {
+    hipLaunchKernelGGL(k_bsel_flags, dim3(nblk((int64_t)C, 256)), dim3(256), 0, st, d_jobs, d_cbeg, nj, C,
+                       flags);
+    PPF_TRY(exclusive_scan_u32(flags, pos, (int64_t)C, ctx->fin_scan, st, total));
+  } else {
+    PPF_HIP(hipMemsetAsync(total, 0, 4, st));
+  }
+  hipLaunchKernelGGL(k_bsel_begins, dim3(nblk(nj + 1, 256)), dim3(256), 0, st, d_cbeg, nj, C, pos, total,
+                     d_kbeg);
+  PPF_HIP(hipGetLastError());
+  out.kbeg.assign((size_t)nj + 1, 0);
+  PPF_HIP(hipMemcpyAsync(out.kbeg.data(), d_kbeg, 8 * ((size_t)nj + 1), hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipStreamSynchronize(st));
+  const int64_t P = (int64_t)out.kbeg[nj];
+  FinishArrays A{};
+  PPF_TRY(alloc(P, A));
+  out.pose.assign((size_t)nj * 16, 0.0f);
+  out.max_idx.assign((size_t)nj, -1);
+  out.tests.assign((size_t)nj, 0);
+  out.mixed.assign((size_t)nj, 0);
+  if (P == 0) {
+    PPF_HIP(hipEventRecord(ctx->ev5, st));
+    PPF_HIP(hipEventRecord(ctx->ev6, st));
+    PPF_HIP(hipEventRecord(ctx->ev3, st));
+    PPF_HIP(hipEventSynchronize(ctx->ev3));
+    return 0;
+  }
+  PPF_TRY(ctx->fin_keys.ensure((size_t)P * 8));
+  uint64_t *keys = ctx->fin_keys.as<uint64_t>();
+  hipLaunchKernelGGL(k_bsel_scatter, dim3(nblk((int64_t)C, 256)), dim3(256), 0, st, d_jobs, d_cbeg, nj, C,
+                     flags, pos, max_count, sh_m, sh_c, sh_j, keys);
+  PPF_HIP(hipGetLastError());
+  PPF_TRY(radix_sort_u64(keys, nullptr, P, 0, sh_j + bj, ctx->sort, st));
+  hipLaunchKernelGGL(k_bsel_decode, dim3(nblk(P, 256)), dim3(256), 0, st, keys, P, max_count, sh_m, sh_c, bc,
+                     A.codes, A.counts);
+  PPF_HIP(hipGetLastError());
+  PPF_HIP(hipEventRecord(ctx->ev5, st));
+
+  // ---- poses (model.cu:191-200)
+  hipLaunchKernelGGL(k_bposes, dim3(nblk(P, 128)), dim3(128), 0, st, A.codes, P, d_kbeg, nj, d_jobs, A.T);
+  PPF_HIP(hipGetLastError());
+  PPF_HIP(hipEventRecord(ctx->ev6, st));
+
+  // ---- spatial-hash clustering (model.cu:202-244): cells of (job, hash)
+  const size_t bytes = (size_t)P * (32 + 16 + 12 + 4 + 108 + 16 + 8) + 256;
+  PPF_TRY(ctx->fin_cl.ensure(bytes));
+  char *p = ctx->fin_cl.as<char>();
+  float4 *cq = reinterpret_cast<float4 *>(p);
+  p += (size_t)P * 16;
+  float4 *ct = reinterpret_cast<float4 *>(p);
+  p += (size_t)P * 16;
+  uint64_t *okeys = reinterpret_cast<uint64_t *>(p);
+  p += (size_t)P * 8;
+  uint64_t *words = reinterpret_cast<uint64_t *>(p);
+  p += (size_t)P * 8;
+  float *trans = reinterpret_cast<float *>(p);
+  p += (size_t)P * 12;
+  float *w = reinterpret_cast<float *>(p);
+  p += (size_t)P * 4;
+  uint32_t *adj = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 108;
+  uint32_t *ukeys = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *ufirst = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *ucount = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *ujob = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *members = reinterpret_cast<uint32_t *>(p);
+  p += (size_t)P * 4;
+  uint32_t *centers = reinterpret_cast<uint32_t *>(p);
+  hipLaunchKernelGGL(k_bcluster_prep, dim3(nblk(P, 128)), dim3(128), 0, st, A.T, A.codes, A.counts, P, d_kbeg,
+                     nj, d_jobs, bg, trans, A.quats, w, words, adj);
+  PPF_HIP(hipGetLastError());
+  // stable on the key bits: the pose index below them is ascending already
+  PPF_TRY(radix_sort_u64(words, nullptr, P, bg, bg + 32 + bj, ctx->sort, st));
+  int64_t U = 0;
+  PPF_TRY(unique_count(ctx, words, P, &U, bg));
+  PPF_TRY(unique_emit(ctx, words, P, U, ukeys, ufirst, ucount, members, bg, ujob));
+  hipLaunchKernelGGL(k_bcell_begins, dim3(nblk(nj + 1, 256)), dim3(256), 0, st, ujob, U, nj, d_jub);
+  hipLaunchKernelGGL(k_bcluster_order_keys, dim3(nblk(P, 256)), dim3(256), 0, st, A.quats, members, P, ufirst,
+                     U, okeys, centers);
+  PPF_HIP(hipGetLastError());
+  PPF_TRY(radix_sort_u64(okeys, centers, P, 0, 32 + std::max(1, bitlen((uint64_t)U)), ctx->sort, st));
+  hipLaunchKernelGGL(k_cluster_gather, dim3(nblk(P, 256)), dim3(256), 0, st, members, P, trans, A.quats, w,
+                     cq, ct);
+  PPF_HIP(hipGetLastError());
+  PPF_TRY(ctx->fin_chunk.ensure((size_t)(U + 1) * 8 + 64));
+  uint32_t *nch = ctx->fin_chunk.as<uint32_t>();
+  uint32_t *cstart = nch + U + 1;
+  uint32_t *ctot = cstart + U;
+  hipLaunchKernelGGL(k_cluster_chunks, dim3(nblk(U, 256)), dim3(256), 0, st, ucount, U, nch);
+  PPF_TRY(exclusive_scan_u32(nch, cstart, U, ctx->fin_scan, st, ctot));
+  uint32_t n_items = 0;
+  PPF_HIP(hipMemcpyAsync(&n_items, ctot, 4, hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipStreamSynchronize(st));
+  if (n_items > 0) {
+    PPF_TRY(ctx->fin_order.ensure((size_t)n_items * 8));
+    uint64_t *order = ctx->fin_order.as<uint64_t>();
+    hipLaunchKernelGGL(k_cluster_chunk_cost, dim3(nblk(n_items, 256)), dim3(256), 0, st, cstart, U, n_items,
+                       ukeys, ufirst, ucount, centers, adj, ujob, d_jub, order);
+    PPF_HIP(hipGetLastError());
+    PPF_TRY(radix_sort_u64(order, nullptr, n_items, 32, 64, ctx->sort, st));
+    auto kern = use_avg ? (use_l1 ? k_cluster_tiled<1, 1> : k_cluster_tiled<0, 1>)
+                        : (use_l1 ? k_cluster_tiled<1, 0> : k_cluster_tiled<0, 0>);
+    PPF_HIP(hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(kern, dim3(n_items), dim3(CL_BLOCK), 0, st, trans, A.quats, cq, ct, adj, ukeys, ufirst,
+                       ucount, U, cstart, centers, order, 0.0f, 0.0f, A.vc, A.trans, d_jt, d_jm, ujob, d_jub,
+                       d_dist);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipEventRecord(ctx->ev1, st));
+  }
+  hipLaunchKernelGGL(k_bargmax, dim3(nblk(P, 256)), dim3(256), 0, st, A.vc, P, d_kbeg, nj, d_amax);
+  hipLaunchKernelGGL(k_bfinal, dim3(nblk(nj, 64)), dim3(64), 0, st, d_kbeg, nj, d_amax, A.T, A.trans, d_jt, d_jm,
+                     d_out);
+  PPF_HIP(hipGetLastError());
+  std::vector<uint32_t> h((size_t)nj * 24);
+  PPF_HIP(hipMemcpyAsync(h.data(), d_out, h.size() * 4, hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipEventRecord(ctx->ev3, st));
+  PPF_HIP(hipEventSynchronize(ctx->ev3));
+  for (int j = 0; j < nj; ++j) {
+    const uint32_t *o = h.data() + 24 * (size_t)j;
+    std::memcpy(out.pose.data() + 16 * (size_t)j, o, 64);
+    out.max_idx[j] = o[16] == 0xffffffffu ? -1 : (int64_t)o[16];
+    out.tests[j] = (int64_t)(((uint64_t)o[19] << 32) | o[18]);
+    out.mixed[j] = (int64_t)(((uint64_t)o[21] << 32) | o[20]);
+  }
+  float ms = 0.0f;
+  PPF_HIP(hipEventElapsedTime(&ms, ctx->ev4, ctx->ev5));
+  out.select_ms = ms;
+  PPF_HIP(hipEventElapsedTime(&ms, ctx->ev5, ctx->ev6));
+  out.pose_ms = ms;
+  PPF_HIP(hipEventElapsedTime(&ms, ctx->ev6, ctx->ev3));
+  out.cluster_ms = ms;
+  out.cluster_kernel_ms = 0.0f;
+  if (n_items > 0) {
+    PPF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out.cluster_kernel_ms = ms;
   }
   return 0;
 }

@@ -45,5 +45,14 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
 // Finish every job on up to n_lanes lanes; results[k] belongs to jobs[k].
 int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,
                  std::vector<std::unique_ptr<ppf_result_s>> &results);
+// Whether batch_finish_fused can take the jobs (GPU clustering, sort keys of
+// 64 bits).
+bool batch_fused_ok(const std::vector<BatchJob> &jobs, const ppf_lookup_params *p);
+// Finish every job together: one launch per stage (threshold/order, poses,
+// clustering, argmax) over all jobs; results[k] belongs to jobs[k] and is
+// identical to the lane path's.  stages (may be NULL) receives the stage
+// times and fused = 1.
+int batch_finish_fused(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p,
+                       std::vector<std::unique_ptr<ppf_result_s>> &results, ppf_batch_info_t *stages);
 
 }  // namespace ppfi

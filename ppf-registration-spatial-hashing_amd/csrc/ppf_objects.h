@@ -3,6 +3,8 @@
 #pragma once
 
 #include <atomic>
+#include <functional>
+#include <memory>
 #include <vector>
 
 #include "../../include/ppf_amd.h"
@@ -22,6 +24,11 @@ constexpr int VOTE_BLOCK = 1024;
 // queues of undecided votes), in bytes; kernels.hip checks the bound
 constexpr size_t VOTE_LDS_RESERVE = 12288;
 
+// Finishing of a batch: together (one launch per stage over all jobs) when
+// the batch allows it, or job by job on lanes.
+constexpr int32_t PPF_FINISH_AUTO = 0;
+constexpr int32_t PPF_FINISH_LANES = 1;
+
 // A context is one HIP stream with its own scratch buffers on one device.
 // The context a caller creates is a root; batched and served work also run
 // on lanes, contexts that share the root's device and device-memory account
@@ -38,6 +45,8 @@ struct ppf_ctx_s {
   ppfi::SortScratch sort;
   ppfi::DevBuf tmp_a, tmp_b, tmp_c, tmp_d, hits, queue;
   ppfi::DevBuf sel;  // kept votes of the lookup being finished (grows, never shrinks)
+  // scratch of the batched finishing (finish_batch)
+  ppfi::DevBuf fin_meta, fin_flags, fin_keys, fin_cl, fin_chunk, fin_order, fin_scan;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
   std::vector<ppf_ctx_s *> lanes;  // root only
@@ -52,6 +61,8 @@ struct ppf_ctx_s {
   // default): vote blocks per CU, finishing lanes
   int32_t tune_vote_per_cu = 0;
   int32_t tune_lanes = 0;
+  // root only: how batches are finished (PPF_FINISH_*; tuner or ppf_debug_set)
+  int32_t finish_mode = 0;
   // root only: progress reports of plain lookups (ppf_ctx_set_progress)
   ppf_progress_fn progress = nullptr;
   void *progress_user = nullptr;
@@ -84,8 +95,16 @@ struct TBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
   bool arena = false;  // a piece of the current scratch arena (not budgeted, never freed)
+  bool view = false;   // a range of an allocation someone else owns (never freed here)
   int alloc(ppf_ctx c, size_t n);
   void release();
+  // Point at n bytes of another owner's allocation.
+  void set_view(void *p, size_t n) {
+    release();
+    ptr = p;
+    bytes = n;
+    view = true;
+  }
   template <class T>
   T *as() const {
     return static_cast<T *>(ptr);
@@ -94,10 +113,11 @@ struct TBuf {
   TBuf() = default;
   TBuf(const TBuf &) = delete;
   TBuf &operator=(const TBuf &) = delete;
-  TBuf(TBuf &&o) noexcept : ctx(o.ctx), ptr(o.ptr), bytes(o.bytes), arena(o.arena) {
+  TBuf(TBuf &&o) noexcept : ctx(o.ctx), ptr(o.ptr), bytes(o.bytes), arena(o.arena), view(o.view) {
     o.ptr = nullptr;
     o.bytes = 0;
     o.arena = false;
+    o.view = false;
   }
   TBuf &operator=(TBuf &&o) noexcept {
     if (this != &o) {
@@ -106,9 +126,11 @@ struct TBuf {
       ptr = o.ptr;
       bytes = o.bytes;
       arena = o.arena;
+      view = o.view;
       o.ptr = nullptr;
       o.bytes = 0;
       o.arena = false;
+      o.view = false;
     }
     return *this;
   }
@@ -161,6 +183,9 @@ struct ppf_result_s {
   int64_t P = 0;
   int64_t n_m = 0;
   TBuf codes, counts, T, vc, trans, quats;
+  // results of a batch finished together: the arrays above are views of the
+  // batch's allocation, which lives until its last result is destroyed
+  std::shared_ptr<void> store;
   float pose[16] = {0};
   bool cpu = false;
   // CPU clustering: the (up to) three best clusters' averaged poses and votes
@@ -204,10 +229,12 @@ int cloud_pair_keys(ppf_ctx ctx, const CloudDev &c, float d_dist, uint32_t rpdf,
 // sorted words -> number of unique keys U (host value); then unique keys
 // (hkeys), first indices (hfirst), counts (hcount) and the low halves of the
 // words (low, may be NULL).  unique_emit must follow unique_count on the same
-// context with the same words.
-int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out);
+// context with the same words.  The key of a word is word >> shift: hkeys
+// gets its low 32 bits, hjob (may be NULL) the bits above, low the bits below.
+int unique_count(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t *U_out, int shift = 32);
 int unique_emit(ppf_ctx ctx, const uint64_t *words, int64_t n, int64_t U, uint32_t *hkeys,
-                uint32_t *hfirst, uint32_t *hcount, uint32_t *low);
+                uint32_t *hfirst, uint32_t *hcount, uint32_t *low, int shift = 32,
+                uint32_t *hjob = nullptr);
 int build_htable(ppf_ctx ctx, const uint32_t *hkeys, int64_t U, uint32_t *htab, uint32_t hcap,
                  uint32_t hshift);
 int build_tile_offsets(ppf_ctx ctx, const uint32_t *hfirst, const uint32_t *hcount,
@@ -239,6 +266,41 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
                     int64_t *max_idx, int64_t *pair_tests, int64_t *mixed_chunks, double *kernel_ms);
+// One job of a batch for the finishing stages: its candidates (code, count)
+// and threshold, and what its poses and clustering read.
+struct FinJob {
+  const unsigned long long *cand_code;
+  const uint32_t *cand_count;
+  float thr_f;  // fl(vote_count_threshold * fl(max_count))
+  uint32_t rpdf;
+  const float *mframe, *sframe, *weights;
+  float d_dist;
+};
+// Results of the batched finishing: job j owns the kept votes
+// [kbeg[j], kbeg[j+1]) of the output arrays (codes, counts, T, vc, trans,
+// quats), in the order a plain lookup of the job gives them.
+struct FinishOut {
+  std::vector<uint64_t> kbeg;
+  std::vector<float> pose;        // [nj][16] final pose (translation from the clustering)
+  std::vector<int64_t> max_idx;   // first maximum of vc within the job (-1: no kept vote)
+  std::vector<int64_t> tests, mixed;
+  float select_ms = 0, pose_ms = 0, cluster_ms = 0, cluster_kernel_ms = 0;
+};
+// Whether finish_batch can take these jobs: every sort key fits 64 bits.
+bool finish_batch_fits(int nj, uint64_t total_cand, uint32_t max_count, int64_t max_ns, int64_t max_nm);
+// Threshold/order, poses and GPU spatial-hash clustering of all jobs, one
+// launch per stage.  n_cand[j] candidates of job j; max_count = the largest
+// of the jobs' maxima.  alloc(P_total) is called once the kept votes are
+// counted and returns the output arrays (codes u64, counts u32, T 16 f32,
+// vc f32, trans 3 f32, quats 4 f32, each P_total long).
+struct FinishArrays {
+  unsigned long long *codes;
+  uint32_t *counts;
+  float *T, *vc, *trans, *quats;
+};
+int finish_batch(ppf_ctx ctx, const std::vector<FinJob> &jobs, const std::vector<uint32_t> &n_cand,
+                 uint32_t max_count, int64_t max_ns, int64_t max_nm, bool use_l1, bool use_avg,
+                 const std::function<int(int64_t, FinishArrays &)> &alloc, FinishOut &out);
 int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *out, int64_t n);
 int debug_copy(ppf_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n);  // n multiple of 4
 

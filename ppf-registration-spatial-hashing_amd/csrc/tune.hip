@@ -7,7 +7,8 @@
 // the server, which shares the vote path):
 //   * vote blocks per CU (1 or the most the LDS allows, 2): two blocks hide
 //     latency, one leaves the whole CU to a block;
-//   * finishing lanes (1, 2 or 4 host threads with their own streams);
+//   * finishing: all jobs together (one launch per stage), or job by job on
+//     1, 2 or 4 lanes (host threads with their own streams);
 //   * one-sweep radix-sort tile (256 or 512 threads x 8 or 16 keys).
 // None of them changes a result: vote counts are integer atomics, every job
 // is finished by the same stage sequence on whatever lane, and the sort is
@@ -131,20 +132,28 @@ int ppf_batch_tune(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const p
     info->best_sort_ms = best_sort;
   }
 
-  // ---- finishing lanes: the same candidates finished with 1, 2 and 4 lanes
+  // ---- finishing: the same candidates finished together (one launch per
+  // stage over all jobs; lanes = 0) and with 4, 2 and 1 lanes
   double best_finish = 1e300;
   int best_lanes = PPF_BATCH_LANES;
-  for (int lanes : {PPF_BATCH_LANES, 2, 1}) {
+  const bool fused_ok = batch_fused_ok(q, params);
+  for (int lanes : {0, PPF_BATCH_LANES, 2, 1}) {
+    if (lanes == 0 && !fused_ok) continue;
     std::vector<std::unique_ptr<ppf_result_s>> res;
     const auto t0 = std::chrono::steady_clock::now();
-    PPF_TRY(batch_finish(root, q, params, lanes, res));
+    if (lanes == 0)
+      PPF_TRY(batch_finish_fused(root, q, params, res, nullptr));
+    else
+      PPF_TRY(batch_finish(root, q, params, lanes, res));
+    res.clear();
     const double t = ms_since(t0);
     if (t < best_finish) {
       best_finish = t;
       best_lanes = lanes;
     }
   }
-  root->tune_lanes = best_lanes;
+  root->tune_lanes = best_lanes > 0 ? best_lanes : 0;
+  root->finish_mode = best_lanes > 0 ? PPF_FINISH_LANES : PPF_FINISH_AUTO;
 
   info->vote_blocks_per_cu = best_per_cu;
   info->finish_lanes = best_lanes;
@@ -162,6 +171,7 @@ int ppf_tune_reset(ppf_ctx ctx) {
   ppf_ctx root = ctx->root;
   root->tune_vote_per_cu = 0;
   root->tune_lanes = 0;
+  root->finish_mode = PPF_FINISH_AUTO;
   root->sort.tile_items = 16;
   root->sort.tile_block = 256;
   for (ppf_ctx_s *l : root->lanes) {

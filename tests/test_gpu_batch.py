@@ -138,11 +138,14 @@ def test_autotuner_choice_keeps_results(ctx):
         base, _ = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
         choice = ppf_amd.batch_tune(jm, js, w.vote_count_threshold, ctx=c)
         print("tuned:", choice, flush=True)
-        assert choice["vote_blocks_per_cu"] in (1, 2) and choice["finish_lanes"] in (1, 2, 4)
-        assert choice["sort_tile_keys"] in (2048, 4096) and choice["sort_tile_threads"] in (256, 512)
+        assert choice["vote_blocks_per_cu"] in (1, 2) and choice["finish_lanes"] in (0, 1, 2, 4)
+        assert choice["sort_tile_keys"] in (2048, 4096, 8192) and choice["sort_tile_threads"] in (256, 512)
         assert choice["tune_ms"] > 0
         tuned, info = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
-        assert info["lanes"] == min(choice["finish_lanes"], len(jm))
+        if choice["finish_lanes"] == 0:
+            assert info["fused"] == 1 and info["lanes"] == 0
+        else:
+            assert info["fused"] == 0 and info["lanes"] == min(choice["finish_lanes"], len(jm))
         for a, b, m, s in zip(base, tuned, jm, js):
             ca, na = a.votes()
             cb, nb = b.votes()
@@ -153,3 +156,107 @@ def test_autotuner_choice_keeps_results(ctx):
         ppf_amd.tune_reset(c)
     finally:
         c.close()
+
+
+def _jobs(w, models, scenes):
+    jm = [models[mi] for ids in w.scene_models for mi in ids]
+    js = [scenes[k] for k, ids in enumerate(w.scene_models) for _ in ids]
+    return jm, js
+
+
+def _stats_same(a, b):
+    for k in ("cluster_tests", "cluster_mixed_chunks"):
+        assert a.info[k] == b.info[k], k
+
+
+@pytest.mark.parametrize("use_l1,use_avg", [(False, False), (True, False), (False, True), (True, True)])
+def test_fused_finishing_equals_lanes(ctx, use_l1, use_avg):
+    """The batch finished together (one threshold/order sort, one pose launch,
+    one clustering launch over every job's chunks, one argmax) gives each job
+    the same kept votes, poses, clustering and final pose, bit for bit, as
+    the job-by-job finishing on lanes and as the plain lookup."""
+    w = _small_batch(seed=24)
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, use_l1_norm=use_l1, use_averaged_clusters=use_avg,
+                            ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    scenes = [ppf_amd.Scene(s, 0.0, w.rpdf, ctx=ctx) for s in w.scenes]
+    jm, js = _jobs(w, models, scenes)
+    kw = dict(use_l1_norm=use_l1, use_averaged_clusters=use_avg, ctx=ctx)
+    fused, fi = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, **kw)
+    ctx.debug_set(3, 1)
+    try:
+        lanes, li = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, **kw)
+    finally:
+        ctx.debug_set(3, 0)
+    assert fi["fused"] == 1 and fi["lanes"] == 0 and li["fused"] == 0 and li["lanes"] >= 1
+    print(f"fused finishing {fi['finish_ms']:.1f} ms, lanes {li['finish_ms']:.1f} ms", flush=True)
+    for m, s, a, b in zip(jm, js, fused, lanes):
+        _same(a, b, True)
+        _stats_same(a, b)
+        plain = m.ppf_lookup(s)
+        _same(a, plain, True)
+        plain.close()
+    for o in fused + lanes + models + scenes:
+        o.close()
+
+
+def test_fused_finishing_with_empty_jobs(ctx):
+    """Jobs without votes (a scene whose pairs match no model key) among jobs
+    with votes, and a batch where no job keeps a vote: same status and
+    results as the plain lookups."""
+    w = _small_batch(seed=25)
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    scenes = [ppf_amd.Scene(s, 0.0, w.rpdf, ctx=ctx) for s in w.scenes]
+    far = (np.array([[0, 0, 0], [1e4, 0, 0]], np.float32), np.array([[0, 0, 1], [0, 1, 0]], np.float32))
+    empty = ppf_amd.Scene(far, 0.0, 1, ctx=ctx)
+    jm, js = _jobs(w, models, scenes)
+    jm = [models[0]] + jm[:3] + [models[1]]
+    js = [empty] + js[:3] + [empty]
+    res, info = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=ctx)
+    assert info["fused"] == 1
+    assert res[0].info["n_kept"] == 0 and res[-1].info["n_kept"] == 0
+    for m, s, r in zip(jm, js, res):
+        plain = m.ppf_lookup(s)
+        _same(r, plain, True)
+        plain.close()
+    # a threshold no count exceeds: every job keeps nothing
+    none, info = ppf_amd.batch_lookup(jm[1:4], js[1:4], 1.0, ctx=ctx)
+    assert info["fused"] == 1
+    for m, s, r in zip(jm[1:4], js[1:4], none):
+        m.vote_count_threshold = 1.0
+        plain = m.ppf_lookup(s)
+        m.vote_count_threshold = w.vote_count_threshold
+        assert r.info["n_kept"] == 0
+        _same(r, plain, True)
+        plain.close()
+    for o in res + none + models + scenes + [empty]:
+        o.close()
+
+
+def test_batched_configuration_extremes_equal_plain(ctx):
+    """The batched configuration's largest scene (about 120,000 points) with
+    its smallest model, its largest model (about 20,000 points) with the
+    smallest scene, and the smallest pair: batched (finished together)
+    equals plain, bit for bit."""
+    w = synth.batched()
+    ns = [len(s[0]) for s in w.scenes]
+    nm = [len(m[0]) for m in w.models]
+    big_s, small_s = int(np.argmax(ns)), int(np.argmin(ns))
+    big_m, small_m = int(np.argmax(nm)), int(np.argmin(nm))
+    pairs = [(big_s, small_m), (small_s, big_m), (small_s, small_m)]
+    print("extreme jobs:", [(ns[s], nm[m]) for s, m in pairs], flush=True)
+    models = {m: ppf_amd.Model(w.models[m], w.d_dists[m], w.vote_count_threshold, ctx=ctx)
+              for m in {m for _, m in pairs}}
+    scenes = {s: ppf_amd.Scene(w.scenes[s], 0.0, w.rpdf, ctx=ctx) for s in {s for s, _ in pairs}}
+    jm = [models[m] for _, m in pairs]
+    js = [scenes[s] for s, _ in pairs]
+    res, info = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=ctx)
+    print(f"batched: vote {info['vote_ms']:.0f} ms, finish {info['finish_ms']:.0f} ms, fused {info['fused']}",
+          flush=True)
+    for m, s, r in zip(jm, js, res):
+        plain = m.ppf_lookup(s)
+        _same(r, plain, True)
+        _stats_same(r, plain)
+        plain.close()
+        r.close()
+    for o in list(models.values()) + list(scenes.values()):
+        o.close()
