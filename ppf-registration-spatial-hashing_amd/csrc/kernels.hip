@@ -1069,20 +1069,13 @@ __global__ void k_cluster_chunks(const uint32_t *__restrict__ ucount, int64_t U,
   nchunk[u] = (ucount[u] + (uint32_t)CL_CHUNK - 1u) / (uint32_t)CL_CHUNK;
 }
 
-// Order of the poses of a cell for forming clustering chunks: a 4-D Morton
-// code of the normalized quaternion (8 bits per component), so that the 128
-// poses of one wave have close rotations.  The order of the members walked
-// in each neighbour cell is unchanged (pose order); only which poses share a
-// wave depends on this key.
-__device__ __forceinline__ uint32_t spread4(uint32_t v) {  // 8 bits -> every 4th bit
-  v &= 0xffu;
-  v = (v | (v << 12)) & 0x000f000fu;
-  v = (v | (v << 6)) & 0x03030303u;
-  v = (v | (v << 3)) & 0x11111111u;
-  return v;
-}
-
-__device__ __forceinline__ uint32_t quat_morton(const float *__restrict__ quats, uint32_t i) {
+// Order of the poses of a cell for forming clustering chunks: the position
+// of the normalized quaternion (8 bits per component) along a 4-D Hilbert
+// curve, so that the 64 poses of a wave's slot have close rotations (their
+// rotation caps are smaller than along a Morton order, whose runs jump).  The
+// order of the members walked in each neighbour cell is unchanged (pose
+// order); only which poses share a wave depends on this key.
+__device__ __forceinline__ uint32_t quat_order_key(const float *__restrict__ quats, uint32_t i) {
   float q[4], n2 = 0.0f;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -1090,15 +1083,14 @@ __device__ __forceinline__ uint32_t quat_morton(const float *__restrict__ quats,
     n2 += q[c] * q[c];
   }
   const float inv = n2 > 0.0f && n2 < 1e30f ? 1.0f / sqrtf(n2) : 0.0f;
-  uint32_t m = 0;
+  uint32_t b[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     float x = q[c] * inv;
     x = x == x ? fminf(fmaxf(x, -1.0f), 1.0f) : 0.0f;
-    const uint32_t b = (uint32_t)((x + 1.0f) * 127.5f);
-    m |= spread4(b) << c;
+    b[c] = (uint32_t)((x + 1.0f) * 127.5f);
   }
-  return m;
+  return ppfc::hilbert4<8>(b[0], b[1], b[2], b[3]);
 }
 
 __global__ void k_cluster_order_keys(const float *__restrict__ quats, const uint64_t *__restrict__ words,
@@ -1108,7 +1100,7 @@ __global__ void k_cluster_order_keys(const float *__restrict__ quats, const uint
   if (k >= P) return;
   const uint64_t wd = words[k];  // sorted (cell hash, pose)
   const uint32_t i = (uint32_t)wd;
-  okeys[k] = (wd & 0xffffffff00000000ull) | (uint64_t)quat_morton(quats, i);
+  okeys[k] = (wd & 0xffffffff00000000ull) | (uint64_t)quat_order_key(quats, i);
   ovals[k] = i;
 }
 
@@ -1253,17 +1245,41 @@ __device__ __forceinline__ float readfirst_f32(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
 
-// Bounds of the 64 poses a wave holds in one slot (centre: lane 0's pose).
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+__device__ __forceinline__ float wave_min_f32(float v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+// Bounds of the 64 poses a wave holds in one slot, around their mean
+// quaternion and the midpoint of their translations (lane 0's pose where a
+// mean is unusable).  Poses with non-finite quaternions never pass the
+// rotation test and are left out of the mean.
 __device__ ClWave cluster_slot_bounds(const ClPose &p, bool have, float d_dist) {
-  ClPose c = p;
-  c.tx = readfirst_f32(c.tx);
-  c.ty = readfirst_f32(c.ty);
-  c.tz = readfirst_f32(c.tz);
-  c.qx = readfirst_f32(c.qx);
-  c.qy = readfirst_f32(c.qy);
-  c.qz = readfirst_f32(c.qz);
-  c.qw = readfirst_f32(c.qw);
-  const bool h0 = __builtin_amdgcn_readfirstlane(have ? 1 : 0) != 0;
+  ClPose f = p;
+  f.tx = readfirst_f32(f.tx);
+  f.ty = readfirst_f32(f.ty);
+  f.tz = readfirst_f32(f.tz);
+  f.qx = readfirst_f32(f.qx);
+  f.qy = readfirst_f32(f.qy);
+  f.qz = readfirst_f32(f.qz);
+  f.qw = readfirst_f32(f.qw);
+  const bool qok = have && isfinite(p.qx) && isfinite(p.qy) && isfinite(p.qz) && isfinite(p.qw);
+  const float sq[4] = {wave_sum_f32(qok ? p.qx : 0.0f), wave_sum_f32(qok ? p.qy : 0.0f),
+                       wave_sum_f32(qok ? p.qz : 0.0f), wave_sum_f32(qok ? p.qw : 0.0f)};
+  const float nq = wave_sum_f32(qok ? 1.0f : 0.0f);
+  const float lo[3] = {wave_min_f32(have ? p.tx : INFINITY), wave_min_f32(have ? p.ty : INFINITY),
+                       wave_min_f32(have ? p.tz : INFINITY)};
+  const float hi[3] = {wave_max_f32(have ? p.tx : -INFINITY), wave_max_f32(have ? p.ty : -INFINITY),
+                       wave_max_f32(have ? p.tz : -INFINITY)};
+  const ClPose c = ppfc::cl_group_centre(sq, nq, lo, hi, f);
+  const bool h0 = __ballot(have) != 0ull;
   ClWave W;
   ppfc::cl_wave_center(c, h0, W);
   float rho = 0.0f, nmax = 0.0f, rad = 0.0f;
@@ -1579,7 +1595,7 @@ __global__ void k_bcluster_order_keys(const float *__restrict__ quats, const uin
       hi = mid - 1;
   }
   const uint32_t i = members[k];
-  okeys[k] = ((uint64_t)lo << 32) | (uint64_t)quat_morton(quats, i);
+  okeys[k] = ((uint64_t)lo << 32) | (uint64_t)quat_order_key(quats, i);
   ovals[k] = i;
 }
 

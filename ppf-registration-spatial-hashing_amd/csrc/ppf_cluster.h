@@ -144,6 +144,63 @@ PPF_HD void cl_wave_finish(ClWave &W, float rho, float nmax, float rad, float d_
   if (B < 1e18f) W.b2 = B * B * (1.0f + 1e-6f);
 }
 
+// Centre of a group for cl_wave_center: the mean of the group's quaternions
+// (sq = their sum over nq poses with finite components) and the midpoint of
+// the box [lo, hi] of its translations; `first` (a pose of the group) when
+// the mean is too short to normalize reliably.  Any centre gives sound
+// bounds (rho and rad are measured from it); a central one gives tighter
+// ones than a pose at the group's edge.
+PPF_HD ClPose cl_group_centre(const float sq[4], float nq, const float lo[3], const float hi[3],
+                              const ClPose &first) {
+  ClPose c = first;
+  if (nq > 0.0f) {
+    const float inv = 1.0f / nq;
+    const float mx = sq[0] * inv, my = sq[1] * inv, mz = sq[2] * inv, mw = sq[3] * inv;
+    if (mx * mx + my * my + mz * mz + mw * mw > 0.25f) {
+      c.qx = mx;
+      c.qy = my;
+      c.qz = mz;
+      c.qw = mw;
+    }
+  }
+  if (lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2]) {
+    c.tx = 0.5f * lo[0] + 0.5f * hi[0];
+    c.ty = 0.5f * lo[1] + 0.5f * hi[1];
+    c.tz = 0.5f * lo[2] + 0.5f * hi[2];
+  }
+  return c;
+}
+
+// Position of a point of the 4-D grid [0, 2^B)^4 along a Hilbert curve
+// (Skilling's transpose form, "Programming the Hilbert curve", AIP Conf.
+// Proc. 707, 2004): consecutive positions are neighbouring cells, so
+// consecutive poses in this order have close rotations.
+template <int B>
+PPF_HD uint32_t hilbert4(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+  uint32_t X[4] = {x0, x1, x2, x3};
+  for (uint32_t q = 1u << (B - 1); q > 1u; q >>= 1) {
+    const uint32_t p = q - 1u;
+    for (int i = 0; i < 4; ++i) {
+      if (X[i] & q) {
+        X[0] ^= p;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & p;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 4; ++i) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t q = 1u << (B - 1); q > 1u; q >>= 1)
+    if (X[3] & q) t ^= q - 1u;
+  for (int i = 0; i < 4; ++i) X[i] ^= t;
+  uint32_t h = 0;
+  for (int b = B - 1; b >= 0; --b)
+    for (int i = 0; i < 4; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+  return h;
+}
+
 // true: the member fails the test against every pose of the group.
 PPF_HD bool cluster_skip(const ClWave &W, float jqx, float jqy, float jqz, float jqw, float jtx,
                          float jty, float jtz, int use_l1) {

@@ -7,6 +7,7 @@
 // setting; exits non-zero on the first violation.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <random>
 #include <vector>
 
@@ -100,8 +101,26 @@ int main() {
       p.vc = 1.0f;
       p.ox = p.tx, p.oy = p.ty, p.oz = p.tz;
     }
+    // centre: the first pose (odd values of setting / 2), or the group's
+    // mean quaternion and translation-box midpoint as the GPU kernel builds it
     ClWave W;
-    ppfc::cl_wave_center(P[0], have[0], W);
+    if ((setting / 2) % 2 == 0) {
+      float sq[4] = {0, 0, 0, 0}, nq = 0.0f;
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int i = 0; i < 128; ++i) {
+        if (!have[i]) continue;
+        const ClPose &p = P[i];
+        if (std::isfinite(p.qx) && std::isfinite(p.qy) && std::isfinite(p.qz) && std::isfinite(p.qw)) {
+          sq[0] += p.qx, sq[1] += p.qy, sq[2] += p.qz, sq[3] += p.qw;
+          nq += 1.0f;
+        }
+        const float t[3] = {p.tx, p.ty, p.tz};
+        for (int k = 0; k < 3; ++k) lo[k] = fminf(lo[k], t[k]), hi[k] = fmaxf(hi[k], t[k]);
+      }
+      ppfc::cl_wave_center(ppfc::cl_group_centre(sq, nq, lo, hi, P[0]), have[0], W);
+    } else {
+      ppfc::cl_wave_center(P[0], have[0], W);
+    }
     float rho = 0, nmax = 0, rad = 0;
     for (int i = 0; i < 128; ++i)
       if (have[i]) ppfc::cl_wave_add(W, P[i], rho, nmax, rad);
@@ -167,5 +186,25 @@ int main() {
                 use_l1, (int)odd, 100.0 * (double)sk / (double)(mm ? mm : 1), mm);
   }
   std::printf("members %lld skipped %lld violations %lld\n", members, skipped, violations);
+  // the chunk order's Hilbert curve: every point of the 4-D grid [0, 8)^4 has
+  // its own position, and consecutive positions are neighbouring cells
+  std::vector<int> at(4096, -1);
+  for (uint32_t x = 0; x < 4096; ++x) {
+    const uint32_t h = ppfc::hilbert4<3>(x & 7, (x >> 3) & 7, (x >> 6) & 7, x >> 9);
+    if (h >= 4096 || at[h] >= 0) {
+      std::printf("HILBERT not a bijection at %u\n", x);
+      return 1;
+    }
+    at[h] = (int)x;
+  }
+  for (int h = 1; h < 4096; ++h) {
+    int l1 = 0;
+    for (int k = 0; k < 4; ++k) l1 += std::abs(((at[h] >> (3 * k)) & 7) - ((at[h - 1] >> (3 * k)) & 7));
+    if (l1 != 1) {
+      std::printf("HILBERT step %d is not to a neighbour\n", h);
+      return 1;
+    }
+  }
+  std::printf("hilbert ok\n");
   return violations ? 1 : 0;
 }

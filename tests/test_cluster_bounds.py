@@ -2,7 +2,9 @@
 members (csrc/ppf_cluster.h): every member the bound rejects fails the exact
 pose-pair test of rot_clustering_kernel (reference kernel.cu:735-756) for all
 poses of the group, including members built just inside and outside both
-thresholds, non-unit, negated, zero and NaN quaternions."""
+thresholds, non-unit, negated, zero and NaN quaternions, with the group's
+centre at its first pose or at its mean (as the kernel builds it); and that
+the 4-D Hilbert order of the chunks is a curve through neighbouring cells."""
 import os
 import subprocess
 
@@ -18,4 +20,4 @@ def test_cluster_skip_bound_is_sound(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stdout[-2000:]
-    assert "violations 0" in r.stdout
+    assert "violations 0" in r.stdout and "hilbert ok" in r.stdout
