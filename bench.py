@@ -24,10 +24,10 @@ Also in the line:
 * the large-hash stress workload: the 30,000-point model's build (9e8 pairs)
   and its lookup on a 2,000-row sample of the 200,000-point scene;
 * the roofline of the dominant kernel (k_vote or k_cluster_tiled, whichever
-  takes more device time in the timed steps): HBM bytes per launch from the
-  committed rocprofv3 counter runs of this workload over the launch duration
-  measured here with HIP events, and the kernel's share of the VALU issue
-  slots (both kernels are instruction-bound);
+  takes more device time in the timed steps): memory-fabric bytes per launch
+  from the committed rocprofv3 counter runs of this workload over the launch
+  duration measured here with HIP events, and the kernel's VALU activity
+  relative to a full-rate FMA stream (both kernels are instruction-bound);
 * a CPU baseline: the oracle (OpenMP port of the reference's loops) on one
   job of the subset, stage by stage: model build, the vote loop on a stride
   sample of reference rows (extrapolated per lookup), poses, and the
@@ -56,7 +56,6 @@ HBM_PEAK_GBS = 8000.0
 SUBSET_MAX_PRODUCT = 6e7
 WIDE_MAX_PRODUCT = 1e8
 PROFILE_JSON = os.path.join(ROOT, "profiles", "batched_subset_counters.json")
-VALU_SLOTS_PER_S = 1024 * 2.4e9 / 4.0  # wave64 VALU instructions the chip issues per second
 
 
 def log(msg):
@@ -64,38 +63,48 @@ def log(msg):
 
 
 def counters(kernel: str):
-    """(HBM bytes, VALU instructions) per dispatch of `kernel` on the headline
-    workload, from the committed rocprofv3 counter runs (FETCH_SIZE and
-    WRITE_SIZE scaled by the copy-kernel calibration recorded beside them)."""
+    """Per-dispatch counters of `kernel` on the headline workload from the
+    committed rocprofv3 runs: L2-miss bytes (FETCH_SIZE and WRITE_SIZE scaled
+    by the copy-kernel calibration recorded beside them), VALU instructions,
+    and the kernel's VALU activity relative to the full-rate FMA stream's
+    (SQ_ACTIVE_INST_VALU per GRBM_GUI_ACTIVE cycle, both runs)."""
     try:
         with open(PROFILE_JSON) as f:
-            k = json.load(f)["kernels"][kernel]
+            j = json.load(f)
+        k = j["kernels"][kernel]
     except (OSError, KeyError, ValueError):
-        return None, None
-    return k.get("hbm_bytes_per_dispatch"), k.get("valu_insts_per_dispatch")
+        return {}
+    return {"traffic": k.get("hbm_bytes_per_dispatch"), "valu": k.get("valu_insts_per_dispatch"),
+            "valu_vs_fma_stream": k.get("valu_activity_vs_fma_stream")}
 
 
 def kernel_roofline(name, ms_per_launch, alg_bytes_per_launch=None):
-    traffic, valu = counters(name)
+    c = counters(name)
+    traffic = c.get("traffic")
     out = {"kernel": name, "bound": "valu", "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "ms_per_launch": ms_per_launch, "traffic": traffic,
            "traffic_source": os.path.relpath(PROFILE_JSON, ROOT) if traffic else None}
     s = ms_per_launch * 1e-3
     if traffic and s > 0:
+        # bytes the L2 fetched from and wrote to the memory fabric (HBM or
+        # the Infinity Cache) per launch, over the launch time measured here
         out["achieved"] = traffic / s / 1e9
         out["frac"] = out["achieved"] / HBM_PEAK_GBS
+        assert out["frac"] <= 1.0, f"{name}: HBM fraction {out['frac']} > 1"
     else:
         out["achieved"] = out["frac"] = None
-    out["valu_issue_share"] = (valu / (VALU_SLOTS_PER_S * s)) if valu and s > 0 else None
+    valu = c.get("valu")
+    out["valu_insts_per_s"] = valu / s if valu and s > 0 else None
+    # VALU instructions per cycle relative to a stream of independent
+    # full-wave FMAs on every SIMD (1.0 = that stream's issue rate; a
+    # kernel with partly masked waves can exceed it)
+    out["valu_vs_fma_stream"] = c.get("valu_vs_fma_stream")
     if alg_bytes_per_launch is not None:
         # the survey's algorithmic bytes (8 B per lookup + 4 B per vote) over
         # the launch time, as a ratio to the HBM peak: >1 means the kernel
         # serves the per-vote pair ids from L2/LDS instead of HBM
         out["alg_bytes_per_launch"] = alg_bytes_per_launch
         out["alg_rate_over_hbm_peak"] = alg_bytes_per_launch / s / 1e9 / HBM_PEAK_GBS if s > 0 else None
-    for key in ("frac", "valu_issue_share"):
-        if out.get(key) is not None:
-            assert out[key] <= 1.0, f"{name}: {key} {out[key]} > 1"
     return out
 
 
@@ -341,12 +350,21 @@ def main():
     ap.add_argument("--large-hash-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="run only a full-rate FMA stream kernel 3x (reference for VALU counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(voting would shard by scene reference point)", file=sys.stderr)
         sys.exit(2)
+    if args.valu_calibration:
+        ctx = ppf_amd.Context(0)
+        for _ in range(3):
+            t = time.perf_counter()
+            ctx.debug_fma_stream(32768, 4000)
+            log(f"FMA stream: {32768 * 4 * 4000 * 16 / (time.perf_counter() - t):.3e} wave-FMA/s (wall clock)")
+        return
     if args.copy_calibration:
         ctx = ppf_amd.Context(0)
         a = np.arange(1 << 26, dtype=np.uint32)

@@ -382,7 +382,10 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
  * op 1: stable radix sort of n uint64 keys_io carrying uint32 vals_io (may be
  *       NULL) on bits [bit_lo, bit_hi);
  * op 2: plain device copy kernel of n uint32 (keys_io in, vals_io out), a
- *       known-size reference for hardware traffic counters. */
+ *       known-size reference for hardware traffic counters;
+ * op 3: n blocks of 256 threads each running bit_lo iterations of 16
+ *       independent FMAs (keys_io receives n*256 floats), the full-rate VALU
+ *       reference for instruction counters. */
 int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
                         int32_t bit_lo, int32_t bit_hi);
 

@@ -1282,9 +1282,18 @@ int ppf_debug_cluster(ppf_ctx ctx, const float *T, const uint64_t *codes, const 
 int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
                         int32_t bit_lo, int32_t bit_hi) {
   ARG_CHECK(ctx && keys_io && n >= 0, "ppf_debug_primitive: bad arguments");
-  ARG_CHECK(op >= 0 && op <= 2, "ppf_debug_primitive: unknown op");
+  ARG_CHECK(op >= 0 && op <= 3, "ppf_debug_primitive: unknown op");
   TRY(set_device(ctx));
   hipStream_t st = ctx->stream;
+  if (op == 3) {
+    ARG_CHECK(n > 0 && n <= (1 << 20) && bit_lo > 0, "ppf_debug_primitive: FMA stream needs blocks and iterations");
+    ppfi::DevBuf o;
+    TRY(o.ensure((size_t)n * 256 * 4));
+    TRY(ppfk::debug_fma_stream(ctx, o.as<float>(), n, bit_lo));
+    HIPC(hipMemcpyAsync(keys_io, o.ptr, (size_t)n * 256 * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return PPF_OK;
+  }
   if (op == 2) {
     ARG_CHECK(vals_io && n % 4 == 0, "ppf_debug_primitive: copy needs an output and n % 4 == 0");
     ppfi::DevBuf a, b;

@@ -1651,6 +1651,25 @@ __global__ void k_bfinal(const uint64_t *__restrict__ kbeg, int nj, const unsign
 }
 
 // ---------------------------------------------------------- diagnostics
+// A stream of independent full-wave FMAs (eight chains per lane, sixteen
+// v_fma_f32 per iteration): the reference VALU issue rate for instruction
+// counters, as the copy kernel is for traffic counters.
+__global__ __launch_bounds__(256) void k_fma_stream(float *__restrict__ out, int iters, float a, float b) {
+  float x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = (float)(threadIdx.x + k);
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[k] = fmaf(x[k], a, b);
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += x[k];
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 __global__ void k_copy_u32(const uint4 *__restrict__ in, uint4 *__restrict__ out, int64_t n4) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n4) out[i] = in[i];
@@ -2286,6 +2305,14 @@ int finish_batch(ppf_ctx ctx, const std::vector<FinJob> &jobs, const std::vector
     PPF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     out.cluster_kernel_ms = ms;
   }
+  return 0;
+}
+
+int debug_fma_stream(ppf_ctx ctx, float *out, int64_t blocks, int iters) {
+  if (blocks <= 0) return 0;
+  hipLaunchKernelGGL(k_fma_stream, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, out, iters, 0.999f,
+                     0.001f);
+  PPF_HIP(hipGetLastError());
   return 0;
 }
 

@@ -303,6 +303,8 @@ int finish_batch(ppf_ctx ctx, const std::vector<FinJob> &jobs, const std::vector
                  const std::function<int(int64_t, FinishArrays &)> &alloc, FinishOut &out);
 int debug_numerics(ppf_ctx ctx, int op, const float *a, const float *b, float *out, int64_t n);
 int debug_copy(ppf_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n);  // n multiple of 4
+// blocks x 256 threads of independent FMA chains, iters x 16 FMAs per thread
+int debug_fma_stream(ppf_ctx ctx, float *out, int64_t blocks, int iters);
 
 }  // namespace ppfk
 

@@ -273,6 +273,12 @@ class Context:
         _check(lib().ppf_debug_primitive(self.handle, 2, x.ctypes.data, out.ctypes.data, len(x), 0, 0))
         return out
 
+    def debug_fma_stream(self, blocks: int, iters: int) -> np.ndarray:
+        """Full-rate VALU reference kernel (ppf_debug_primitive op 3)."""
+        out = np.zeros(blocks * 256, np.float32)
+        _check(lib().ppf_debug_primitive(self.handle, 3, out.ctypes.data, None, blocks, iters, 0))
+        return out
+
     def set_progress(self, fn=None, rows_per_launch: int = 0):
         """ppf_ctx_set_progress: fn(stage, done, total) after each vote launch
         of rows_per_launch reference rows and after each later stage of a
