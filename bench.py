@@ -217,6 +217,7 @@ def run_served(ctx, models, w, rates=(50.0, 200.0, 2000.0), n_requests=24):
     srv.close()
     return {"workload": desc, "rates": out,
             "server": {"rounds": st["rounds"], "max_requests_in_round": st["max_requests_in_round"],
+                       "finish_groups": st["finish_groups"], "max_finish_group": st["max_finish_group"],
                        "peak_inflight_bytes": st["peak_inflight_bytes"]}}
 
 

@@ -339,6 +339,8 @@ typedef struct {
   uint64_t inflight_bytes, peak_inflight_bytes, budget_bytes;
   int64_t max_requests_in_round; /* most distinct requests voted in one round */
   double vote_ms;
+  int64_t finish_groups;   /* finishing calls of the lanes (a group: jobs finished together) */
+  int64_t max_finish_group; /* most jobs finished together */
 } ppf_server_stats_t;
 
 int ppf_server_create(ppf_ctx ctx, int32_t n_models, const ppf_model *models,

@@ -129,33 +129,32 @@ struct BatchStore {
 };
 }  // namespace
 
-bool batch_fused_ok(const std::vector<BatchJob> &jobs, const ppf_lookup_params *p) {
-  if (p->cpu_clustering || jobs.empty()) return false;
+bool finish_together_ok(const std::vector<FinishInput> &in, const ppf_lookup_params *p) {
+  if (p->cpu_clustering || in.empty()) return false;
   uint64_t C = 0;
   uint32_t mx = 0;
   int64_t ns = 0, nm = 0;
-  for (const BatchJob &b : jobs) {
+  for (const FinishInput &b : in) {
     C += b.n_cand;
     mx = std::max(mx, b.max_count);
     ns = std::max(ns, b.s->cloud.n);
     nm = std::max(nm, b.m->cloud.n);
   }
-  return ppfk::finish_batch_fits((int)jobs.size(), C, mx, ns, nm);
+  return ppfk::finish_batch_fits((int)in.size(), C, mx, ns, nm);
 }
 
-int batch_finish_fused(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p,
-                       std::vector<std::unique_ptr<ppf_result_s>> &results, ppf_batch_info_t *stages) {
+int finish_together(ppf_ctx ctx, const std::vector<FinishInput> &in, const ppf_lookup_params *p,
+                    std::vector<std::unique_ptr<ppf_result_s>> &results, ppf_batch_info_t *stages) {
   ppf_ctx root = ctx->root;
-  const size_t nj = jobs.size();
+  const size_t nj = in.size();
   std::vector<ppfk::FinJob> fj(nj);
   std::vector<uint32_t> n_cand(nj);
   uint32_t mx = 0;
   int64_t ns = 0, nm = 0;
   for (size_t j = 0; j < nj; ++j) {
-    const BatchJob &b = jobs[j];
-    auto *cc = b.cand.as<unsigned long long>();
-    fj[j].cand_code = cc;
-    fj[j].cand_count = reinterpret_cast<const uint32_t *>(cc + b.cap);
+    const FinishInput &b = in[j];
+    fj[j].cand_code = b.cand_code;
+    fj[j].cand_count = b.cand_count;
     fj[j].thr_f = b.thr * (float)b.max_count;  // model.cu:164, as select_and_sort
     fj[j].rpdf = b.s->rpdf;
     fj[j].mframe = b.m->frames.as<float>();
@@ -185,12 +184,12 @@ int batch_finish_fused(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_looku
     return 0;
   };
   ppfk::FinishOut out;
-  PPF_TRY(ppfk::finish_batch(root, fj, n_cand, mx, ns, nm, p->use_l1_norm != 0, p->use_averaged_clusters != 0,
+  PPF_TRY(ppfk::finish_batch(ctx, fj, n_cand, mx, ns, nm, p->use_l1_norm != 0, p->use_averaged_clusters != 0,
                              alloc, out));
   results.clear();
   results.resize(nj);
   for (size_t j = 0; j < nj; ++j) {
-    const BatchJob &b = jobs[j];
+    const FinishInput &b = in[j];
     std::unique_ptr<ppf_result_s> r(new ppf_result_s());
     r->ctx = root;
     r->n_m = b.m->cloud.n;
@@ -203,7 +202,7 @@ int batch_finish_fused(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_looku
     r->P = P;
     r->info.n_kept = P;
     r->info.max_idx = out.max_idx[j];
-    // the batch's stage times: every job's finishing ran in the same launches
+    // the group's stage times: every job's finishing ran in the same launches
     r->info.select_ms = out.select_ms;
     r->info.pose_ms = out.pose_ms;
     r->info.cluster_ms = out.cluster_ms;
@@ -233,6 +232,28 @@ int batch_finish_fused(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_looku
     stages->cluster_kernel_ms = out.cluster_kernel_ms;
   }
   return 0;
+}
+
+namespace {
+std::vector<FinishInput> finish_inputs(const std::vector<BatchJob> &jobs) {
+  std::vector<FinishInput> in(jobs.size());
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const BatchJob &b = jobs[j];
+    auto *cc = b.cand.as<unsigned long long>();
+    in[j] = FinishInput{b.m, b.s, b.thr, cc, reinterpret_cast<const uint32_t *>(cc + b.cap), b.n_cand,
+                        b.max_count, b.votes};
+  }
+  return in;
+}
+}  // namespace
+
+bool batch_fused_ok(const std::vector<BatchJob> &jobs, const ppf_lookup_params *p) {
+  return finish_together_ok(finish_inputs(jobs), p);
+}
+
+int batch_finish_fused(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p,
+                       std::vector<std::unique_ptr<ppf_result_s>> &results, ppf_batch_info_t *stages) {
+  return finish_together(ctx, finish_inputs(jobs), p, results, stages);
 }
 
 int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,

@@ -45,6 +45,25 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
 // Finish every job on up to n_lanes lanes; results[k] belongs to jobs[k].
 int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,
                  std::vector<std::unique_ptr<ppf_result_s>> &results);
+// A job after voting: its model, scene, threshold and candidates.
+struct FinishInput {
+  ppf_model m;
+  ppf_scene s;
+  float thr;
+  const unsigned long long *cand_code;
+  const uint32_t *cand_count;
+  uint32_t n_cand, max_count;
+  int64_t votes;
+};
+// Whether finish_together can take the jobs (GPU clustering, sort keys of 64
+// bits).
+bool finish_together_ok(const std::vector<FinishInput> &in, const ppf_lookup_params *p);
+// Threshold/order, poses, clustering and argmax of every job at once on ctx
+// (a root or a lane): one launch per stage; results[k] belongs to in[k],
+// identical to a plain lookup's, and the results share one allocation.
+// stages (may be NULL) receives the stage times and fused = 1.
+int finish_together(ppf_ctx ctx, const std::vector<FinishInput> &in, const ppf_lookup_params *p,
+                    std::vector<std::unique_ptr<ppf_result_s>> &results, ppf_batch_info_t *stages);
 // Whether batch_finish_fused can take the jobs (GPU clustering, sort keys of
 // 64 bits).
 bool batch_fused_ok(const std::vector<BatchJob> &jobs, const ppf_lookup_params *p);

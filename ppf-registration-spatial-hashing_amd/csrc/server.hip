@@ -6,7 +6,8 @@
 // is finished; vote rounds are single persistent launches of the vote kernel
 // over row slices of every voting job (VoteJob.r_begin selects the slice);
 // finishing (threshold/order, poses, clustering) runs on the server's
-// finishing lanes, concurrently with the next rounds.
+// finishing lanes, concurrently with the next rounds; a lane finishes every
+// queued job with the same clustering flags together (finish_together).
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -186,6 +187,53 @@ class HipBackend : public ppfs::Backend {
     h->r = res.release();
     r.results[(size_t)j.target] = h;
     return PPF_OK;
+  }
+
+  // Jobs with GPU clustering and the same clustering flags are finished
+  // together (finish_together: one launch per stage for the group).
+  int64_t group_key(const ppfs::Request &r, const ppfs::Job &) override {
+    if (r.params.cpu_clustering) return -1;
+    return (r.params.use_l1_norm ? 1 : 0) | (r.params.use_averaged_clusters ? 2 : 0);
+  }
+
+  void finish_group(int lane, std::vector<std::pair<ppfs::Request *, ppfs::Job *>> &g, std::vector<int> &rc,
+                    std::vector<int> &status, std::vector<std::string> &err) override {
+    ppf_ctx L = lanes[(size_t)lane];
+    std::vector<FinishInput> in;
+    for (auto &x : g) {
+      auto sc = std::static_pointer_cast<SceneState>(x.first->dev);
+      auto js = std::static_pointer_cast<JobState>(x.second->dev);
+      auto *cc = js->cand.as<unsigned long long>();
+      in.push_back(FinishInput{models[(size_t)x.second->model_id], sc->scene, x.first->params.vote_count_threshold,
+                               cc, reinterpret_cast<const uint32_t *>(cc + js->cap), js->n_cand, js->max_count,
+                               js->votes});
+    }
+    std::vector<std::unique_ptr<ppf_result_s>> res;
+    int e = hipSetDevice(L->device) == hipSuccess ? PPF_OK : PPF_ERR_HIP;
+    if (e == PPF_OK && !finish_together_ok(in, &g[0].first->params)) e = -1;
+    if (e == PPF_OK) e = finish_together(L, in, &g[0].first->params, res, nullptr);
+    if (e == -1) {  // keys too wide for one sort: one by one
+      ppfs::Backend::finish_group(lane, g, rc, status, err);
+      return;
+    }
+    const std::string msg = e != PPF_OK ? std::string(ppf_last_error()) : std::string();
+    for (size_t k = 0; k < g.size(); ++k) {
+      auto js = std::static_pointer_cast<JobState>(g[k].second->dev);
+      js->cand.release();
+      if (e != PPF_OK) {
+        rc[k] = e;
+        err[k] = msg;
+        continue;
+      }
+      ppfs::Request &r = *g[k].first;
+      const int32_t t = g[k].second->target;
+      std::memcpy(r.poses[(size_t)t].data(), res[k]->pose, 64);
+      status[k] = res[k]->info.status;
+      auto h = std::make_shared<ResultHolder>();
+      h->r = res[k].release();
+      r.results[(size_t)t] = h;
+      rc[k] = PPF_OK;
+    }
   }
 
   void release(ppfs::Request &r) override {
@@ -406,6 +454,8 @@ int ppf_server_stats(ppf_server s, ppf_server_stats_t *st) {
   st->budget_bytes = s->cfg.budget;
   st->max_requests_in_round = x.max_requests_in_round;
   st->vote_ms = x.vote_ms;
+  st->finish_groups = x.finish_groups;
+  st->max_finish_group = x.max_finish_group;
   return PPF_OK;
 }
 

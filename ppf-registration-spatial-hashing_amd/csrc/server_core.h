@@ -11,7 +11,8 @@
 // share of the round's work.  Requests that arrive meanwhile join at the next
 // round.  A job whose rows are all voted moves to the lanes: host threads
 // with their own streams that run threshold/order, poses and clustering while
-// the next rounds vote.  When all of a request's jobs are finished its device
+// the next rounds vote; a lane takes every queued job it can finish together
+// with the first (one launch per stage for all of them).  When all of a request's jobs are finished its device
 // memory returns to the budget.
 //
 // The reference has no served form: it registers one (scene, model) pair at
@@ -97,6 +98,17 @@ class Backend {
   // r.results[j.target], and the job's outcome (PPF_OK or PPF_ERR_NO_VOTES)
   // in *job_status.
   virtual int finish(int lane, Request &r, Job &j, int *job_status, std::string &err) = 0;
+  // Jobs whose keys are equal (and >= 0) may be finished together by one
+  // finish_group call; -1: this job is finished alone.  Called under the
+  // scheduler's lock: it reads only what is fixed at submission.
+  virtual int64_t group_key(const Request &, const Job &) { return -1; }
+  // Lane thread `lane`: finish group[k] = (request, job) for every k, as
+  // finish() would one by one: rc[k], status[k] and err[k] per job.
+  virtual void finish_group(int lane, std::vector<std::pair<Request *, Job *>> &group, std::vector<int> &rc,
+                            std::vector<int> &status, std::vector<std::string> &err) {
+    for (size_t k = 0; k < group.size(); ++k)
+      rc[k] = finish(lane, *group[k].first, *group[k].second, &status[k], err[k]);
+  }
   // Any thread: free the request's device state (results stay).
   virtual void release(Request &r) = 0;
   // Reference rows the device votes at once (blocks of the vote launch).
@@ -107,6 +119,7 @@ struct Config {
   uint64_t budget = 0;          // device bytes for requests in flight
   int max_active = 64;          // requests in flight at once
   int lanes = 4;                // finishing lanes
+  int max_group = 64;           // jobs a lane finishes together at most
   double round_ms = 50.0;       // target duration of one vote round
   double tail_rows = 8.0;       // a round lasts at least this many rows per slot
   double aging = 1.0;           // ms of priority a request gains per ms in flight
@@ -120,6 +133,7 @@ struct Stats {
   uint64_t inflight_bytes = 0, peak_inflight_bytes = 0;
   int64_t max_requests_in_round = 0;  // most distinct requests voted in one round
   double vote_ms = 0;
+  int64_t finish_groups = 0, max_finish_group = 0;  // lane finishing calls, most jobs in one
 };
 
 class Scheduler {
@@ -474,39 +488,76 @@ class Scheduler {
         if (lanes_stop_) break;
         continue;
       }
-      auto f = finq_.front();
+      // the front job, and with it every queued job the backend can finish
+      // together with it (same group key, not failed), in queue order
+      std::vector<std::pair<Request *, Job *>> group{finq_.front()};
       finq_.pop_front();
-      Request &r = *f.first;
-      Job &j = *f.second;
-      busy_.push_back(&r);
-      lk.unlock();
-      std::string err;
-      int rc = PPF_OK, js = PPF_OK;
-      if (j.status == PPF_OK) rc = be_->finish(lane, r, j, &js, err);
-      lk.lock();
-      if (rc == PPF_OK && j.status == PPF_OK) j.status = js;
-      for (auto b = busy_.begin(); b != busy_.end(); ++b)
-        if (*b == &r) {
-          busy_.erase(b);
-          break;
-        }
-      if (rc != PPF_OK && j.status == PPF_OK) {
-        j.status = rc;
-        j.error = err;
-      }
-      if (--r.jobs_left == 0) {
-        lk.unlock();
-        be_->release(r);
-        lk.lock();
-        retire_locked(r);
-        int worst = r.status;
-        std::string e = r.error;
-        for (Job &x : r.jobs)
-          if (worst == PPF_OK && x.status != PPF_OK && x.status != PPF_ERR_NO_VOTES) {
-            worst = x.status;
-            e = x.error;
+      const int64_t key =
+          group[0].second->status == PPF_OK ? be_->group_key(*group[0].first, *group[0].second) : -1;
+      if (key >= 0) {
+        for (auto q = finq_.begin(); q != finq_.end() && (int)group.size() < cfg_.max_group;) {
+          if (q->second->status == PPF_OK && be_->group_key(*q->first, *q->second) == key) {
+            group.push_back(*q);
+            q = finq_.erase(q);
+          } else {
+            ++q;
           }
-        finalize_locked(r, worst == PPF_OK ? State::Done : State::Failed, worst, e);
+        }
+      }
+      for (auto &g : group) busy_.push_back(g.first);
+      const size_t n = group.size();
+      ++st_.finish_groups;
+      st_.max_finish_group = std::max<int64_t>(st_.max_finish_group, (int64_t)n);
+      std::vector<int> rc(n, PPF_OK), js(n, PPF_OK);
+      std::vector<std::string> err(n);
+      std::vector<std::pair<Request *, Job *>> todo;
+      std::vector<size_t> where;
+      for (size_t k = 0; k < n; ++k)
+        if (group[k].second->status == PPF_OK) {
+          todo.push_back(group[k]);
+          where.push_back(k);
+        }
+      lk.unlock();
+      if (todo.size() == 1) {
+        rc[where[0]] = be_->finish(lane, *todo[0].first, *todo[0].second, &js[where[0]], err[where[0]]);
+      } else if (!todo.empty()) {
+        std::vector<int> rc2(todo.size(), PPF_OK), js2(todo.size(), PPF_OK);
+        std::vector<std::string> e2(todo.size());
+        be_->finish_group(lane, todo, rc2, js2, e2);
+        for (size_t k = 0; k < todo.size(); ++k) {
+          rc[where[k]] = rc2[k];
+          js[where[k]] = js2[k];
+          err[where[k]] = e2[k];
+        }
+      }
+      lk.lock();
+      for (size_t k = 0; k < n; ++k) {
+        Request &r = *group[k].first;
+        Job &j = *group[k].second;
+        if (rc[k] == PPF_OK && j.status == PPF_OK) j.status = js[k];
+        for (auto b = busy_.begin(); b != busy_.end(); ++b)
+          if (*b == &r) {
+            busy_.erase(b);
+            break;
+          }
+        if (rc[k] != PPF_OK && j.status == PPF_OK) {
+          j.status = rc[k];
+          j.error = err[k];
+        }
+        if (--r.jobs_left == 0) {
+          lk.unlock();
+          be_->release(r);
+          lk.lock();
+          retire_locked(r);
+          int worst = r.status;
+          std::string e = r.error;
+          for (Job &x : r.jobs)
+            if (worst == PPF_OK && x.status != PPF_OK && x.status != PPF_ERR_NO_VOTES) {
+              worst = x.status;
+              e = x.error;
+            }
+          finalize_locked(r, worst == PPF_OK ? State::Done : State::Failed, worst, e);
+        }
       }
     }
   }

@@ -92,7 +92,8 @@ class ServerStats(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "cancelled", "rejected",
                                          "rounds", "rows_voted", "queued", "active")] + \
                [(k, C.c_uint64) for k in ("inflight_bytes", "peak_inflight_bytes", "budget_bytes")] + \
-               [("max_requests_in_round", C.c_int64), ("vote_ms", C.c_double)]
+               [("max_requests_in_round", C.c_int64), ("vote_ms", C.c_double),
+                ("finish_groups", C.c_int64), ("max_finish_group", C.c_int64)]
 
 
 _lib = None

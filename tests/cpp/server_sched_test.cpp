@@ -42,6 +42,9 @@ class FakeBackend : public Backend {
   int64_t max_rows_in_round = 0;
   double row_us = 20.0;  // simulated device time per row
   std::atomic<int> finishing{0}, max_finishing{0};
+  // grouping: jobs of even / odd model ids are finished together
+  bool grouping = false;
+  std::atomic<int> max_group{0}, groups{0};
 
   void note(const std::string &s) {
     std::lock_guard<std::mutex> g(mu);
@@ -111,6 +114,20 @@ class FakeBackend : public Backend {
     r.poses[(size_t)j.target][2] = (float)f->passes;
     *js = j.model_id == 99 ? PPF_ERR_NO_VOTES : PPF_OK;
     return PPF_OK;
+  }
+  int64_t group_key(const Request &, const Job &j) override { return grouping ? (j.model_id & 1) : -1; }
+  void finish_group(int lane, std::vector<std::pair<Request *, Job *>> &g, std::vector<int> &rc,
+                    std::vector<int> &js, std::vector<std::string> &err) override {
+    ++groups;
+    int m = max_group.load();
+    while ((int)g.size() > m && !max_group.compare_exchange_weak(m, (int)g.size())) {
+    }
+    for (auto &x : g)
+      if ((x.second->model_id & 1) != (g[0].second->model_id & 1) || x.second->status != PPF_OK) {
+        std::printf("FAIL: a group mixes keys or holds a failed job\n");
+        ++g_fail;
+      }
+    Backend::finish_group(lane, g, rc, js, err);
   }
   void release(Request &r) override {
     note("release " + std::to_string(r.ticket));
@@ -316,6 +333,34 @@ static void test_lanes_overlap() {
   std::printf("ok lanes_overlap (max %d)\n", be.max_finishing.load());
 }
 
+static void test_group_finishing() {
+  FakeBackend be;
+  be.grouping = true;
+  be.fail_finish.insert(5);
+  Config c;
+  c.budget = 1ull << 40;
+  c.lanes = 1;
+  c.round_ms = 1.0;
+  Scheduler s(&be, c);
+  std::vector<uint64_t> t;
+  // one row per job: all jobs leave voting in the first rounds and queue up
+  // behind the single lane, which then takes them in groups by key
+  for (int k = 0; k < 10; ++k) t.push_back(s.submit(make_req(8, 8, {0, 1, 2, 3, 99})));
+  for (uint64_t k : t) {
+    std::unique_ptr<Request> r;
+    CHECK(s.wait(k, 60.0, &r) == PPF_OK);
+    if (k == 5) {
+      // target 0 fails in finish; the request fails, its other jobs finish
+      CHECK(r->state == State::Failed && r->status == PPF_ERR_HIP);
+      continue;
+    }
+    CHECK(r->state == State::Done && r->status == PPF_OK);
+    for (size_t j = 0; j < r->poses.size(); ++j) CHECK(r->poses[j][0] == 1.0f && r->poses[j][1] == 1.0f);
+  }
+  CHECK(be.max_group.load() >= 2);
+  std::printf("ok group_finishing (%d groups, max %d jobs)\n", be.groups.load(), be.max_group.load());
+}
+
 int main() {
   test_coverage_and_rounds();
   test_budget_admission();
@@ -324,6 +369,7 @@ int main() {
   test_error_isolation();
   test_cancel_timeout_shutdown();
   test_lanes_overlap();
+  test_group_finishing();
   if (g_fail) {
     std::printf("FAILED\n");
     return 1;

@@ -135,3 +135,44 @@ def test_mixed_per_reference_argmax_requests(ctx):
         srv.close()
         for m in models:
             m.close()
+
+
+def test_grouped_finishing_equals_plain(ctx):
+    """One finishing lane: jobs that leave voting while the lane is busy queue
+    up and are finished together (one launch per stage for the group); jobs
+    with other clustering flags are never grouped with them.  Every result
+    equals the plain lookup of its pair, bit for bit."""
+    w = _workload()
+    models = {}
+    srv_models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    srv = ppf_amd.Server(srv_models, lanes=1, round_ms=1.0, ctx=ctx)
+    try:
+        reqs = [(s, ids, k % 3 == 2) for k, (s, ids) in enumerate(zip(w.scenes, w.scene_models))]
+        reqs = reqs + reqs
+        tickets = [srv.submit(s, ids, w.rpdf, w.vote_count_threshold, use_l1_norm=l1) for s, ids, l1 in reqs]
+        got = []
+        for t in tickets:
+            poses, st, info, res = srv.wait(t, 120.0, results=True)
+            assert info["rc"] == ppf_amd.PPF_OK, info
+            got.append([_digest(r, True) for r in res])
+            for r in res:
+                r.close()
+        stats = srv.stats()
+        print("finish groups:", stats["finish_groups"], "largest:", stats["max_finish_group"], flush=True)
+        assert stats["completed"] == len(reqs) and stats["failed"] == 0
+        assert stats["max_finish_group"] >= 2
+        for (s, ids, l1), dig in zip(reqs, got):
+            sc = ppf_amd.Scene(s, 0.0, w.rpdf, ctx=ctx)
+            for k, mi in enumerate(ids):
+                key = (mi, l1)
+                if key not in models:
+                    models[key] = ppf_amd.Model(w.models[mi], w.d_dists[mi], w.vote_count_threshold,
+                                                use_l1_norm=l1, ctx=ctx)
+                r = models[key].ppf_lookup(sc)
+                assert _digest(r, True) == dig[k]
+                r.close()
+            sc.close()
+    finally:
+        srv.close()
+        for m in srv_models + list(models.values()):
+            m.close()
