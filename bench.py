@@ -5,12 +5,12 @@ Headline workload (the batched configuration): 16 seeded superquadric models
 of 1k-20k points (log-uniform) built once and resident on the device, 64
 scenes of 5k-120k points (log-uniform) each holding 1-4 of them on a ground
 plane with occlusion and noise, ref_point_df 5, threshold 0.3, GPU
-spatial-hash clustering.  All 148 (scene, model) jobs of that configuration
-take about half an hour of GPU time (one job alone -- a 33,856-point scene
-against the 16,863-point model, 1.2e14 votes and 6.5e7 kept poses -- takes
-150 s), so a step here is the fixed subset of jobs with n_scene * n_model
-<= 6e7 (SUBSET_MAX_PRODUCT), in generator order, voted, thresholded, posed
-and clustered by ONE ppf_batch_lookup call.  value = registrations/s of the
+spatial-hash clustering.  The 148 (scene, model) jobs of that configuration
+hold about 320 times the votes of its 25 smallest (sum of n_scene^2 * n_model
+/ ref_point_df: 7.4e14 against 2.3e12), far beyond a bench run, so a step
+here is the fixed subset of jobs with n_scene * n_model <= 6e7
+(SUBSET_MAX_PRODUCT), in generator order, voted, thresholded, posed and
+clustered by ONE ppf_batch_lookup call.  value = registrations/s of the
 batched form over the timed steps; the plain per-scene pipeline (one
 ppf_lookup per job, in sequence) runs the same jobs beside it.
 

@@ -21,3 +21,10 @@ def ctx():
     c = ppf_amd.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def batched_workload():
+    """The batched configuration (16 models, 64 scenes), generated once."""
+    import synth
+    return synth.batched()

@@ -232,12 +232,12 @@ def test_fused_finishing_with_empty_jobs(ctx):
         o.close()
 
 
-def test_batched_configuration_extremes_equal_plain(ctx):
+def test_batched_configuration_extremes_equal_plain(ctx, batched_workload):
     """The batched configuration's largest scene (about 120,000 points) with
     its smallest model, its largest model (about 20,000 points) with the
     smallest scene, and the smallest pair: batched (finished together)
     equals plain, bit for bit."""
-    w = synth.batched()
+    w = batched_workload
     ns = [len(s[0]) for s in w.scenes]
     nm = [len(m[0]) for m in w.models]
     big_s, small_s = int(np.argmax(ns)), int(np.argmin(ns))

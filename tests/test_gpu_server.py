@@ -176,3 +176,44 @@ def test_grouped_finishing_equals_plain(ctx):
         srv.close()
         for m in srv_models + list(models.values()):
             m.close()
+
+
+def test_served_configuration_requests_alone_mix_plain(ctx, batched_workload):
+    """Requests of the served configuration at its sizes (scenes of
+    5k-100k points with several targets among the 16 resident models of the
+    batched configuration; targets kept when n_scene * n_model <= 6e7, the
+    bench's bound): each result is the same alone, in the mix and as a plain
+    lookup, bit for bit."""
+    w = batched_workload
+    reqs = synth.served_requests(w, seed=5, n_requests=6,
+                                 keep=lambda n_s, mi: n_s * len(w.models[mi][0]) <= 6e7)
+    print("requests:", [(len(s[0]), ids) for s, ids in reqs], flush=True)
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    srv = ppf_amd.Server(models, round_ms=20.0, ctx=ctx)
+    try:
+        alone = []
+        for s, ids in reqs:
+            t = srv.submit(s, ids, w.rpdf, w.vote_count_threshold)
+            poses, st, info, res = srv.wait(t, 300.0, results=True)
+            assert info["rc"] == ppf_amd.PPF_OK, info
+            alone.append([_digest(r, True) for r in res])
+            for r in res:
+                r.close()
+        tickets = [srv.submit(s, ids, w.rpdf, w.vote_count_threshold) for s, ids in reqs]
+        for t, dig_a in zip(tickets, alone):
+            poses, st, info, res = srv.wait(t, 300.0, results=True)
+            assert info["rc"] == ppf_amd.PPF_OK, info
+            assert [_digest(r, True) for r in res] == dig_a
+            for r in res:
+                r.close()
+        for (s, ids), dig_a in zip(reqs, alone):
+            sc = ppf_amd.Scene(s, 0.0, w.rpdf, ctx=ctx)
+            for k, mi in enumerate(ids):
+                r = models[mi].ppf_lookup(sc)
+                assert _digest(r, True) == dig_a[k]
+                r.close()
+            sc.close()
+    finally:
+        srv.close()
+        for m in models:
+            m.close()
