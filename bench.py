@@ -264,6 +264,38 @@ def run_large_hash(ctx, rpdf=100):
             "registrations_per_s": 1.0 / dt, "max_idx": i["max_idx"]}
 
 
+def run_dataset_like(ctx, steps=3):
+    """The dataset-like workload (four superquadric models of 3.5k-7.2k
+    points against one 50,000-point scene, ref_point_df 5, threshold 0.3, GPU
+    clustering) through the plain pipeline: one step = the four ppf_lookup
+    calls; per-model vote and clustering kernel times of the last step."""
+    w = synth.dataset_like()
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    scene = ppf_amd.Scene(w.scenes[0], 0.0, w.rpdf, ctx=ctx)
+    per = []
+    t_steps = []
+    for step in range(steps + 1):  # the first is a warm-up
+        t = time.perf_counter()
+        per = []
+        for m in models:
+            r = m.ppf_lookup(scene)
+            r.pose()
+            per.append(dict(r.info))
+            r.close()
+        if step:
+            t_steps.append(time.perf_counter() - t)
+        log(f"dataset-like step {step}: {time.perf_counter() - t:.2f}s")
+    for o in models + [scene]:
+        o.close()
+    L = sum(i["lookups"] for i in per)
+    s = float(np.mean(t_steps))
+    return {"models": [len(m[0]) for m in w.models], "scene_points": len(w.scenes[0][0]), "ref_point_df": w.rpdf,
+            "s_per_step": s, "registrations_per_s": len(per) / s, "pair_lookups_per_s": L / s,
+            "votes": [i["votes"] for i in per], "n_kept": [i["n_kept"] for i in per],
+            "vote_ms": [i["vote_ms"] for i in per], "cluster_kernel_ms": [i["cluster_kernel_ms"] for i in per],
+            "votes_per_s_in_kernel": sum(i["votes"] for i in per) / (sum(i["vote_ms"] for i in per) * 1e-3)}
+
+
 def cpu_info():
     model = None
     try:
@@ -349,6 +381,8 @@ def main():
     ap.add_argument("--large-hash-rpdf", type=int, default=100,
                     help="reference-point stride of the large-hash lookup (5 = the full configuration)")
     ap.add_argument("--large-hash-only", action="store_true")
+    ap.add_argument("--dataset-like-only", action="store_true",
+                    help="only the dataset-like workload through the plain pipeline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     ap.add_argument("--valu-calibration", action="store_true",
@@ -373,6 +407,9 @@ def main():
             b = ctx.debug_copy(a)
         assert np.array_equal(a, b)
         log(f"copied {a.nbytes} bytes x3")
+        return
+    if args.dataset_like_only:
+        print(json.dumps({"dataset_like": run_dataset_like(ppf_amd.Context(0))}), flush=True)
         return
     if args.large_hash_only:
         print(json.dumps({"large_hash": run_large_hash(ppf_amd.Context(0), args.large_hash_rpdf)}),

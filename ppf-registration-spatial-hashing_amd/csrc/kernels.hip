@@ -1058,7 +1058,7 @@ __global__ void k_cluster_prep(const float *__restrict__ T, const unsigned long 
 // wave per block: the waves of a larger block wait for each other at every
 // member tile, and their pose slots test different numbers of members.
 constexpr int CL_BLOCK = 64;
-constexpr int CL_PER = 2;
+constexpr int CL_PER = 2;  // 3 and 4 measured slower
 constexpr int CL_CHUNK = CL_BLOCK * CL_PER;
 
 // Work items of the tiled clustering: (unique cell hash u, CL_CHUNK-pose chunk).
@@ -1365,10 +1365,13 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
   unsigned long long tests = 0;
   if (s_uniform) {
     // one bound per slot: 64 consecutive poses of the rotation order each
-    static_assert(CL_PER == 2, "two pose slots per lane");
-    const ClWave W0 = cluster_slot_bounds(P[0], have[0], d_dist);
-    const ClWave W1 = cluster_slot_bounds(P[1], have[1], d_dist);
-    const bool any0 = __ballot(have[0]) != 0ull, any1 = __ballot(have[1]) != 0ull;
+    ClWave W[CL_PER];
+    bool any[CL_PER];
+#pragma unroll
+    for (int r = 0; r < CL_PER; ++r) {
+      W[r] = cluster_slot_bounds(P[r], have[r], d_dist);
+      any[r] = __ballot(have[r]) != 0ull;
+    }
     for (int blk = 0; blk < 27; ++blk) {
       const uint32_t h = s_adj[blk];
       if (h == 0u) continue;
@@ -1389,20 +1392,28 @@ __global__ __launch_bounds__(CL_BLOCK) void k_cluster_tiled(
         __syncthreads();
         for (uint32_t g0 = 0; g0 < cn; g0 += WAVE) {
           const uint32_t gj = g0 + (uint32_t)lane;
-          bool live0 = false, live1 = false;
+          bool live[CL_PER];
+#pragma unroll
+          for (int r = 0; r < CL_PER; ++r) live[r] = false;
           if (gj < cn) {
             const float4 mq = s_q[gj], mt = s_t[gj];
-            live0 = any0 && !ppfc::cluster_skip(W0, mq.x, mq.y, mq.z, mq.w, mt.x, mt.y, mt.z, use_l1);
-            live1 = any1 && !ppfc::cluster_skip(W1, mq.x, mq.y, mq.z, mq.w, mt.x, mt.y, mt.z, use_l1);
+#pragma unroll
+            for (int r = 0; r < CL_PER; ++r)
+              live[r] = any[r] && !ppfc::cluster_skip(W[r], mq.x, mq.y, mq.z, mq.w, mt.x, mt.y, mt.z, use_l1);
           }
-          const uint64_t m0 = __ballot(live0), m1 = __ballot(live1);
-          uint64_t mask = m0 | m1;
+          uint64_t m[CL_PER], mask = 0;
+#pragma unroll
+          for (int r = 0; r < CL_PER; ++r) {
+            m[r] = __ballot(live[r]);
+            mask |= m[r];
+          }
           while (mask) {
             const uint32_t b = (uint32_t)__builtin_ctzll(mask);
             mask &= mask - 1;
             const float4 q0 = s_q[g0 + b], t0 = s_t[g0 + b];
-            if ((m0 >> b) & 1ull) cl_test<use_l1, use_avg>(P[0], q0, t0, s_max);
-            if ((m1 >> b) & 1ull) cl_test<use_l1, use_avg>(P[1], q0, t0, s_max);
+#pragma unroll
+            for (int r = 0; r < CL_PER; ++r)
+              if ((m[r] >> b) & 1ull) cl_test<use_l1, use_avg>(P[r], q0, t0, s_max);
           }
         }
         __syncthreads();
