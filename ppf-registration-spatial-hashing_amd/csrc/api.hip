@@ -144,7 +144,8 @@ void ctx_release(ppf_ctx_s *c) {
   c->sort.tmp_keys.release();
   c->sort.tmp_vals.release();
   for (DevBuf *b : {&c->tmp_a, &c->tmp_b, &c->tmp_c, &c->tmp_d, &c->hits, &c->queue, &c->sel, &c->fin_meta,
-                    &c->fin_flags, &c->fin_keys, &c->fin_cl, &c->fin_chunk, &c->fin_order, &c->fin_scan})
+                    &c->fin_flags, &c->fin_keys, &c->fin_cl, &c->fin_chunk, &c->fin_order, &c->fin_scan,
+                    &c->fin_kd, &c->tmp_e})
     b->release();
   for (hipEvent_t *ev : {&c->ev0, &c->ev1, &c->ev2, &c->ev3, &c->ev4, &c->ev5, &c->ev6})
     if (*ev) {

@@ -1063,10 +1063,11 @@ constexpr int CL_CHUNK = CL_BLOCK * CL_PER;
 
 // Work items of the tiled clustering: (unique cell hash u, CL_CHUNK-pose chunk).
 __global__ void k_cluster_chunks(const uint32_t *__restrict__ ucount, int64_t U,
-                                 uint32_t *__restrict__ nchunk) {
+                                 uint32_t *__restrict__ nchunk, uint32_t *__restrict__ max_count) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= U) return;
   nchunk[u] = (ucount[u] + (uint32_t)CL_CHUNK - 1u) / (uint32_t)CL_CHUNK;
+  atomicMax(max_count, ucount[u]);
 }
 
 // Order of the poses of a cell for forming clustering chunks: the position
@@ -1661,6 +1662,247 @@ __global__ void k_bfinal(const uint64_t *__restrict__ kbeg, int nj, const unsign
   o[21] = (uint32_t)(jmixed[j] >> 32);
 }
 
+// ------------------------------------------------ k-d slots within cells
+// The poses of each cell are split recursively at the median of the widest
+// of seven coordinates -- the normalized quaternion and the translation
+// times KD_TRANS_SCALE / d_dist -- into runs of WAVE poses: a run of s > 64
+// poses splits after round_up(floor(s / 2), 64) of them, so every slot but a
+// cell's last is full and starts at a multiple of 64 from the cell's first
+// pose.  A slot of a k-d leaf is more compact than 64 poses consecutive in
+// the Hilbert order, so its bounds skip more members.  Which poses share a
+// slot changes no result.
+constexpr float KD_TRANS_SCALE = 0.2f;
+constexpr int64_t KD_MIN_POSES = 8192;  // below this the levels cost more than they save
+
+// The run [b, e) of cell-relative position rel after `level` splits of [0, n).
+__device__ __forceinline__ void kd_run(uint32_t n, uint32_t rel, int level, uint32_t &b, uint32_t &e) {
+  b = 0u;
+  e = n;
+  for (int l = 0; l < level && e - b > (uint32_t)WAVE; ++l) {
+    const uint32_t L = ((e - b) / 2u + (uint32_t)WAVE - 1u) / (uint32_t)WAVE * (uint32_t)WAVE;
+    if (rel < b + L)
+      e = b + L;
+    else
+      b += L;
+  }
+}
+
+// order-preserving u32 of a float (for atomic min/max)
+__device__ __forceinline__ uint32_t ord_f32(float x) {
+  const uint32_t b = ppfm::fbits(x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t o) {
+  return ppfm::bitsf((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// seven k-d coordinates of every pose (job j's translation scale for the
+// poses in [kbeg[j], kbeg[j+1]))
+__global__ void k_kd_feat(const float *__restrict__ trans, const float *__restrict__ quats, int64_t P,
+                          const uint64_t *__restrict__ kbeg, int nj, const float2 *__restrict__ job_dist,
+                          float d_dist, float *__restrict__ feat) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  // job_dist[j] = (s_max, d_dist) of job j when the poses are a batch's
+  const float d = job_dist ? job_dist[job_of(kbeg, nj, (uint64_t)i)].y : d_dist;
+  const float sc = d > 0.0f ? KD_TRANS_SCALE / d : 0.0f;
+  float q[4], n2 = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    q[c] = quats[4 * i + c];
+    n2 += q[c] * q[c];
+  }
+  const float inv = n2 > 0.0f && n2 < 1e30f ? 1.0f / sqrtf(n2) : 0.0f;
+  float *f = feat + 8 * i;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float x = q[c] * inv;
+    f[c] = x == x ? x : 0.0f;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float x = trans[3 * i + c] * sc;
+    f[4 + c] = (x == x && fabsf(x) < 1e30f) ? x : 0.0f;
+  }
+}
+
+// cell[k] = the cell of position k: the last u with ufirst[u] <= k
+__global__ void k_kd_cells(int64_t P, const uint32_t *__restrict__ ufirst, int64_t U, uint32_t *__restrict__ cell) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  int64_t lo = 0, hi = U - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (ufirst[mid] <= (uint32_t)k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  cell[k] = (uint32_t)lo;
+}
+
+// The run of position k (in `centers`) at this level: global first position
+// gb and size; leaf = at most one slot.
+__device__ __forceinline__ void kd_locate(int64_t k, const uint32_t *__restrict__ cell,
+                                          const uint32_t *__restrict__ ufirst,
+                                          const uint32_t *__restrict__ ucount, int level, uint32_t &gb,
+                                          uint32_t &size) {
+  const uint32_t u = cell[k];
+  uint32_t b, e;
+  kd_run(ucount[u], (uint32_t)k - ufirst[u], level, b, e);
+  gb = ufirst[u] + b;
+  size = e - b;
+}
+
+// per run with more than one slot: min and max of each coordinate.  Runs are
+// contiguous and longer than a wave, so a wave spans at most two of them:
+// each is reduced across the wave first (and across the block when the whole
+// block lies in one run) before the atomics.
+constexpr int KD_BLOCK = 1024;
+
+__device__ __forceinline__ void kd_wave_reduce(uint32_t lo[7], uint32_t hi[7]) {
+#pragma unroll
+  for (int c = 0; c < 7; ++c)
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+      lo[c] = min(lo[c], (uint32_t)__shfl_xor((int)lo[c], o, WAVE));
+      hi[c] = max(hi[c], (uint32_t)__shfl_xor((int)hi[c], o, WAVE));
+    }
+}
+
+__global__ __launch_bounds__(KD_BLOCK) void k_kd_bounds(const uint32_t *__restrict__ centers, int64_t P,
+                                                        const uint32_t *__restrict__ cell,
+                                                        const uint32_t *__restrict__ ufirst,
+                                                        const uint32_t *__restrict__ ucount, int level,
+                                                        const float *__restrict__ feat,
+                                                        uint32_t *__restrict__ mm) {
+  __shared__ uint32_t s_gb;
+  __shared__ int s_mixed;
+  __shared__ uint32_t s_lo[KD_BLOCK / WAVE][7], s_hi[KD_BLOCK / WAVE][7];
+  const int tid = threadIdx.x, wv = tid / WAVE, lane = tid % WAVE;
+  const int64_t k = (int64_t)blockIdx.x * KD_BLOCK + tid;
+  const bool valid = k < P;
+  uint32_t gb = 0xffffffffu, size = 0u;
+  if (valid) kd_locate(k, cell, ufirst, ucount, level, gb, size);
+  const bool act = valid && size > (uint32_t)WAVE;
+  uint32_t flo[7], fhi[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    const uint32_t o = act ? ord_f32(feat[8 * (size_t)centers[k] + c]) : 0u;
+    flo[c] = act ? o : 0xffffffffu;
+    fhi[c] = o;
+  }
+  if (tid == 0) {
+    s_gb = 0xffffffffu;
+    s_mixed = 0;
+  }
+  __syncthreads();
+  if (act) atomicMin(&s_gb, gb);
+  __syncthreads();
+  if (act && gb != s_gb) s_mixed = 1;
+  __syncthreads();
+  if (!s_mixed) {
+    // every active position of the block is in one run (or none is active)
+    if (s_gb == 0xffffffffu) return;
+    kd_wave_reduce(flo, fhi);
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < 7; ++c) {
+        s_lo[wv][c] = flo[c];
+        s_hi[wv][c] = fhi[c];
+      }
+    }
+    __syncthreads();
+    if (tid < 7) {
+      uint32_t lo = 0xffffffffu, hi = 0u;
+      for (int q = 0; q < KD_BLOCK / WAVE; ++q) {
+        lo = min(lo, s_lo[q][tid]);
+        hi = max(hi, s_hi[q][tid]);
+      }
+      atomicMin(&mm[16 * (size_t)s_gb + tid], lo);
+      atomicMax(&mm[16 * (size_t)s_gb + 8 + tid], hi);
+    }
+    return;
+  }
+  // at most two runs per wave: the first active lane's, then the rest
+  const uint64_t am = __ballot(act);
+  if (am == 0ull) return;
+  const uint32_t g0 = (uint32_t)__builtin_amdgcn_readlane((int)gb, (int)__builtin_ctzll(am));
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool mine = act && (pass == 0 ? gb == g0 : gb != g0);
+    const uint64_t mm_ = __ballot(mine);
+    if (mm_ == 0ull) break;
+    const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gb, (int)__builtin_ctzll(mm_));
+    uint32_t lo[7], hi[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      lo[c] = mine ? flo[c] : 0xffffffffu;
+      hi[c] = mine ? fhi[c] : 0u;
+    }
+    kd_wave_reduce(lo, hi);
+    if (lane < 7) {
+      uint32_t l = lo[0], h = hi[0];
+#pragma unroll
+      for (int c = 1; c < 7; ++c)
+        if (lane == c) {
+          l = lo[c];
+          h = hi[c];
+        }
+      atomicMin(&mm[16 * (size_t)g + lane], l);
+      atomicMax(&mm[16 * (size_t)g + 8 + lane], h);
+    }
+  }
+}
+
+// reset the bounds of the runs that start at k
+__global__ void k_kd_reset(int64_t P, const uint32_t *__restrict__ cell, const uint32_t *__restrict__ ufirst,
+                           const uint32_t *__restrict__ ucount, int level, uint32_t *__restrict__ mm) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  uint32_t gb, size;
+  kd_locate(k, cell, ufirst, ucount, level, gb, size);
+  if (gb != (uint32_t)k || size <= (uint32_t)WAVE) return;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    mm[16 * (size_t)k + c] = 0xffffffffu;
+    mm[16 * (size_t)k + 8 + c] = 0u;
+  }
+}
+
+// sort key of position k: its run's first position, then its coordinate
+// along the run's widest axis (8 bits: equal ones keep their order, so the
+// split lands within 1/256 of the run's extent of the median); single-slot
+// runs keep their order
+__global__ void k_kd_keys(const uint32_t *__restrict__ centers, int64_t P, const uint32_t *__restrict__ cell,
+                          const uint32_t *__restrict__ ufirst, const uint32_t *__restrict__ ucount, int level,
+                          const float *__restrict__ feat, const uint32_t *__restrict__ mm,
+                          uint64_t *__restrict__ keys) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  uint32_t gb, size;
+  kd_locate(k, cell, ufirst, ucount, level, gb, size);
+  uint32_t q = 0u;
+  if (size > (uint32_t)WAVE) {
+    int best = 0;
+    float wbest = -1.0f, lob = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      const float lo = unord_f32(mm[16 * (size_t)gb + c]), hi = unord_f32(mm[16 * (size_t)gb + 8 + c]);
+      const float wdt = hi - lo;
+      if (wdt > wbest) {
+        wbest = wdt;
+        best = c;
+        lob = lo;
+      }
+    }
+    if (wbest > 0.0f) {
+      const float x = (feat[8 * (size_t)centers[k] + best] - lob) / wbest;
+      q = (uint32_t)(fminf(fmaxf(x, 0.0f), 1.0f) * 255.0f);
+    }
+  }
+  keys[k] = ((uint64_t)gb << 8) | (uint64_t)q;
+}
+
 // ---------------------------------------------------------- diagnostics
 // A stream of independent full-wave FMAs (eight chains per lane, sixteen
 // v_fma_f32 per iteration): the reference VALU issue rate for instruction
@@ -2011,6 +2253,35 @@ int compute_poses(ppf_ctx ctx, const unsigned long long *codes, int64_t P, const
   return 0;
 }
 
+// Reorder each cell's poses in `centers` into k-d slots (the kernels above);
+// n_max = the largest cell.  Cells of at most one slot, and few poses, are
+// left in their order.
+int kd_slots(ppf_ctx ctx, hipStream_t st, uint32_t *centers, int64_t P, const uint32_t *ufirst,
+             const uint32_t *ucount, int64_t U, uint32_t n_max, const float *trans, const float *quats,
+             const uint64_t *kbeg, int nj, const float2 *job_dist, float d_dist, DevBuf &buf) {
+  if (P < KD_MIN_POSES || n_max <= (uint32_t)WAVE || U <= 0) return 0;
+  const int levels = bitlen((uint64_t)((n_max - 1u) / (uint32_t)WAVE)) + 1;
+  PPF_TRY(buf.ensure((size_t)P * (32 + 64 + 8 + 4)));
+  float *feat = buf.as<float>();
+  uint32_t *mm = reinterpret_cast<uint32_t *>(feat + (size_t)P * 8);
+  uint64_t *keys = reinterpret_cast<uint64_t *>(mm + (size_t)P * 16);
+  uint32_t *cell = reinterpret_cast<uint32_t *>(keys + P);
+  hipLaunchKernelGGL(k_kd_feat, dim3(nblk(P, 256)), dim3(256), 0, st, trans, quats, P, kbeg, nj, job_dist, d_dist,
+                     feat);
+  hipLaunchKernelGGL(k_kd_cells, dim3(nblk(P, 256)), dim3(256), 0, st, P, ufirst, U, cell);
+  const int kbits = 8 + std::max(1, bitlen((uint64_t)P));
+  for (int level = 0; level < levels; ++level) {
+    hipLaunchKernelGGL(k_kd_reset, dim3(nblk(P, 256)), dim3(256), 0, st, P, cell, ufirst, ucount, level, mm);
+    hipLaunchKernelGGL(k_kd_bounds, dim3(nblk(P, KD_BLOCK)), dim3(KD_BLOCK), 0, st, centers, P, cell, ufirst,
+                       ucount, level, feat, mm);
+    hipLaunchKernelGGL(k_kd_keys, dim3(nblk(P, 256)), dim3(256), 0, st, centers, P, cell, ufirst, ucount, level,
+                       feat, mm, keys);
+    PPF_HIP(hipGetLastError());
+    PPF_TRY(radix_sort_u64(keys, centers, P, 0, kbits, ctx->sort, st));
+  }
+  return 0;
+}
+
 int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes,
                     const uint32_t *counts, const float *weights, int64_t P, float d_dist,
                     bool use_l1, bool use_avg, float *vc, float *trans_out, float *quats,
@@ -2075,11 +2346,15 @@ int cluster_spatial(ppf_ctx ctx, const float *T, const unsigned long long *codes
   uint32_t *nch = ctx->tmp_a.as<uint32_t>();
   uint32_t *cstart = nch + U + 1;
   uint32_t *ctot = cstart + U;
-  hipLaunchKernelGGL(k_cluster_chunks, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, ucount, U, nch);
+  PPF_HIP(hipMemsetAsync(ctot + 1, 0, 4, ctx->stream));
+  hipLaunchKernelGGL(k_cluster_chunks, dim3(nblk(U, 256)), dim3(256), 0, ctx->stream, ucount, U, nch, ctot + 1);
   PPF_TRY(exclusive_scan_u32(nch, cstart, U, ctx->tmp_d, ctx->stream, ctot));
-  uint32_t n_items = 0;
-  PPF_HIP(hipMemcpyAsync(&n_items, ctot, 4, hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t nit[2] = {0, 0};
+  PPF_HIP(hipMemcpyAsync(nit, ctot, 8, hipMemcpyDeviceToHost, ctx->stream));
   PPF_HIP(hipStreamSynchronize(ctx->stream));
+  const uint32_t n_items = nit[0];
+  PPF_TRY(kd_slots(ctx, ctx->stream, centers, P, ufirst, ucount, U, nit[1], trans, quats, nullptr, 1, nullptr,
+                   d_dist, ctx->tmp_e));
   if (n_items > 0) {
     PPF_TRY(ctx->tmp_c.ensure((size_t)n_items * 8));
     uint64_t *order = ctx->tmp_c.as<uint64_t>();
@@ -2268,11 +2543,15 @@ int finish_batch(ppf_ctx ctx, const std::vector<FinJob> &jobs, const std::vector
   uint32_t *nch = ctx->fin_chunk.as<uint32_t>();
   uint32_t *cstart = nch + U + 1;
   uint32_t *ctot = cstart + U;
-  hipLaunchKernelGGL(k_cluster_chunks, dim3(nblk(U, 256)), dim3(256), 0, st, ucount, U, nch);
+  PPF_HIP(hipMemsetAsync(ctot + 1, 0, 4, st));
+  hipLaunchKernelGGL(k_cluster_chunks, dim3(nblk(U, 256)), dim3(256), 0, st, ucount, U, nch, ctot + 1);
   PPF_TRY(exclusive_scan_u32(nch, cstart, U, ctx->fin_scan, st, ctot));
-  uint32_t n_items = 0;
-  PPF_HIP(hipMemcpyAsync(&n_items, ctot, 4, hipMemcpyDeviceToHost, st));
+  uint32_t nit[2] = {0, 0};
+  PPF_HIP(hipMemcpyAsync(nit, ctot, 8, hipMemcpyDeviceToHost, st));
   PPF_HIP(hipStreamSynchronize(st));
+  const uint32_t n_items = nit[0];
+  PPF_TRY(kd_slots(ctx, st, centers, P, ufirst, ucount, U, nit[1], trans, A.quats, d_kbeg, nj, d_dist, 0.0f,
+                   ctx->fin_kd));
   if (n_items > 0) {
     PPF_TRY(ctx->fin_order.ensure((size_t)n_items * 8));
     uint64_t *order = ctx->fin_order.as<uint64_t>();

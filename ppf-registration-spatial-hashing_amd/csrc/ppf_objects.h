@@ -46,7 +46,8 @@ struct ppf_ctx_s {
   ppfi::DevBuf tmp_a, tmp_b, tmp_c, tmp_d, hits, queue;
   ppfi::DevBuf sel;  // kept votes of the lookup being finished (grows, never shrinks)
   // scratch of the batched finishing (finish_batch)
-  ppfi::DevBuf fin_meta, fin_flags, fin_keys, fin_cl, fin_chunk, fin_order, fin_scan;
+  ppfi::DevBuf fin_meta, fin_flags, fin_keys, fin_cl, fin_chunk, fin_order, fin_scan, fin_kd;
+  ppfi::DevBuf tmp_e;  // k-d slot grouping of a plain lookup's clustering
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr;
   std::vector<ppf_ctx_s *> lanes;  // root only
