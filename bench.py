@@ -285,11 +285,22 @@ def run_dataset_like(ctx, steps=3):
         if step:
             t_steps.append(time.perf_counter() - t)
         log(f"dataset-like step {step}: {time.perf_counter() - t:.2f}s")
+    # the same four jobs as one ppf_batch_lookup (what ppf_registration does
+    # with the models of a scene)
+    b_steps = []
+    for step in range(steps + 1):
+        dt, binfo, _ = run_batched(ctx, models, [scene] * len(models), [(k, k) for k in range(len(models))],
+                                   w.vote_count_threshold)
+        if step:
+            b_steps.append(dt)
     for o in models + [scene]:
         o.close()
     L = sum(i["lookups"] for i in per)
     s = float(np.mean(t_steps))
+    bs = float(np.mean(b_steps))
     return {"models": [len(m[0]) for m in w.models], "scene_points": len(w.scenes[0][0]), "ref_point_df": w.rpdf,
+            "batched_s_per_step": bs, "batched_registrations_per_s": len(per) / bs,
+            "batched_vote_ms": binfo["vote_ms"], "batched_cluster_kernel_ms": binfo["cluster_kernel_ms"],
             "s_per_step": s, "registrations_per_s": len(per) / s, "pair_lookups_per_s": L / s,
             "votes": [i["votes"] for i in per], "n_kept": [i["n_kept"] for i in per],
             "vote_ms": [i["vote_ms"] for i in per], "cluster_kernel_ms": [i["cluster_kernel_ms"] for i in per],

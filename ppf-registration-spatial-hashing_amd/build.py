@@ -26,7 +26,8 @@ FLAGS = [
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", CSRC, "-o", tmp, *srcs]
+    cmd = [HIPCC, *FLAGS, "-I", CSRC, "-o", tmp, *srcs, "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx",
+           "-Wl,-rpath,/opt/rocm/lib"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if verbose or res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

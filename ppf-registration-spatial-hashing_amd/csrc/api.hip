@@ -448,8 +448,11 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
   uint32_t *scount = reinterpret_cast<uint32_t *>(scode + sel_n);
   int64_t P = 0;
   HIPC(hipEventRecord(ctx->ev4, st));
-  TRY(ppfk::select_and_sort(ctx, ccode, ccount, n_cand, max_count, p->vote_count_threshold,
-                            s->cloud.n, m->cloud.n, scode, scount, &P));
+  {
+    TraceRange tr("ppf.select");
+    TRY(ppfk::select_and_sort(ctx, ccode, ccount, n_cand, max_count, p->vote_count_threshold,
+                              s->cloud.n, m->cloud.n, scode, scount, &P));
+  }
   HIPC(hipEventRecord(ctx->ev5, st));
   // progress reports of plain lookups only (lanes finish batched jobs concurrently)
   const ppf_progress_fn prog = ctx == ctx->root ? ctx->root->progress : nullptr;
@@ -457,6 +460,7 @@ int finish_lookup(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params
   r->P = P;
   r->info.n_kept = P;
   if (P > 0) {
+    TraceRange tr("ppf.poses_and_cluster");
     TRY(r->codes.alloc(ctx, (size_t)P * 8));
     TRY(r->counts.alloc(ctx, (size_t)P * 4));
     TRY(r->T.alloc(ctx, (size_t)P * 64));
@@ -789,6 +793,7 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   ARG_CHECK(n >= 2 && n <= PPF_MAX_MODEL_POINTS, "ppf_model_build: need 2 <= n <= 65535 points");
   ARG_CHECK(d_dist > 0.0f && std::isfinite(d_dist), "ppf_model_build: d_dist must be > 0");
   TRY(set_device(ctx));
+  TraceRange tr("ppf.model_build");
   std::unique_ptr<ppf_model_s> m(new ppf_model_s());
   m->ctx = ctx;
   m->d_dist = d_dist;
@@ -896,6 +901,7 @@ int ppf_scene_upload(ppf_ctx ctx, const float *points, const float *normals, int
   *out = nullptr;
   ARG_CHECK(points && normals, "ppf_scene_upload: NULL cloud");
   ARG_CHECK(n >= 2 && n < ((int64_t)1 << 31), "ppf_scene_upload: need 2 <= n < 2^31 points");
+  TraceRange tr("ppf.scene_upload");
   ARG_CHECK(rpdf >= 1, "ppf_scene_upload: ref_point_downsample_factor must be >= 1");
   TRY(set_device(ctx));
   return ppfi::scene_create(ctx, points, normals, n, rpdf, out);
@@ -935,6 +941,7 @@ namespace {
 int lookup_core(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *p, uint32_t fixed_cap,
                 int max_blocks, ppf_result_s *r) {
   hipStream_t st = ctx->stream;
+  TraceRange tr("ppf.lookup");
   r->ctx = ctx->root;
   r->n_m = m->cloud.n;
   r->cpu = p->cpu_clustering != 0;
@@ -954,6 +961,7 @@ int lookup_core(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *
   int64_t votes = 0;
   float vote_ms = 0.0f;
   for (int attempt = 0;; ++attempt) {
+    TraceRange trv("ppf.vote");
     // [meta: 32 B, 256 B reserved][codes: cap x u64][counts: cap x u32].  The
     // counters come first so that the 64-bit vote total stays 8-byte aligned
     // for any cap (a misaligned 64-bit atomic faults).

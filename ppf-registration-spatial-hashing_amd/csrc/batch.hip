@@ -331,13 +331,17 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
           "ppf_batch_lookup: objects belong to another context");
   }
   PPF_HIP(hipSetDevice(root->device));
+  TraceRange tr("ppf.batch");
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<size_t> order;
   std::vector<BatchJob> q;
   make_batch_jobs(n_jobs, models, scenes, params, order, q);
   float vote_ms = 0.0f;
   int launches = 0;
-  PPF_TRY(batch_votes(root, q, params->exact_votes != 0, &vote_ms, &launches));
+  {
+    TraceRange trv("ppf.batch.vote");
+    PPF_TRY(batch_votes(root, q, params->exact_votes != 0, &vote_ms, &launches));
+  }
   std::vector<std::unique_ptr<ppf_result_s>> res;
   const auto t1 = std::chrono::steady_clock::now();
   const int lanes = root->tune_lanes > 0 ? root->tune_lanes : PPF_BATCH_LANES;

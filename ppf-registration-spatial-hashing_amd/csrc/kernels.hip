@@ -2260,6 +2260,7 @@ int kd_slots(ppf_ctx ctx, hipStream_t st, uint32_t *centers, int64_t P, const ui
              const uint32_t *ucount, int64_t U, uint32_t n_max, const float *trans, const float *quats,
              const uint64_t *kbeg, int nj, const float2 *job_dist, float d_dist, DevBuf &buf) {
   if (P < KD_MIN_POSES || n_max <= (uint32_t)WAVE || U <= 0) return 0;
+  TraceRange tr("ppf.cluster.kd_slots");
   const int levels = bitlen((uint64_t)((n_max - 1u) / (uint32_t)WAVE)) + 1;
   PPF_TRY(buf.ensure((size_t)P * (32 + 64 + 8 + 4)));
   float *feat = buf.as<float>();
@@ -2445,6 +2446,7 @@ int finish_batch(ppf_ctx ctx, const std::vector<FinJob> &jobs, const std::vector
     PPF_HIP(hipMemsetAsync(meta + o_amax, 0, o_out - o_amax, st));
   }
   PPF_HIP(hipEventRecord(ctx->ev4, st));
+  TraceRange tr("ppf.finish_together");
 
   // ---- threshold and order (model.cu:148-170)
   PPF_TRY(ctx->fin_flags.ensure((size_t)C * 8 + 16));

@@ -3,11 +3,23 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdint.h>
 
 #include <string>
 
 namespace ppfi {
+
+// A roctx range over a host scope: the stages of the path appear by name
+// in rocprofv3 --marker-trace (the reference's DEBUG stage timers,
+// scene.cu:72-98, model.cu:271-305).  Costs a function call when no tool
+// is attached.
+struct TraceRange {
+  explicit TraceRange(const char *name) { roctxRangePush(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
 
 constexpr int WAVE = 64;
 

@@ -106,6 +106,7 @@ class HipBackend : public ppfs::Backend {
   }
 
   int vote_round(std::vector<ppfs::Slice> &slices, double *ms, std::string &err) override {
+    TraceRange tr("ppf.server.vote_round");
     *ms = 0.0;
     for (int exact = 0; exact < 2; ++exact) {
       std::vector<ppfk::VoteJob> vj;
@@ -160,6 +161,7 @@ class HipBackend : public ppfs::Backend {
   }
 
   int finish(int lane, ppfs::Request &r, ppfs::Job &j, int *job_status, std::string &err) override {
+    TraceRange tr("ppf.server.finish");
     ppf_ctx L = lanes[(size_t)lane];
     if (hipSetDevice(L->device) != hipSuccess) {
       err = "server: hipSetDevice failed";
@@ -198,6 +200,7 @@ class HipBackend : public ppfs::Backend {
 
   void finish_group(int lane, std::vector<std::pair<ppfs::Request *, ppfs::Job *>> &g, std::vector<int> &rc,
                     std::vector<int> &status, std::vector<std::string> &err) override {
+    TraceRange tr("ppf.server.finish_group");
     ppf_ctx L = lanes[(size_t)lane];
     std::vector<FinishInput> in;
     for (auto &x : g) {
