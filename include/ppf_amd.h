@@ -387,7 +387,11 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
  *       known-size reference for hardware traffic counters;
  * op 3: n blocks of 256 threads each running bit_lo iterations of 16
  *       independent FMAs (keys_io receives n*256 floats), the full-rate VALU
- *       reference for instruction counters. */
+ *       reference for instruction counters;
+ * op 4: stable sort of n uint64 keys_io by their high 32 bits as the model
+ *       build sorts its pair words: by the ranks of the distinct high halves
+ *       when there are at most 8,192 of them and n >= 2^20 (vals_io[0] = 1),
+ *       else by the radix sort (vals_io[0] = 0). */
 int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
                         int32_t bit_lo, int32_t bit_hi);
 

@@ -143,6 +143,8 @@ void ctx_release(ppf_ctx_s *c) {
   c->sort.scan.release();
   c->sort.tmp_keys.release();
   c->sort.tmp_vals.release();
+  c->sort.rank.release();
+  c->sort.rank_tab.release();
   for (DevBuf *b : {&c->tmp_a, &c->tmp_b, &c->tmp_c, &c->tmp_d, &c->hits, &c->queue, &c->sel, &c->fin_meta,
                     &c->fin_flags, &c->fin_keys, &c->fin_cl, &c->fin_chunk, &c->fin_order, &c->fin_scan,
                     &c->fin_kd, &c->tmp_e})
@@ -810,7 +812,12 @@ int ppf_model_build(ppf_ctx ctx, const float *points, const float *normals, int6
   ppfi::DevBuf words;
   TRY(words.ensure((size_t)N * 8));
   TRY(ppfk::cloud_pair_words(ctx, m->cloud, d_dist, 1, words.as<uint64_t>()));
-  TRY(ppfi::radix_sort_u64(words.as<uint64_t>(), nullptr, N, 32, 64, ctx->sort, st));
+  {
+    // a model has a few thousand distinct keys: sort by their ranks
+    bool ranked = false;
+    TRY(ppfi::sort_key32_ranked(words.as<uint64_t>(), N, ctx->sort, st, &ranked));
+    if (!ranked) TRY(ppfi::radix_sort_u64(words.as<uint64_t>(), nullptr, N, 32, 64, ctx->sort, st));
+  }
   int64_t U = 0;
   TRY(ppfk::unique_count(ctx, words.as<uint64_t>(), N, &U));
   m->U = U;
@@ -1291,7 +1298,7 @@ int ppf_debug_cluster(ppf_ctx ctx, const float *T, const uint64_t *codes, const 
 int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_io, int64_t n,
                         int32_t bit_lo, int32_t bit_hi) {
   ARG_CHECK(ctx && keys_io && n >= 0, "ppf_debug_primitive: bad arguments");
-  ARG_CHECK(op >= 0 && op <= 3, "ppf_debug_primitive: unknown op");
+  ARG_CHECK(op >= 0 && op <= 4, "ppf_debug_primitive: unknown op");
   TRY(set_device(ctx));
   hipStream_t st = ctx->stream;
   if (op == 3) {
@@ -1312,6 +1319,19 @@ int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_i
     TRY(ppfk::debug_copy(ctx, a.as<uint32_t>(), b.as<uint32_t>(), n));
     HIPC(hipMemcpyAsync(vals_io, b.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
+    return PPF_OK;
+  }
+  if (op == 4) {
+    ARG_CHECK(vals_io, "ppf_debug_primitive: key sort needs vals_io[0] for the path taken");
+    ppfi::DevBuf k;
+    TRY(k.ensure((size_t)n * 8 + 8));
+    HIPC(hipMemcpy(k.ptr, keys_io, (size_t)n * 8, hipMemcpyHostToDevice));
+    bool ranked = false;
+    TRY(ppfi::sort_key32_ranked(k.as<uint64_t>(), n, ctx->sort, st, &ranked));
+    if (!ranked) TRY(ppfi::radix_sort_u64(k.as<uint64_t>(), nullptr, n, 32, 64, ctx->sort, st));
+    HIPC(hipMemcpyAsync(keys_io, k.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    vals_io[0] = ranked ? 1u : 0u;
     return PPF_OK;
   }
   if (op == 0) {

@@ -78,11 +78,17 @@ struct DevBuf {
 // buffers); tmp_keys/tmp_vals are ping-pong space of the same length.
 struct SortScratch {
   DevBuf counts, scan, tmp_keys, tmp_vals;
+  DevBuf rank, rank_tab;  // sort_key32_ranked's key table and perfect hash
   int tile_items = 16;   // keys per thread of a one-sweep tile (8 or 16; tuned)
   int tile_block = 256;  // threads per one-sweep tile (256 or 512; tuned)
 };
 int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
                    SortScratch &s, hipStream_t st);
+// The same result as radix_sort_u64(words, NULL, n, 32, 64, ...) -- a stable
+// sort by the high 32 bits -- in two passes over the ranks of the distinct
+// high halves when there are at most 8,192 of them (*done = true); otherwise
+// (or for fewer than 2^20 words) nothing is done and *done = false.
+int sort_key32_ranked(uint64_t *words, int64_t n, SortScratch &s, hipStream_t st, bool *done);
 
 // ------------------------------------------------------- scans / reduces
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp, hipStream_t st,

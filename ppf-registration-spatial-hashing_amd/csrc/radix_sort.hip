@@ -17,6 +17,8 @@
 
 #include "ppf_internal.h"
 
+#include <vector>
+
 namespace ppfi {
 
 namespace {
@@ -284,17 +286,48 @@ __global__ __launch_bounds__(256) void k_os_starts(uint32_t *__restrict__ hist, 
   }
 }
 
+// Rank table of a sort by few distinct keys (sort_key32_ranked): a perfect
+// hash of the distinct high halves, built on the host once they are known --
+// bucket b = h1(key), slot = h2(key ^ seed(disp[b])) -- so a lookup is two
+// gathers from 16-bit tables small enough for L1 (every key looked up is in
+// the set, so no key is stored).
+constexpr uint32_t RK_EMPTY = 0xFFFFFFFFu;
+__host__ __device__ __forceinline__ uint32_t rk_hash(uint32_t key, int lg) { return (key * 2654435761u) >> (32 - lg); }
+struct RankTable {
+  const uint16_t *disp;  // per bucket
+  const uint16_t *rank;  // per slot
+  uint32_t m1, m2;       // odd multipliers of h1 and h2
+  int lgb, lgs;          // log2 of the buckets and of the slots
+};
+__host__ __device__ __forceinline__ uint32_t rk_bucket(uint32_t key, const RankTable &rt) {
+  return (key * rt.m1) >> (32 - rt.lgb);
+}
+__host__ __device__ __forceinline__ uint32_t rk_slot(uint32_t key, uint32_t d, const RankTable &rt) {
+  return ((key ^ (d * 0x9E3779B9u)) * rt.m2) >> (32 - rt.lgs);
+}
+
+// The digit of word k at `shift`: of the word itself, or (RANKED) of the rank
+// of its high 32 bits.
+template <bool RANKED>
+__device__ __forceinline__ uint32_t os_digit(uint64_t k, int shift, const RankTable &rt) {
+  if (!RANKED) return (uint32_t)(k >> shift) & 255u;
+  const uint32_t key = (uint32_t)(k >> 32);
+  const uint32_t slot = rk_slot(key, rt.disp[rk_bucket(key, rt)], rt);
+  return ((uint32_t)rt.rank[slot] >> shift) & 255u;
+}
+
 // BLOCK threads x ITEMS keys per thread per tile (the tuner picks the shape);
 // threads 0..255 own one digit each in the count and look-back steps.
-template <bool VALS, int BLOCK, int ITEMS>
+template <bool VALS, int BLOCK, int ITEMS, bool RANKED = false>
 __global__ __launch_bounds__(BLOCK) void k_os_pass(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ starts,
-    uint32_t *__restrict__ status, uint32_t *__restrict__ counters) {
+    uint32_t *__restrict__ status, uint32_t *__restrict__ counters, RankTable rt) {
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int WV = BLOCK / WAVE;
   __shared__ uint64_t s_key[TILE];
   __shared__ uint32_t s_val[VALS ? TILE : 1];
+  __shared__ uint8_t s_dig[RANKED ? TILE : 1];  // looked-up digits, in tile order
   __shared__ uint32_t s_cnt[WV][256];
   __shared__ uint32_t s_off[256];   // tile-local start of each digit
   __shared__ uint32_t s_glob[256];  // global position of tile-local position 0 of each digit
@@ -318,10 +351,13 @@ __global__ __launch_bounds__(BLOCK) void k_os_pass(
     if (VALS) v[i] = vin[idx];
   }
   // stable rank inside the wave: slots in order, lanes in order
+  uint32_t dg[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) dg[i] = os_digit<RANKED>(k[i], shift, rt);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const bool valid = base_w + (int64_t)i * WAVE + l < n;
-    const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+    const uint32_t d = dg[i];
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -390,20 +426,152 @@ __global__ __launch_bounds__(BLOCK) void k_os_pass(
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     if (base_w + (int64_t)i * WAVE + l < n) {
-      const uint32_t dd = (uint32_t)(k[i] >> shift) & 255u;
+      const uint32_t dd = dg[i];
       const uint32_t lp = s_off[dd] + s_cnt[w][dd] + rank[i];
       s_key[lp] = k[i];
       if (VALS) s_val[lp] = v[i];
+      if (RANKED) s_dig[lp] = (uint8_t)dd;
     }
   }
   __syncthreads();
   const int nt = (int)min((int64_t)TILE, n - base);
   for (int i = tid; i < nt; i += BLOCK) {
     const uint64_t kk = s_key[i];
-    const uint32_t pos = s_glob[(uint32_t)(kk >> shift) & 255u] + (uint32_t)i;
+    const uint32_t pos = s_glob[RANKED ? (uint32_t)s_dig[i] : os_digit<false>(kk, shift, rt)] + (uint32_t)i;
     kout[pos] = kk;
     if (VALS) vout[pos] = s_val[i];
   }
+}
+
+// ------------------------------------------------ sort by few distinct keys
+// The words' high 32 bits (the PPF keys of a model: a few thousand distinct
+// values) are collected with their counts in one read pass -- a hash table
+// in each block's LDS, merged into a global one -- sorted, and replaced by
+// their rank: two one-sweep passes over the rank's 8-bit digits then place
+// the words (40 bytes moved per word instead of 72 for four passes over the
+// key).  More distinct keys than RK_MAX: the caller sorts by the key itself.
+constexpr int RK_MAX = 8192;  // distinct keys at most (13-bit ranks)
+constexpr int RK_LG = 14;     // global and block tables: 16,384 probed slots (+1)
+constexpr int RK_BLOCK = 1024;
+constexpr int RK_UNROLL = 4;
+
+__global__ void k_rk_init(uint2 *__restrict__ gent, uint32_t *__restrict__ meta) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= (1u << RK_LG)) gent[i] = make_uint2(RK_EMPTY, 0u);
+  if (i < 4) meta[i] = 0u;
+}
+
+// meta[0]: distinct keys, meta[1]: too many (the caller falls back)
+__global__ __launch_bounds__(RK_BLOCK) void k_rk_discover(const uint64_t *__restrict__ words, int64_t n,
+                                                          uint2 *__restrict__ gent,
+                                                          uint32_t *__restrict__ meta) {
+  constexpr uint32_t NS = 1u << RK_LG;
+  __shared__ uint32_t s_tag[NS];
+  __shared__ uint32_t s_cnt[NS + 1];  // [NS]: the key RK_EMPTY
+  __shared__ uint32_t s_n, s_ovf;
+  const int tid = threadIdx.x;
+  for (uint32_t i = tid; i < NS; i += RK_BLOCK) {
+    s_tag[i] = RK_EMPTY;
+    s_cnt[i] = 0u;
+  }
+  if (tid == 0) {
+    s_cnt[NS] = 0u;
+    s_n = 0u;
+    s_ovf = 0u;
+  }
+  __syncthreads();
+  const int64_t step = (int64_t)gridDim.x * RK_BLOCK * RK_UNROLL;
+  for (int64_t i0 = (int64_t)blockIdx.x * RK_BLOCK * RK_UNROLL + tid; i0 < n; i0 += step) {
+    uint32_t kk[RK_UNROLL];
+#pragma unroll
+    for (int u = 0; u < RK_UNROLL; ++u) {
+      const int64_t idx = i0 + (int64_t)u * RK_BLOCK;
+      kk[u] = idx < n ? (uint32_t)(words[idx] >> 32) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < RK_UNROLL; ++u) {
+      if (i0 + (int64_t)u * RK_BLOCK >= n) break;
+      const uint32_t key = kk[u];
+      if (key == RK_EMPTY) {
+        atomicAdd(&s_cnt[NS], 1u);
+        continue;
+      }
+      uint32_t slot = rk_hash(key, RK_LG);
+      uint32_t p = 0;
+      for (; p < NS; ++p) {
+        uint32_t cur = s_tag[slot];
+        if (cur == RK_EMPTY) {
+          cur = atomicCAS(&s_tag[slot], RK_EMPTY, key);
+          if (cur == RK_EMPTY) {
+            if (atomicAdd(&s_n, 1u) >= (uint32_t)RK_MAX) s_ovf = 1u;
+            cur = key;
+          }
+        }
+        if (cur == key) {
+          atomicAdd(&s_cnt[slot], 1u);
+          break;
+        }
+        slot = (slot + 1u) & (NS - 1u);
+      }
+      if (p == NS) s_ovf = 1u;
+    }
+    if (s_ovf) break;
+  }
+  __syncthreads();
+  if (s_ovf) {
+    if (tid == 0) atomicOr(&meta[1], 1u);
+    return;
+  }
+  // merge into the global table
+  for (uint32_t i = tid; i <= NS; i += RK_BLOCK) {
+    const uint32_t c = s_cnt[i];
+    if (c == 0u) continue;
+    uint32_t slot = NS;
+    if (i < NS) {
+      const uint32_t key = s_tag[i];
+      slot = rk_hash(key, RK_LG);
+      uint32_t p = 0;
+      for (; p < NS; ++p) {
+        const uint32_t cur = atomicCAS(&gent[slot].x, RK_EMPTY, key);
+        if (cur == RK_EMPTY) {
+          if (atomicAdd(&meta[0], 1u) >= (uint32_t)RK_MAX) atomicOr(&meta[1], 1u);
+          break;
+        }
+        if (cur == key) break;
+        slot = (slot + 1u) & (NS - 1u);
+      }
+      if (p == NS) {
+        atomicOr(&meta[1], 1u);
+        continue;
+      }
+    } else if (atomicCAS(&gent[NS].x, RK_EMPTY, 0u) == RK_EMPTY) {  // first block with the key RK_EMPTY
+      if (atomicAdd(&meta[0], 1u) >= (uint32_t)RK_MAX) atomicOr(&meta[1], 1u);
+    }
+    atomicAdd(&gent[slot].y, c);
+  }
+}
+
+// the distinct keys, as (key << 32 | slot), in table order
+__global__ void k_rk_compact(const uint2 *__restrict__ gent, uint64_t *__restrict__ ukeys,
+                             uint32_t *__restrict__ pos) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr uint32_t NS = 1u << RK_LG;
+  if (slot > NS) return;
+  const uint32_t tag = gent[slot].x;
+  if (tag == RK_EMPTY) return;
+  const uint32_t key = slot == NS ? RK_EMPTY : tag;
+  const uint32_t p = atomicAdd(pos, 1u);
+  if (p < (uint32_t)RK_MAX) ukeys[p] = ((uint64_t)key << 32) | slot;
+}
+
+// the histograms of both digits of the ranks of the sorted distinct keys
+__global__ void k_rk_hist(const uint64_t *__restrict__ ukeys, uint32_t U, const uint2 *__restrict__ gent,
+                          uint32_t *__restrict__ hist) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= U) return;
+  const uint32_t c = gent[(uint32_t)ukeys[r]].y;
+  atomicAdd(&hist[r & 255u], c);
+  atomicAdd(&hist[256 + (r >> 8)], c);
 }
 
 }  // namespace
@@ -479,7 +647,7 @@ static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bi
       kern = vals ? (items == 8 ? k_os_pass<true, 256, 8> : k_os_pass<true, 256, 16>)
                   : (items == 8 ? k_os_pass<false, 256, 8> : k_os_pass<false, 256, 16>);
     hipLaunchKernelGGL(kern, dim3((unsigned)n_tiles), dim3(block), 0, st, ka, va, kb, vb, n, shift,
-                       hist + 256 * p, status, ctr + 2 * p);
+                       hist + 256 * p, status, ctr + 2 * p, RankTable{});
     PPF_HIP(hipGetLastError());
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -497,6 +665,138 @@ static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bi
       set_error("radix_sort_u64: look-back wait exceeded its spin limit");
       return 2;
     }
+  return 0;
+}
+
+// The perfect hash of the sorted distinct keys (hash and displace): buckets
+// in decreasing size each take the first 16-bit seed that sends all their
+// keys to distinct free slots.  False if some bucket finds none.
+static bool rk_build(const std::vector<uint32_t> &keys, RankTable &rt, std::vector<uint16_t> &disp,
+                     std::vector<uint16_t> &rank) {
+  const uint32_t U = (uint32_t)keys.size();
+  const uint32_t B = 1u << rt.lgb, S = 1u << rt.lgs;
+  std::vector<uint32_t> bstart(B + 1, 0u), members(U);
+  for (uint32_t r = 0; r < U; ++r) ++bstart[rk_bucket(keys[r], rt) + 1];
+  for (uint32_t b = 0; b < B; ++b) bstart[b + 1] += bstart[b];
+  {
+    std::vector<uint32_t> fill(bstart.begin(), bstart.end() - 1);
+    for (uint32_t r = 0; r < U; ++r) members[fill[rk_bucket(keys[r], rt)]++] = r;
+  }
+  std::vector<uint32_t> order(B);
+  for (uint32_t b = 0; b < B; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return bstart[a + 1] - bstart[a] > bstart[b + 1] - bstart[b];
+  });
+  disp.assign(B, 0);
+  rank.assign(S, 0);
+  std::vector<uint8_t> used(S, 0);
+  std::vector<uint32_t> slots;
+  for (uint32_t b : order) {
+    const uint32_t m0 = bstart[b], m1 = bstart[b + 1];
+    if (m0 == m1) break;  // the rest are empty
+    bool placed = false;
+    for (uint32_t d = 0; d < 65536u && !placed; ++d) {
+      slots.clear();
+      bool ok = true;
+      for (uint32_t m = m0; m < m1 && ok; ++m) {
+        const uint32_t sl = rk_slot(keys[members[m]], d, rt);
+        if (used[sl] || std::find(slots.begin(), slots.end(), sl) != slots.end()) ok = false;
+        slots.push_back(sl);
+      }
+      if (!ok) continue;
+      for (uint32_t m = m0; m < m1; ++m) {
+        used[slots[m - m0]] = 1;
+        rank[slots[m - m0]] = (uint16_t)members[m];
+      }
+      disp[b] = (uint16_t)d;
+      placed = true;
+    }
+    if (!placed) return false;
+  }
+  return true;
+}
+
+int sort_key32_ranked(uint64_t *words, int64_t n, SortScratch &s, hipStream_t st, bool *done) {
+  *done = false;
+  if (n < ((int64_t)1 << 20) || n >= ((int64_t)1 << 30)) return 0;
+  constexpr size_t NE = ((size_t)1 << RK_LG) + 1;
+  // [entries: NE uint2][meta: 4 u32 (+ pad)][ukeys: RK_MAX u64]
+  PPF_TRY(s.rank.ensure(NE * 8 + 32 + (size_t)RK_MAX * 8));
+  uint2 *gent = s.rank.as<uint2>();
+  uint32_t *meta = reinterpret_cast<uint32_t *>(gent + NE);
+  uint64_t *ukeys = reinterpret_cast<uint64_t *>(meta + 8);
+  hipLaunchKernelGGL(k_rk_init, dim3((unsigned)((NE + 255) / 256)), dim3(256), 0, st, gent, meta);
+  const unsigned blocks =
+      (unsigned)std::min<int64_t>((n + (int64_t)RK_BLOCK * RK_UNROLL * 16 - 1) / ((int64_t)RK_BLOCK * RK_UNROLL * 16), 256);
+  hipLaunchKernelGGL(k_rk_discover, dim3(blocks), dim3(RK_BLOCK), 0, st, words, n, gent, meta);
+  PPF_HIP(hipGetLastError());
+  uint32_t mh[4] = {0, 0, 0, 0};
+  PPF_HIP(hipMemcpyAsync(mh, meta, 16, hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipStreamSynchronize(st));
+  if (mh[1] != 0u || mh[0] == 0u || mh[0] > (uint32_t)RK_MAX) return 0;  // the caller sorts by the key
+  const uint32_t U = mh[0];
+  // [hist/starts: 2 x 256][counters: 2 per pass][status: n_tiles x 256]; the
+  // tile keeps its digits in LDS too, so 256 x 16 keys leave room for three
+  // tiles per CU
+  constexpr int block = 256, items = 16;
+  const int64_t tile = (int64_t)block * items;
+  const int64_t n_tiles = (n + tile - 1) / tile;
+  const size_t hist_words = 2 * 256, ctr_words = 4;
+  // sized before the sort of the distinct keys, which uses the same scratch
+  PPF_TRY(s.counts.ensure((hist_words + ctr_words + (size_t)n_tiles * 256) * 4));
+  PPF_TRY(s.tmp_keys.ensure((size_t)n * 8));
+  hipLaunchKernelGGL(k_rk_compact, dim3((unsigned)((NE + 255) / 256)), dim3(256), 0, st, gent, ukeys, meta + 2);
+  PPF_HIP(hipGetLastError());
+  PPF_TRY(radix_sort_u64(ukeys, nullptr, U, 32, 64, s, st));
+  std::vector<uint64_t> uk(U);
+  PPF_HIP(hipMemcpyAsync(uk.data(), ukeys, (size_t)U * 8, hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipStreamSynchronize(st));
+  std::vector<uint32_t> keys(U);
+  for (uint32_t r = 0; r < U; ++r) keys[r] = (uint32_t)(uk[r] >> 32);
+  // slots: a power of two >= 1.6 U; buckets: a quarter of that
+  int lgs = 1;
+  while ((1u << lgs) * 5u < U * 8u) ++lgs;
+  RankTable rt{nullptr, nullptr, 0u, 0u, std::max(1, lgs - 2), lgs};
+  std::vector<uint16_t> disp, rank;
+  static const uint32_t mults[4][2] = {{0x9E3779B1u, 0x85EBCA6Bu}, {0xC2B2AE35u, 0x27D4EB2Fu},
+                                       {0x165667B1u, 0xD3A2646Du}, {0xFD7046C5u, 0xB55A4F09u}};
+  bool built = false;
+  for (int t = 0; t < 4 && !built; ++t) {
+    rt.m1 = mults[t][0];
+    rt.m2 = mults[t][1];
+    built = rk_build(keys, rt, disp, rank);
+  }
+  if (!built) return 0;  // the caller sorts by the key
+  PPF_TRY(s.rank_tab.ensure((disp.size() + rank.size()) * 2));
+  uint16_t *d_disp = s.rank_tab.as<uint16_t>();
+  uint16_t *d_rank = d_disp + disp.size();
+  PPF_HIP(hipMemcpy(d_disp, disp.data(), disp.size() * 2, hipMemcpyHostToDevice));
+  PPF_HIP(hipMemcpy(d_rank, rank.data(), rank.size() * 2, hipMemcpyHostToDevice));
+  rt.disp = d_disp;
+  rt.rank = d_rank;
+  uint32_t *hist = s.counts.as<uint32_t>();
+  uint32_t *ctr = hist + hist_words;
+  uint32_t *status = ctr + ctr_words;
+  PPF_HIP(hipMemsetAsync(hist, 0, (hist_words + ctr_words) * 4, st));
+  hipLaunchKernelGGL(k_rk_hist, dim3((U + 255) / 256), dim3(256), 0, st, ukeys, U, gent, hist);
+  hipLaunchKernelGGL(k_os_starts, dim3(1), dim3(256), 0, st, hist, 2);
+  PPF_HIP(hipGetLastError());
+  uint64_t *ka = words, *kb = s.tmp_keys.as<uint64_t>();
+  for (int p = 0; p < 2; ++p) {
+    PPF_HIP(hipMemsetAsync(status, 0, (size_t)n_tiles * 256 * 4, st));
+    hipLaunchKernelGGL((k_os_pass<false, block, items, true>), dim3((unsigned)n_tiles), dim3(block), 0, st, ka,
+                       nullptr, kb, nullptr, n, 8 * p, hist + 256 * p, status, ctr + 2 * p, rt);
+    PPF_HIP(hipGetLastError());
+    std::swap(ka, kb);
+  }
+  uint32_t h_ctr[4];
+  PPF_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof h_ctr, hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipStreamSynchronize(st));
+  if (h_ctr[1] || h_ctr[3]) {
+    set_error("sort_key32_ranked: look-back wait exceeded its spin limit");
+    return 2;
+  }
+  *done = true;
   return 0;
 }
 

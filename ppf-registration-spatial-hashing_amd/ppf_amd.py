@@ -323,6 +323,15 @@ class Context:
         _check(lib().ppf_debug_primitive(self.handle, 1, k.ctypes.data, _ptr(v), len(k), bit_lo, bit_hi))
         return k, v
 
+    def debug_key_sort(self, words: np.ndarray):
+        """The model build's stable sort of pair words by their high 32 bits
+        (ppf_debug_primitive op 4); returns (sorted words, True when the
+        rank-compressed path sorted them)."""
+        k = np.ascontiguousarray(words, dtype=np.uint64).copy()
+        flag = np.zeros(1, np.uint32)
+        _check(lib().ppf_debug_primitive(self.handle, 4, k.ctypes.data, flag.ctypes.data, len(k), 0, 0))
+        return k, bool(flag[0])
+
 
 _default_ctx: Optional[Context] = None
 

@@ -72,6 +72,46 @@ def test_model_hash_array_matches_oracle(ctx, medium):
     assert info["n_unique_keys"] == len(ok) and info["n_pairs"] == len(m[0]) ** 2
 
 
+def test_model_hash_array_ranked_sort_matches_oracle(ctx):
+    """A 1,500-point model (2.25e6 pairs): the build sorts its pair words by
+    the ranks of the distinct keys; the hash array and pair map are the
+    oracle's bit for bit."""
+    w = synth.plain_small(seed=5, n_model=1500, n_clutter=10)
+    m, d = w.models[0], w.d_dists[0]
+    model = ppf_amd.Model(m, d, ctx=ctx)
+    hk, cnt, first, k2d = model.hash_array()
+    keys = O.cloud_keys(m[0], m[1], 1, d)
+    ok, oc, of, ok2d = O.hash_array(keys)
+    assert np.array_equal(hk, ok)
+    assert np.array_equal(cnt, oc.astype(np.uint32))
+    assert np.array_equal(first, of.astype(np.uint32))
+    assert np.array_equal(k2d, ok2d.astype(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["one", "few_with_extremes", "model_like", "at_limit", "over_limit", "short"])
+def test_device_key_sort_ranked(ctx, case):
+    """The model build's sort of pair words by their high 32 bits: by ranks
+    of the distinct keys up to 8,192 of them, by the radix sort beyond (and
+    below 2^20 words); stable either way (random low halves)."""
+    cases = ["one", "few_with_extremes", "model_like", "at_limit", "over_limit", "short"]
+    rng = np.random.default_rng(100 + cases.index(case))
+    n, U = {"one": (1 << 20, 1), "few_with_extremes": ((1 << 20) + 12345, 300),
+            "model_like": ((1 << 22) + 777, 4200), "at_limit": ((1 << 21) + 3, 8192),
+            "over_limit": ((1 << 21) + 3, 8193), "short": ((1 << 20) - 1, 100)}[case]
+    distinct = np.unique(rng.integers(0, 2**32, 4 * U, dtype=np.uint64))
+    distinct = rng.permutation(distinct)[:U]
+    if case == "few_with_extremes":
+        distinct[:2] = [0, 0xFFFFFFFF]
+    hi = distinct[rng.integers(0, U, n)]
+    hi[:U] = distinct  # every distinct key present
+    hi = rng.permutation(hi)
+    words = (hi << np.uint64(32)) | rng.integers(0, 2**32, n, dtype=np.uint64)
+    out, ranked = ctx.debug_key_sort(words)
+    assert ranked == (case not in ("over_limit", "short"))
+    order = np.argsort(words >> np.uint64(32), kind="stable")
+    assert np.array_equal(out, words[order])
+
+
 def test_large_model_hash_array_full_size(ctx):
     """The 30,000-point model (9.0e8 pairs, the large-hash workload) at full
     size: the hash array's invariants (every pair counted once, keys strictly
