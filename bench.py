@@ -515,6 +515,7 @@ def main():
         "batched_split_ms_per_step": {"vote": vote_ms / args.steps,
                                       "finish_wall": sum(i["finish_ms"] for i in step_info) / args.steps,
                                       "finish_together": fused,
+                                      "vote_stage_from_graphs": all(i["vote_graph"] for i in step_info),
                                       "select": sum(i["select_ms"] for i in step_info) / args.steps,
                                       "poses": sum(i["pose_ms"] for i in step_info) / args.steps,
                                       "cluster": sum(i["cluster_ms"] for i in step_info) / args.steps,

@@ -275,7 +275,8 @@ typedef struct {
   double finish_ms;      /* wall time of threshold/order, poses and clustering */
   double total_ms;       /* wall time of the call */
   int32_t fused;         /* 1: the stages after voting ran once for all jobs */
-  int32_t reserved;
+  int32_t vote_graph;    /* 1: the vote stage ran as captured HIP graphs, replayed
+                            while the batch's launch signature stays the same */
   double select_ms;      /* fused: device time of threshold/order of all jobs */
   double pose_ms;        /* fused: device time of the poses of all jobs */
   double cluster_ms;     /* fused: device time of clustering and final poses */
@@ -402,7 +403,9 @@ int ppf_debug_primitive(ppf_ctx ctx, int32_t op, void *keys_io, uint32_t *vals_i
  * keys per thread of the one-sweep radix sort's tiles (8 or 16), knob 2:
  * threads per tile (256 or 512) (the autotuner's choice otherwise).  knob 3:
  * how ppf_batch_lookup finishes its jobs: 0 together when it can (default),
- * 1 job by job on lanes.  Results do not depend on any of them. */
+ * 1 job by job on lanes.  knob 4: ppf_batch_lookup's vote stage from captured
+ * HIP graphs (1, default) or launched directly (0).  Results do not depend on
+ * any of them. */
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value);
 
 /* GPU spatial-hash clustering (Model::ClusterTransformations + max_element,

@@ -139,6 +139,7 @@ void ctx_release(ppf_ctx_s *c) {
     delete c->noalloc;
     c->noalloc = nullptr;
   }
+  ppfk::release_vote_graphs(c);
   c->sort.counts.release();
   c->sort.scan.release();
   c->sort.tmp_keys.release();
@@ -1230,7 +1231,12 @@ int ppf_debug_numerics(ppf_ctx ctx, int32_t op, const float *a, const float *b, 
 
 int ppf_debug_set(ppf_ctx ctx, int32_t knob, int64_t value) {
   ARG_CHECK(ctx, "ppf_debug_set: NULL context");
-  ARG_CHECK(knob >= 0 && knob <= 3, "ppf_debug_set: unknown knob");
+  ARG_CHECK(knob >= 0 && knob <= 4, "ppf_debug_set: unknown knob");
+  if (knob == 4) {
+    ARG_CHECK(value == 0 || value == 1, "ppf_debug_set: vote graphs must be 0 or 1");
+    ctx->root->use_graphs = (int32_t)value;
+    return PPF_OK;
+  }
   if (knob == 3) {
     ARG_CHECK(value == PPF_FINISH_AUTO || value == PPF_FINISH_LANES, "ppf_debug_set: finish mode must be 0 or 1");
     ctx->root->finish_mode = (int32_t)value;

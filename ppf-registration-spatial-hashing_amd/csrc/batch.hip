@@ -37,9 +37,11 @@ namespace {
 namespace ppfi {
 
 int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vote_ms,
-                int *launches) {
+                int *launches, bool *graphs) {
   *vote_ms = 0.0f;
   *launches = 0;
+  const bool want_graph = graphs && *graphs && ctx->root->use_graphs != 0;
+  if (graphs) *graphs = want_graph;
   const size_t nj = jobs.size();
   if (nj == 0) return 0;
   TBuf meta;
@@ -71,7 +73,9 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
     }
     PPF_HIP(hipMemcpyAsync(md, mh.data(), nj * 16, hipMemcpyHostToDevice, ctx->stream));
     float ms = 0.0f;
-    PPF_TRY(ppfk::run_votes(ctx, vj.data(), (int)vj.size(), items, max_tm, max_ns, exact, &ms));
+    bool g = want_graph;
+    PPF_TRY(ppfk::run_votes(ctx, vj.data(), (int)vj.size(), items, max_tm, max_ns, exact, &ms, 0, &g));
+    if (graphs) *graphs = *graphs && g;
     *vote_ms += ms;
     ++*launches;
     PPF_HIP(hipMemcpy(mh.data(), md, nj * 16, hipMemcpyDeviceToHost));
@@ -338,9 +342,10 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
   make_batch_jobs(n_jobs, models, scenes, params, order, q);
   float vote_ms = 0.0f;
   int launches = 0;
+  bool graphs = true;
   {
     TraceRange trv("ppf.batch.vote");
-    PPF_TRY(batch_votes(root, q, params->exact_votes != 0, &vote_ms, &launches));
+    PPF_TRY(batch_votes(root, q, params->exact_votes != 0, &vote_ms, &launches, &graphs));
   }
   std::vector<std::unique_ptr<ppf_result_s>> res;
   const auto t1 = std::chrono::steady_clock::now();
@@ -369,6 +374,7 @@ int ppf_batch_lookup(ppf_ctx ctx, int32_t n_jobs, const ppf_model *models, const
     info->vote_launches = launches;
     info->lanes = fused ? 0 : std::min<int>(lanes, n_jobs);
     info->fused = fused ? 1 : 0;
+    info->vote_graph = graphs ? 1 : 0;
     info->select_ms = stages.select_ms;
     info->pose_ms = stages.pose_ms;
     info->cluster_ms = stages.cluster_ms;

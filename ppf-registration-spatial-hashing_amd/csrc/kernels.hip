@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "ppf_cluster.h"
 #include "ppf_geom.h"
@@ -2134,9 +2135,44 @@ uint32_t vote_hit_stride_u4(uint32_t hit_cap) {
 
 size_t vote_hit_bytes_per_block(int64_t max_ns) { return (size_t)vote_hit_stride_u4((uint32_t)max_ns) * 16; }
 
+// Capture what `body` launches on stream st into an instantiated graph.
+static int capture_graph(hipStream_t st, const std::function<int()> &body, hipGraphExec_t *out) {
+  *out = nullptr;
+  PPF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = body();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != 0) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("vote graph capture: ") + hipGetErrorString(e));
+    return PPF_ERR_HIP;
+  }
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    *out = nullptr;
+    set_error(std::string("vote graph instantiation: ") + hipGetErrorString(ei));
+    return PPF_ERR_HIP;
+  }
+  return 0;
+}
+
+void release_vote_graphs(ppf_ctx ctx) {
+  for (hipGraphExec_t *g : {&ctx->vote_pre, &ctx->vote_run})
+    if (*g) {
+      (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
+}
+
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
-              int64_t max_ns, bool exact, float *vote_ms, int max_blocks) {
+              int64_t max_ns, bool exact, float *vote_ms, int max_blocks, bool *graph) {
   *vote_ms = 0.0f;
+  const bool want_graph = graph && *graph;
+  if (graph) *graph = false;
   if (n_items == 0 || n_jobs <= 0) return 0;
   const uint32_t misc_words = vote_misc_words(max_tm);
   const size_t lds = vote_lds_bytes(misc_words);
@@ -2161,8 +2197,58 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
                          ctx->stream));
   PPF_HIP(hipMemsetAsync(ctx->queue.ptr, 0, 16, ctx->stream));
   // heaviest rows first, when there are more rows than blocks
+  const bool use_order = n_items > (uint64_t)G && n_items < ((uint64_t)1 << 32);
+  if (want_graph && n_items < ((uint64_t)1 << 30)) {
+    // The same launches as below as two captured graphs -- the prelude (queue
+    // reset, row costs, their sort) and the vote launch -- instantiated once
+    // per launch signature and replayed while it stays the same (repeated
+    // batches); the prelude clears with fill kernels, not memset nodes.
+    if (use_order) {
+      PPF_TRY(ctx->tmp_c.ensure((size_t)n_items * 8));
+      PPF_TRY(radix_sort_async_prepare((int64_t)n_items, 32, 64, ctx->sort));
+    }
+    const uint64_t sig[] = {(uint64_t)(uintptr_t)kern, (uint64_t)G, (uint64_t)lds, (uint64_t)n_jobs, n_items,
+                            hit_cap, stride_u4, misc_words, (uint64_t)(uintptr_t)ctx->tmp_a.ptr,
+                            (uint64_t)(uintptr_t)ctx->queue.ptr, (uint64_t)(uintptr_t)ctx->hits.ptr,
+                            (uint64_t)(uintptr_t)ctx->tmp_c.ptr, (uint64_t)(uintptr_t)ctx->sort.counts.ptr,
+                            (uint64_t)(uintptr_t)ctx->sort.tmp_keys.ptr, (uint64_t)ctx->sort.tile_block,
+                            (uint64_t)ctx->sort.tile_items, (uint64_t)use_order};
+    static_assert(sizeof sig == sizeof ctx->vote_sig, "vote graph signature size");
+    if (!ctx->vote_pre || !ctx->vote_run || std::memcmp(sig, ctx->vote_sig, sizeof sig) != 0) {
+      release_vote_graphs(ctx);
+      uint64_t *keys = ctx->tmp_c.as<uint64_t>();
+      PPF_TRY(capture_graph(ctx->stream, [&]() -> int {
+        PPF_TRY(fill_u32(ctx->queue.as<uint32_t>(), 0u, 4, ctx->stream));
+        if (use_order) {
+          hipLaunchKernelGGL(k_row_cost, dim3((unsigned)((n_items + 3) / 4)), dim3(256), 0, ctx->stream,
+                             ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items, keys);
+          PPF_HIP(hipGetLastError());
+          PPF_TRY(radix_sort_u64_async(keys, (int64_t)n_items, 32, 64, ctx->sort, ctx->stream));
+        }
+        return 0;
+      }, &ctx->vote_pre));
+      PPF_TRY(capture_graph(ctx->stream, [&]() -> int {
+        hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream, ctx->tmp_a.as<const VoteJob>(),
+                           n_jobs, n_items, use_order ? (const uint64_t *)keys : nullptr,
+                           ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4,
+                           misc_words);
+        PPF_HIP(hipGetLastError());
+        return 0;
+      }, &ctx->vote_run));
+      std::memcpy(ctx->vote_sig, sig, sizeof sig);
+    }
+    PPF_HIP(hipGraphLaunch(ctx->vote_pre, ctx->stream));
+    PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    PPF_HIP(hipGraphLaunch(ctx->vote_run, ctx->stream));
+    PPF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    PPF_HIP(hipEventSynchronize(ctx->ev1));
+    PPF_HIP(hipEventElapsedTime(vote_ms, ctx->ev0, ctx->ev1));
+    if (use_order) PPF_TRY(radix_sort_async_check((int64_t)n_items, 32, 64, ctx->sort, ctx->stream));
+    *graph = true;
+    return 0;
+  }
   const uint64_t *order = nullptr;
-  if (n_items > (uint64_t)G && n_items < ((uint64_t)1 << 32)) {
+  if (use_order) {
     PPF_TRY(ctx->tmp_c.ensure((size_t)n_items * 8));
     uint64_t *keys = ctx->tmp_c.as<uint64_t>();
     hipLaunchKernelGGL(k_row_cost, dim3((unsigned)((n_items + 3) / 4)), dim3(256), 0, ctx->stream,

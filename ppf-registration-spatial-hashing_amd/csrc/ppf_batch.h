@@ -41,7 +41,10 @@ void make_batch_jobs(int32_t n_jobs, const ppf_model *models, const ppf_scene *s
                      const ppf_lookup_params *params, std::vector<size_t> &order, std::vector<BatchJob> &q);
 // Vote every job in one persistent launch (more only for jobs whose candidate
 // buffer overflowed, rerun with their final maximum preset).
-int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vote_ms, int *launches);
+// graphs (in/out, may be NULL): asked for, the vote launches run as the
+// context's captured vote graphs; true on return when every one did.
+int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vote_ms, int *launches,
+                bool *graphs = nullptr);
 // Finish every job on up to n_lanes lanes; results[k] belongs to jobs[k].
 int batch_finish(ppf_ctx ctx, std::vector<BatchJob> &jobs, const ppf_lookup_params *p, int n_lanes,
                  std::vector<std::unique_ptr<ppf_result_s>> &results);

@@ -64,6 +64,12 @@ struct ppf_ctx_s {
   int32_t tune_lanes = 0;
   // root only: how batches are finished (PPF_FINISH_*; tuner or ppf_debug_set)
   int32_t finish_mode = 0;
+  // the batched vote stage as two instantiated graphs (prelude, vote launch)
+  // and the launch signature they were captured for
+  hipGraphExec_t vote_pre = nullptr, vote_run = nullptr;
+  uint64_t vote_sig[17] = {};
+  // root only: run the batched vote stage from captured graphs (ppf_debug_set)
+  int32_t use_graphs = 1;
   // root only: progress reports of plain lookups (ppf_ctx_set_progress)
   ppf_progress_fn progress = nullptr;
   void *progress_user = nullptr;
@@ -251,9 +257,13 @@ int vote_order(ppf_ctx ctx, uint64_t *words, uint32_t *vals, int64_t N, const Cl
 int build_records(ppf_ctx ctx, const uint32_t *pair, int64_t N, const float *mframe,
                   const CloudDev &m, int32_t TM, uint32_t *rec);
 // max_blocks > 0 caps the persistent grid (each block holds max_ns hits of
-// scratch); results do not depend on the grid size.
+// scratch); results do not depend on the grid size.  graph (in/out): asked
+// for, the launches run as the context's captured vote graphs (*graph = true
+// when they did).
 int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, int max_tm,
-              int64_t max_ns, bool exact, float *vote_ms, int max_blocks = 0);
+              int64_t max_ns, bool exact, float *vote_ms, int max_blocks = 0, bool *graph = nullptr);
+// Destroy the context's captured vote graphs.
+void release_vote_graphs(ppf_ctx ctx);
 // Device scratch of one vote block for scenes of up to max_ns points.
 size_t vote_hit_bytes_per_block(int64_t max_ns);
 // Blocks of a vote launch that run at once (reference rows voted in parallel).

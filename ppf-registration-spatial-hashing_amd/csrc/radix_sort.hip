@@ -443,6 +443,11 @@ __global__ __launch_bounds__(BLOCK) void k_os_pass(
   }
 }
 
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // ------------------------------------------------ sort by few distinct keys
 // The words' high 32 bits (the PPF keys of a model: a few thousand distinct
 // values) are collected with their counts in one read pass -- a hash table
@@ -610,34 +615,61 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp
   return scan_level(in, out, n, tmp.as<uint32_t>(), st, total_out_dev);
 }
 
-// One-sweep form (n < 2^30): a histogram kernel, a start kernel, then one
-// kernel per 8-bit digit.
-static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
-                               SortScratch &s, hipStream_t st) {
-  const int npass = (bit_hi - bit_lo + 7) / 8;
-  // tiles of 512 x 16 keys for sorts of 2^24 keys and more (the model
-  // build's, measured fastest there); the tuned shape below that
+// Tile shape of a one-sweep sort of n keys: 512 x 16 keys for sorts of 2^24
+// keys and more (the model build's, measured fastest there); the tuned shape
+// below that.  Scratch: [hist/starts: 8 x 256][counters: 2 per pass]
+// [status: n_tiles x 256] words.
+struct OsShape {
+  int block, items;
+  int64_t n_tiles;
+  size_t counts_bytes;
+};
+static OsShape onesweep_shape(int64_t n, const SortScratch &s) {
   const bool big = n >= ((int64_t)1 << 24);
-  const int items = big ? 16 : (s.tile_items == 8 ? 8 : 16);
-  const int block = big ? 512 : (s.tile_block == 512 ? 512 : 256);
-  const int64_t tile = (int64_t)block * items;
-  const int64_t n_tiles = (n + tile - 1) / tile;
-  // [hist/starts: npass x 256][counters: 2 per pass][status: n_tiles x 256]
+  OsShape sh;
+  sh.items = big ? 16 : (s.tile_items == 8 ? 8 : 16);
+  sh.block = big ? 512 : (s.tile_block == 512 ? 512 : 256);
+  const int64_t tile = (int64_t)sh.block * sh.items;
+  sh.n_tiles = (n + tile - 1) / tile;
+  sh.counts_bytes = ((size_t)OS_MAX_PASSES * 256 + 2 * OS_MAX_PASSES + (size_t)sh.n_tiles * 256) * 4;
+  return sh;
+}
+
+// One-sweep form (n < 2^30): a histogram kernel, a start kernel, then one
+// kernel per 8-bit digit.  graph_safe: fill kernels instead of memsets and no
+// read-back of the look-back time-out flags (radix_sort_async_check reads
+// them), so that the sequence can be captured in a HIP graph.
+static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
+                               SortScratch &s, hipStream_t st, bool graph_safe = false) {
+  const int npass = (bit_hi - bit_lo + 7) / 8;
+  const OsShape shp = onesweep_shape(n, s);
+  const int items = shp.items, block = shp.block;
+  const int64_t n_tiles = shp.n_tiles;
   const size_t hist_words = (size_t)OS_MAX_PASSES * 256, ctr_words = 2 * OS_MAX_PASSES;
-  PPF_TRY(s.counts.ensure((hist_words + ctr_words + (size_t)n_tiles * 256) * 4));
+  PPF_TRY(s.counts.ensure(shp.counts_bytes));
   PPF_TRY(s.tmp_keys.ensure((size_t)n * 8));
   if (vals) PPF_TRY(s.tmp_vals.ensure((size_t)n * 4));
   uint32_t *hist = s.counts.as<uint32_t>();
   uint32_t *ctr = hist + hist_words;
   uint32_t *status = ctr + ctr_words;
-  PPF_HIP(hipMemsetAsync(hist, 0, (hist_words + ctr_words) * 4, st));
+  auto zero = [&](uint32_t *p, size_t words) -> int {
+    if (!graph_safe) {
+      PPF_HIP(hipMemsetAsync(p, 0, words * 4, st));
+      return 0;
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, p, 0u,
+                       (int64_t)words);
+    PPF_HIP(hipGetLastError());
+    return 0;
+  };
+  PPF_TRY(zero(hist, hist_words + ctr_words));
   const unsigned hist_blocks = (unsigned)std::min<int64_t>((n + OS_BLOCK * 64 - 1) / (OS_BLOCK * 64), 2048);
   hipLaunchKernelGGL(k_os_hist, dim3(hist_blocks), dim3(OS_BLOCK), 0, st, keys, n, bit_lo, npass, hist);
   hipLaunchKernelGGL(k_os_starts, dim3(1), dim3(256), 0, st, hist, npass);
   uint64_t *ka = keys, *kb = s.tmp_keys.as<uint64_t>();
   uint32_t *va = vals, *vb = vals ? s.tmp_vals.as<uint32_t>() : nullptr;
   for (int p = 0; p < npass; ++p) {
-    PPF_HIP(hipMemsetAsync(status, 0, (size_t)n_tiles * 256 * 4, st));
+    PPF_TRY(zero(status, (size_t)n_tiles * 256));
     const int shift = bit_lo + 8 * p;
     decltype(&k_os_pass<true, 256, 16>) kern;
     if (block == 512)
@@ -656,6 +688,7 @@ static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bi
     PPF_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
     if (vals) PPF_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   }
+  if (graph_safe) return 0;
   // look-back time-out flags (never expected)
   uint32_t h_ctr[2 * OS_MAX_PASSES];
   PPF_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof h_ctr, hipMemcpyDeviceToHost, st));
@@ -797,6 +830,52 @@ int sort_key32_ranked(uint64_t *words, int64_t n, SortScratch &s, hipStream_t st
     return 2;
   }
   *done = true;
+  return 0;
+}
+
+int fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+  PPF_HIP(hipGetLastError());
+  return 0;
+}
+
+static bool async_sortable(int64_t n, int bit_lo, int bit_hi) {
+  return n < ((int64_t)1 << 30) && bit_hi > bit_lo && bit_hi - bit_lo <= 8 * OS_MAX_PASSES;
+}
+
+int radix_sort_async_prepare(int64_t n, int bit_lo, int bit_hi, SortScratch &s) {
+  if (n <= 1 || bit_hi <= bit_lo) return 0;
+  if (!async_sortable(n, bit_lo, bit_hi)) {
+    set_error("radix_sort_u64_async: more than 2^30 keys or 64 bits");
+    return 1;
+  }
+  PPF_TRY(s.counts.ensure(onesweep_shape(n, s).counts_bytes));
+  PPF_TRY(s.tmp_keys.ensure((size_t)n * 8));
+  return 0;
+}
+
+int radix_sort_u64_async(uint64_t *keys, int64_t n, int bit_lo, int bit_hi, SortScratch &s, hipStream_t st) {
+  if (n <= 1 || bit_hi <= bit_lo) return 0;
+  if (!async_sortable(n, bit_lo, bit_hi)) {
+    set_error("radix_sort_u64_async: more than 2^30 keys or 64 bits");
+    return 1;
+  }
+  return radix_sort_onesweep(keys, nullptr, n, bit_lo, bit_hi, s, st, true);
+}
+
+int radix_sort_async_check(int64_t n, int bit_lo, int bit_hi, SortScratch &s, hipStream_t st) {
+  if (n <= 1 || bit_hi <= bit_lo) return 0;
+  const int npass = (bit_hi - bit_lo + 7) / 8;
+  uint32_t h_ctr[2 * OS_MAX_PASSES];
+  PPF_HIP(hipMemcpyAsync(h_ctr, s.counts.as<uint32_t>() + (size_t)OS_MAX_PASSES * 256, sizeof h_ctr,
+                         hipMemcpyDeviceToHost, st));
+  PPF_HIP(hipStreamSynchronize(st));
+  for (int p = 0; p < npass; ++p)
+    if (h_ctr[2 * p + 1]) {
+      set_error("radix_sort_u64_async: look-back wait exceeded its spin limit");
+      return 2;
+    }
   return 0;
 }
 

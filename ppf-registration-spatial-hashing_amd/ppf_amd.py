@@ -67,7 +67,7 @@ class BatchInfo(C.Structure):
     _fields_ = [("n_jobs", C.c_int64), ("lookups", C.c_int64), ("votes", C.c_int64),
                 ("vote_launches", C.c_int32), ("lanes", C.c_int32), ("vote_ms", C.c_double),
                 ("finish_ms", C.c_double), ("total_ms", C.c_double), ("fused", C.c_int32),
-                ("reserved", C.c_int32), ("select_ms", C.c_double), ("pose_ms", C.c_double),
+                ("vote_graph", C.c_int32), ("select_ms", C.c_double), ("pose_ms", C.c_double),
                 ("cluster_ms", C.c_double), ("cluster_kernel_ms", C.c_double)]
 
 

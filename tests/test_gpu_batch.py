@@ -199,6 +199,50 @@ def test_fused_finishing_equals_lanes(ctx, use_l1, use_avg):
         o.close()
 
 
+def test_vote_graph_replays_equal_direct_launches():
+    """The batched vote stage captured as HIP graphs: the first call captures,
+    repeated calls replay (with plain lookups and another batch, which
+    re-captures, in between), and every replay gives the results of the
+    directly launched vote stage and of the plain lookups, bit for bit."""
+    c = ppf_amd.Context(0)
+    try:
+        w = _small_batch(seed=26)
+        models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=c) for m, d in zip(w.models, w.d_dists)]
+        scenes = [ppf_amd.Scene(s, 0.0, w.rpdf, ctx=c) for s in w.scenes]
+        jm, js = _jobs(w, models, scenes)
+        c.debug_set(4, 0)
+        direct, di = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
+        c.debug_set(4, 1)
+        assert di["vote_graph"] == 0
+        plain = [m.ppf_lookup(s) for m, s in zip(jm, js)]
+        for m, s, a, p in zip(jm, js, direct, plain):
+            _same(a, p, True)
+        calls = []
+        for rep in range(4):
+            if rep == 2:  # another batch in between: a different launch signature
+                other, oi = ppf_amd.batch_lookup(jm[:2], js[:2], w.vote_count_threshold, ctx=c)
+                assert oi["vote_graph"] == 1
+                for m, s, r in zip(jm[:2], js[:2], other):
+                    q = m.ppf_lookup(s)
+                    _same(r, q, True)
+                    q.close()
+                for o in other:
+                    o.close()
+            res, info = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
+            assert info["vote_graph"] == 1
+            for a, b in zip(res, direct):
+                _same(a, b, True)
+            # eager work between replays
+            q = jm[0].ppf_lookup(js[0])
+            _same(q, plain[0], True)
+            q.close()
+            calls.append(res)
+        for o in direct + plain + [r for res in calls for r in res] + models + scenes:
+            o.close()
+    finally:
+        c.close()
+
+
 def test_fused_finishing_with_empty_jobs(ctx):
     """Jobs without votes (a scene whose pairs match no model key) among jobs
     with votes, and a batch where no job keeps a vote: same status and
