@@ -394,6 +394,10 @@ def main():
     ap.add_argument("--large-hash-only", action="store_true")
     ap.add_argument("--dataset-like-only", action="store_true",
                     help="only the dataset-like workload through the plain pipeline")
+    ap.add_argument("--dataset-like-steps", type=int, default=3,
+                    help="timed steps of the dataset-like workload (after one warm-up step)")
+    ap.add_argument("--served-only", action="store_true",
+                    help="only the served form (and the plain pipeline serving the same arrivals)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     ap.add_argument("--valu-calibration", action="store_true",
@@ -420,7 +424,8 @@ def main():
         log(f"copied {a.nbytes} bytes x3")
         return
     if args.dataset_like_only:
-        print(json.dumps({"dataset_like": run_dataset_like(ppf_amd.Context(0))}), flush=True)
+        print(json.dumps({"dataset_like": run_dataset_like(ppf_amd.Context(0), args.dataset_like_steps)}),
+              flush=True)
         return
     if args.large_hash_only:
         print(json.dumps({"large_hash": run_large_hash(ppf_amd.Context(0), args.large_hash_rpdf)}),
@@ -440,6 +445,9 @@ def main():
     models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
     build_s = time.time() - t0
     log(f"16 models built in {build_s:.2f}s")
+    if args.served_only:
+        print(json.dumps({"served": run_served(ctx, models, w)}), flush=True)
+        return
     used = sorted({si for si, _ in wide})
     scenes = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in used}
 

@@ -1786,10 +1786,23 @@ __global__ __launch_bounds__(KD_BLOCK) void k_kd_bounds(const uint32_t *__restri
   uint32_t gb = 0xffffffffu, size = 0u;
   if (valid) kd_locate(k, cell, ufirst, ucount, level, gb, size);
   const bool act = valid && size > (uint32_t)WAVE;
+  // the pose's 32-byte record as two 16-byte loads (one cache line)
+  float fv[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (act) {
+    const float4 *rp = reinterpret_cast<const float4 *>(feat + 8 * (size_t)centers[k]);
+    const float4 a = rp[0], b = rp[1];
+    fv[0] = a.x;
+    fv[1] = a.y;
+    fv[2] = a.z;
+    fv[3] = a.w;
+    fv[4] = b.x;
+    fv[5] = b.y;
+    fv[6] = b.z;
+  }
   uint32_t flo[7], fhi[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
-    const uint32_t o = act ? ord_f32(feat[8 * (size_t)centers[k] + c]) : 0u;
+    const uint32_t o = act ? ord_f32(fv[c]) : 0u;
     flo[c] = act ? o : 0xffffffffu;
     fhi[c] = o;
   }
@@ -2205,7 +2218,7 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
     // batches); the prelude clears with fill kernels, not memset nodes.
     if (use_order) {
       PPF_TRY(ctx->tmp_c.ensure((size_t)n_items * 8));
-      PPF_TRY(radix_sort_async_prepare((int64_t)n_items, 32, 64, ctx->sort));
+      PPF_TRY(radix_sort_async_prepare((int64_t)n_items, 32, 64, ctx->sort, false, ctx->stream));
     }
     const uint64_t sig[] = {(uint64_t)(uintptr_t)kern, (uint64_t)G, (uint64_t)lds, (uint64_t)n_jobs, n_items,
                             hit_cap, stride_u4, misc_words, (uint64_t)(uintptr_t)ctx->tmp_a.ptr,
@@ -2223,7 +2236,7 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
           hipLaunchKernelGGL(k_row_cost, dim3((unsigned)((n_items + 3) / 4)), dim3(256), 0, ctx->stream,
                              ctx->tmp_a.as<const VoteJob>(), n_jobs, n_items, keys);
           PPF_HIP(hipGetLastError());
-          PPF_TRY(radix_sort_u64_async(keys, (int64_t)n_items, 32, 64, ctx->sort, ctx->stream));
+          PPF_TRY(radix_sort_u64_async(keys, nullptr, (int64_t)n_items, 32, 64, ctx->sort, ctx->stream));
         }
         return 0;
       }, &ctx->vote_pre));
@@ -2357,6 +2370,9 @@ int kd_slots(ppf_ctx ctx, hipStream_t st, uint32_t *centers, int64_t P, const ui
                      feat);
   hipLaunchKernelGGL(k_kd_cells, dim3(nblk(P, 256)), dim3(256), 0, st, P, ufirst, U, cell);
   const int kbits = 8 + std::max(1, bitlen((uint64_t)P));
+  // the levels' sorts are queued without host round trips; their look-back
+  // time-out flags are read once, after the last
+  PPF_TRY(radix_sort_async_prepare(P, 0, kbits, ctx->sort, true, st));
   for (int level = 0; level < levels; ++level) {
     hipLaunchKernelGGL(k_kd_reset, dim3(nblk(P, 256)), dim3(256), 0, st, P, cell, ufirst, ucount, level, mm);
     hipLaunchKernelGGL(k_kd_bounds, dim3(nblk(P, KD_BLOCK)), dim3(KD_BLOCK), 0, st, centers, P, cell, ufirst,
@@ -2364,8 +2380,9 @@ int kd_slots(ppf_ctx ctx, hipStream_t st, uint32_t *centers, int64_t P, const ui
     hipLaunchKernelGGL(k_kd_keys, dim3(nblk(P, 256)), dim3(256), 0, st, centers, P, cell, ufirst, ucount, level,
                        feat, mm, keys);
     PPF_HIP(hipGetLastError());
-    PPF_TRY(radix_sort_u64(keys, centers, P, 0, kbits, ctx->sort, st));
+    PPF_TRY(radix_sort_u64_async(keys, centers, P, 0, kbits, ctx->sort, st));
   }
+  PPF_TRY(radix_sort_async_check(P, 0, kbits, ctx->sort, st));
   return 0;
 }
 

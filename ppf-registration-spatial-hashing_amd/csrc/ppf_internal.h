@@ -84,12 +84,15 @@ struct SortScratch {
 };
 int radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
                    SortScratch &s, hipStream_t st);
-// Keys-only radix_sort_u64 (n < 2^30) that can be captured in a HIP graph:
-// no memset, no allocation (radix_sort_async_prepare sizes the scratch first)
-// and no host synchronization; radix_sort_async_check reads the look-back
-// time-out flags once the sort has run.
-int radix_sort_async_prepare(int64_t n, int bit_lo, int bit_hi, SortScratch &s);
-int radix_sort_u64_async(uint64_t *keys, int64_t n, int bit_lo, int bit_hi, SortScratch &s, hipStream_t st);
+// radix_sort_u64 (n < 2^30) that can be queued back to back and captured in
+// a HIP graph: no allocation, no memset (fill kernels) and no host
+// synchronization.  radix_sort_async_prepare (outside any capture) sizes the
+// scratch for sorts of up to n keys (with values if `vals`) and clears the
+// look-back time-out flags, which the sorts that follow only set;
+// radix_sort_async_check reads them once those sorts have run.
+int radix_sort_async_prepare(int64_t n, int bit_lo, int bit_hi, SortScratch &s, bool vals, hipStream_t st);
+int radix_sort_u64_async(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, SortScratch &s,
+                         hipStream_t st);
 int radix_sort_async_check(int64_t n, int bit_lo, int bit_hi, SortScratch &s, hipStream_t st);
 // p[0..n) = v by a kernel (a graph node, where a memset node is not wanted)
 int fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t st);
