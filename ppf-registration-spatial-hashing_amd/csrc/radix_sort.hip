@@ -641,10 +641,10 @@ static OsShape onesweep_shape(int64_t n, const SortScratch &s) {
 }
 
 // One-sweep form (n < 2^30): a histogram kernel, a start kernel, then one
-// kernel per 8-bit digit.  async: fill kernels instead of memsets (measured
-// faster per call than the runtime's fill, and capturable in a HIP graph) and
-// no read-back of the look-back time-out flags (radix_sort_async_check reads
-// them), so that sorts queue back to back.
+// kernel per 8-bit digit.  Counters are cleared by a fill kernel (measured
+// faster per call than the runtime's memset, and capturable in a HIP graph).
+// async: no read-back of the look-back time-out flags (radix_sort_async_check
+// reads them), so that sorts queue back to back.
 static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi,
                                SortScratch &s, hipStream_t st, bool async = false) {
   const int npass = (bit_hi - bit_lo + 7) / 8;
@@ -659,10 +659,6 @@ static int radix_sort_onesweep(uint64_t *keys, uint32_t *vals, int64_t n, int bi
   uint32_t *ctr = hist + hist_words;
   uint32_t *status = ctr + ctr_words;
   auto zero = [&](uint32_t *p, size_t words) -> int {
-    if (!async) {
-      PPF_HIP(hipMemsetAsync(p, 0, words * 4, st));
-      return 0;
-    }
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, p, 0u,
                        (int64_t)words);
     PPF_HIP(hipGetLastError());
