@@ -2230,7 +2230,7 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
     if (!ctx->vote_pre || !ctx->vote_run || std::memcmp(sig, ctx->vote_sig, sizeof sig) != 0) {
       release_vote_graphs(ctx);
       uint64_t *keys = ctx->tmp_c.as<uint64_t>();
-      PPF_TRY(capture_graph(ctx->stream, [&]() -> int {
+      int rc = capture_graph(ctx->stream, [&]() -> int {
         PPF_TRY(fill_u32(ctx->queue.as<uint32_t>(), 0u, 4, ctx->stream));
         if (use_order) {
           hipLaunchKernelGGL(k_row_cost, dim3((unsigned)((n_items + 3) / 4)), dim3(256), 0, ctx->stream,
@@ -2239,17 +2239,26 @@ int run_votes(ppf_ctx ctx, const VoteJob *h_jobs, int n_jobs, uint64_t n_items, 
           PPF_TRY(radix_sort_u64_async(keys, nullptr, (int64_t)n_items, 32, 64, ctx->sort, ctx->stream));
         }
         return 0;
-      }, &ctx->vote_pre));
-      PPF_TRY(capture_graph(ctx->stream, [&]() -> int {
-        hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream, ctx->tmp_a.as<const VoteJob>(),
-                           n_jobs, n_items, use_order ? (const uint64_t *)keys : nullptr,
-                           ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4,
-                           misc_words);
-        PPF_HIP(hipGetLastError());
-        return 0;
-      }, &ctx->vote_run));
-      std::memcpy(ctx->vote_sig, sig, sizeof sig);
+      }, &ctx->vote_pre);
+      if (rc == 0)
+        rc = capture_graph(ctx->stream, [&]() -> int {
+          hipLaunchKernelGGL(kern, dim3(G), dim3(VOTE_BLOCK), lds, ctx->stream, ctx->tmp_a.as<const VoteJob>(),
+                             n_jobs, n_items, use_order ? (const uint64_t *)keys : nullptr,
+                             ctx->queue.as<unsigned long long>(), ctx->hits.as<uint4>(), hit_cap, stride_u4,
+                             misc_words);
+          PPF_HIP(hipGetLastError());
+          return 0;
+        }, &ctx->vote_run);
+      if (rc == 0) {
+        std::memcpy(ctx->vote_sig, sig, sizeof sig);
+      } else {
+        // a runtime that cannot capture these launches: launch them directly
+        release_vote_graphs(ctx);
+        (void)hipGetLastError();
+      }
     }
+  }
+  if (want_graph && ctx->vote_pre && ctx->vote_run && n_items < ((uint64_t)1 << 30)) {
     PPF_HIP(hipGraphLaunch(ctx->vote_pre, ctx->stream));
     PPF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     PPF_HIP(hipGraphLaunch(ctx->vote_run, ctx->stream));
