@@ -100,7 +100,7 @@ def kernel_roofline(name, ms_per_launch, alg_bytes_per_launch=None):
     # kernel with partly masked waves can exceed it)
     out["valu_vs_fma_stream"] = c.get("valu_vs_fma_stream")
     if alg_bytes_per_launch is not None:
-        # the survey's algorithmic bytes (8 B per lookup + 4 B per vote) over
+        # the vote stage's algorithmic bytes (8 B per lookup + 4 B per vote) over
         # the launch time, as a ratio to the HBM peak: >1 means the kernel
         # serves the per-vote pair ids from L2/LDS instead of HBM
         out["alg_bytes_per_launch"] = alg_bytes_per_launch
