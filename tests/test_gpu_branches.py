@@ -50,6 +50,42 @@ def test_plain_poses_take_no_shared_hash_path(ctx):
     assert np.array_equal(_bits(got["vote_counts"]), _bits(ref["vote_counts"]))
 
 
+def _quat_to_mat(q):
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], 1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], 1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)], 1)
+
+
+@pytest.mark.parametrize("P,cells", [(20000, 20), (70000, 200)])
+def test_kd_grouped_cells_match_oracle(ctx, P, cells):
+    """Cells of hundreds to a thousand poses: the clustering regroups each
+    cell's poses as k-d leaves (level sorts of three passes below 2^16 poses,
+    four above), which changes no result: vote counts, translations and the
+    maximum equal the oracle's bit for bit."""
+    rng = np.random.default_rng(P)
+    d = 0.5
+    base = rng.normal(size=(6, 4))
+    base /= np.linalg.norm(base, axis=1, keepdims=True)
+    q = base[rng.integers(0, 6, P)] + rng.normal(scale=0.05, size=(P, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    T = np.tile(np.eye(4), (P, 1, 1))
+    T[:, :3, :3] = _quat_to_mat(q)
+    side = int(np.ceil(cells ** (1 / 3)))
+    cell = rng.integers(0, side, (P, 3))
+    T[:, :3, 3] = (cell + rng.uniform(0.05, 0.95, (P, 3))) * d
+    T = T.astype(np.float32)
+    codes = np.arange(P, dtype=np.uint64) << np.uint64(6)
+    counts = rng.integers(1, 50, P).astype(np.uint32)
+    print(f"k-d grouped clustering: {P} poses in about {side ** 3} cells", flush=True)
+    got = ctx.debug_cluster(T, codes, counts, d)
+    ref = O.cluster_spatial(T, codes, counts, d)
+    assert np.array_equal(_bits(got["vote_counts"]), _bits(ref["vote_counts"]))
+    assert np.array_equal(_bits(got["trans"]), _bits(ref["trans"]))
+    assert got["max_idx"] == ref["max_idx"]
+
+
 @pytest.mark.parametrize("per_ref", [False, True])
 def test_hit_by_hit_rows_match_oracle(per_ref):
     w = synth.plain_small(seed=3, n_model=500, n_clutter=700)
