@@ -146,7 +146,7 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else None
 
 
-def run_served(ctx, models, w, rates=(50.0, 200.0, 2000.0), n_requests=24):
+def run_served(ctx, models, w, rates=(50.0, 200.0, 2000.0), n_requests=24, round_ms=20.0):
     """Served form against the plain pipeline on the same Poisson arrivals."""
     t0 = time.time()
     reqs = synth.served_requests(
@@ -170,7 +170,7 @@ def run_served(ctx, models, w, rates=(50.0, 200.0, 2000.0), n_requests=24):
     plain_request(*reqs[0])  # warm-up
     rng = np.random.default_rng(11)
     out = []
-    srv = ppf_amd.Server(models, round_ms=20.0, ctx=ctx)
+    srv = ppf_amd.Server(models, round_ms=round_ms, ctx=ctx)
     for rate in rates:
         arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
         arr -= arr[0]
@@ -398,6 +398,8 @@ def main():
                     help="timed steps of the dataset-like workload (after one warm-up step)")
     ap.add_argument("--served-only", action="store_true",
                     help="only the served form (and the plain pipeline serving the same arrivals)")
+    ap.add_argument("--served-round-ms", type=float, default=20.0,
+                    help="target duration of one vote round of the server")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     ap.add_argument("--valu-calibration", action="store_true",
@@ -446,7 +448,7 @@ def main():
     build_s = time.time() - t0
     log(f"16 models built in {build_s:.2f}s")
     if args.served_only:
-        print(json.dumps({"served": run_served(ctx, models, w)}), flush=True)
+        print(json.dumps({"served": run_served(ctx, models, w, round_ms=args.served_round_ms)}), flush=True)
         return
     used = sorted({si for si, _ in wide})
     scenes = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in used}
@@ -561,7 +563,7 @@ def main():
     for s in scenes.values():
         s.close()
 
-    out["served"] = run_served(ctx, models, w)
+    out["served"] = run_served(ctx, models, w, round_ms=args.served_round_ms)
     out["large_hash"] = run_large_hash(ctx, args.large_hash_rpdf)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
