@@ -6,8 +6,9 @@ of 1k-20k points (log-uniform) built once and resident on the device, 64
 scenes of 5k-120k points (log-uniform) each holding 1-4 of them on a ground
 plane with occlusion and noise, ref_point_df 5, threshold 0.3, GPU
 spatial-hash clustering.  The 148 (scene, model) jobs of that configuration
-hold about 320 times the votes of its 25 smallest (sum of n_scene^2 * n_model
-/ ref_point_df: 7.4e14 against 2.3e12), far beyond a bench run, so a step
+hold about 1,000 times the votes of its 25 smallest (votes per lookup grow
+with the mean bucket, about n_model^2 / unique keys), far beyond a bench run
+(--full-config runs them all: about half an hour batched), so a step
 here is the fixed subset of jobs with n_scene * n_model <= 6e7
 (SUBSET_MAX_PRODUCT), in generator order, voted, thresholded, posed and
 clustered by ONE ppf_batch_lookup call.  value = registrations/s of the
@@ -140,6 +141,52 @@ def run_plain(models, scenes, jobs):
         infos.append(dict(r.info))
         r.close()
     return time.perf_counter() - t0, lat, infos
+
+
+def run_full_config(ctx, form, n_chunks=60):
+    """All 148 jobs of the batched configuration (the headline subset is its 25
+    smallest), through one form: `batched` -- ppf_batch_lookup over
+    consecutive jobs in calls of about equal cost (sum of n_scene^2 * n_model;
+    a heavier job is a call of its own), so that each call reports before the
+    next -- or `plain` -- one ppf_lookup per job.  Seconds exclude model builds
+    and scene uploads."""
+    w = synth.batched()
+    jobs = subset(w, float("inf"))
+    models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
+    cost = [float(len(w.scenes[si][0])) ** 2 * len(w.models[mi][0]) for si, mi in jobs]
+    budget = sum(cost) / n_chunks
+    chunks, cur, acc = [], [], 0.0
+    for j, c in zip(jobs, cost):
+        if cur and acc + c > budget:
+            chunks.append(cur)
+            cur, acc = [], 0.0
+        cur.append(j)
+        acc += c
+    if cur:
+        chunks.append(cur)
+    total, votes, lookups = 0.0, 0, 0
+    if form == "plain":
+        ctx.set_progress(lambda st, done, tot: log(f"  {st}: {done}/{tot}"), 4000)
+    for ci, chunk in enumerate(chunks):
+        scenes = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in sorted({si for si, _ in chunk})}
+        if form == "batched":
+            dt, info, _ = run_batched(ctx, models, scenes, chunk, w.vote_count_threshold)
+            votes += info["votes"]
+            lookups += info["lookups"]
+        else:
+            dt, _, infos = run_plain(models, scenes, chunk)
+            votes += sum(i["votes"] for i in infos)
+            lookups += sum(i["lookups"] for i in infos)
+        total += dt
+        log(f"{form} call {ci + 1}/{len(chunks)}: {len(chunk)} jobs in {dt:.1f}s")
+        for o in scenes.values():
+            o.close()
+    ctx.set_progress(None)
+    for m in models:
+        m.close()
+    return {"form": form, "jobs": len(jobs), "calls": len(chunks) if form == "batched" else len(jobs),
+            "seconds": total, "registrations_per_s": len(jobs) / total, "lookups": lookups, "votes": votes,
+            "votes_per_s": votes / total}
 
 
 def pct(x, q):
@@ -400,6 +447,9 @@ def main():
                     help="only the served form (and the plain pipeline serving the same arrivals)")
     ap.add_argument("--served-round-ms", type=float, default=20.0,
                     help="target duration of one vote round of the server")
+    ap.add_argument("--full-config", choices=("batched", "plain"),
+                    help="only all 148 jobs of the batched configuration through one form (about half "
+                         "an hour batched, longer plain)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     ap.add_argument("--valu-calibration", action="store_true",
@@ -428,6 +478,9 @@ def main():
     if args.dataset_like_only:
         print(json.dumps({"dataset_like": run_dataset_like(ppf_amd.Context(0), args.dataset_like_steps)}),
               flush=True)
+        return
+    if args.full_config:
+        print(json.dumps({"full_config": run_full_config(ppf_amd.Context(0), args.full_config)}), flush=True)
         return
     if args.large_hash_only:
         print(json.dumps({"large_hash": run_large_hash(ppf_amd.Context(0), args.large_hash_rpdf)}),
