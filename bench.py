@@ -143,7 +143,7 @@ def run_plain(models, scenes, jobs):
     return time.perf_counter() - t0, lat, infos
 
 
-def run_full_config(ctx, form, n_chunks=60):
+def run_full_config(ctx, form, n_chunks=60, part=(0, 1)):
     """All 148 jobs of the batched configuration (the headline subset is its 25
     smallest), through one form: `batched` -- ppf_batch_lookup over
     consecutive jobs in calls of about equal cost (sum of n_scene^2 * n_model;
@@ -164,6 +164,11 @@ def run_full_config(ctx, form, n_chunks=60):
         acc += c
     if cur:
         chunks.append(cur)
+    # part i of n: the i-th of n contiguous shares of the calls
+    i, n = part
+    lo, hi = len(chunks) * i // n, len(chunks) * (i + 1) // n
+    chunks = chunks[lo:hi]
+    jobs = [j for c in chunks for j in c]
     total, votes, lookups = 0.0, 0, 0
     if form == "plain":
         ctx.set_progress(lambda st, done, tot: log(f"  {st}: {done}/{tot}"), 4000)
@@ -184,7 +189,8 @@ def run_full_config(ctx, form, n_chunks=60):
     ctx.set_progress(None)
     for m in models:
         m.close()
-    return {"form": form, "jobs": len(jobs), "calls": len(chunks) if form == "batched" else len(jobs),
+    return {"form": form, "part": f"{i}/{n}", "jobs": len(jobs),
+            "calls": len(chunks) if form == "batched" else len(jobs),
             "seconds": total, "registrations_per_s": len(jobs) / total, "lookups": lookups, "votes": votes,
             "votes_per_s": votes / total}
 
@@ -447,6 +453,8 @@ def main():
                     help="only the served form (and the plain pipeline serving the same arrivals)")
     ap.add_argument("--served-round-ms", type=float, default=20.0,
                     help="target duration of one vote round of the server")
+    ap.add_argument("--full-part", default="0/1",
+                    help="with --full-config: run the i-th of n contiguous shares of its calls (i/n)")
     ap.add_argument("--full-config", choices=("batched", "plain"),
                     help="only all 148 jobs of the batched configuration through one form (about half "
                          "an hour batched, longer plain)")
@@ -480,7 +488,9 @@ def main():
               flush=True)
         return
     if args.full_config:
-        print(json.dumps({"full_config": run_full_config(ppf_amd.Context(0), args.full_config)}), flush=True)
+        part = tuple(int(x) for x in args.full_part.split("/"))
+        print(json.dumps({"full_config": run_full_config(ppf_amd.Context(0), args.full_config, part=part)}),
+              flush=True)
         return
     if args.large_hash_only:
         print(json.dumps({"large_hash": run_large_hash(ppf_amd.Context(0), args.large_hash_rpdf)}),
