@@ -8,7 +8,7 @@ plane with occlusion and noise, ref_point_df 5, threshold 0.3, GPU
 spatial-hash clustering.  The 148 (scene, model) jobs of that configuration
 hold about 1,000 times the votes of its 25 smallest (votes per lookup grow
 with the mean bucket, about n_model^2 / unique keys), far beyond a bench run
-(--full-config runs them all: about half an hour batched), so a step
+(--full-config runs them all: about an hour batched), so a step
 here is the fixed subset of jobs with n_scene * n_model <= 6e7
 (SUBSET_MAX_PRODUCT), in generator order, voted, thresholded, posed and
 clustered by ONE ppf_batch_lookup call.  value = registrations/s of the
@@ -147,9 +147,9 @@ def run_full_config(ctx, form, n_chunks=60, part=(0, 1)):
     """All 148 jobs of the batched configuration (the headline subset is its 25
     smallest), through one form: `batched` -- ppf_batch_lookup over
     consecutive jobs in calls of about equal cost (sum of n_scene^2 * n_model;
-    a heavier job is a call of its own), so that each call reports before the
-    next -- or `plain` -- one ppf_lookup per job.  Seconds exclude model builds
-    and scene uploads."""
+    a heavier job is a call of its own) -- or `plain` -- one ppf_lookup per
+    job.  Both vote in launches of 4,000 reference rows with a progress line
+    after each.  Seconds exclude model builds and scene uploads."""
     w = synth.batched()
     jobs = subset(w, float("inf"))
     models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=ctx) for m, d in zip(w.models, w.d_dists)]
@@ -170,8 +170,8 @@ def run_full_config(ctx, form, n_chunks=60, part=(0, 1)):
     chunks = chunks[lo:hi]
     jobs = [j for c in chunks for j in c]
     total, votes, lookups = 0.0, 0, 0
-    if form == "plain":
-        ctx.set_progress(lambda st, done, tot: log(f"  {st}: {done}/{tot}"), 4000)
+    # both forms vote in launches of 4,000 reference rows and report after each
+    ctx.set_progress(lambda st, done, tot: log(f"  {st}: {done}/{tot}") if st == "vote" else None, 4000)
     for ci, chunk in enumerate(chunks):
         scenes = {si: ppf_amd.Scene(w.scenes[si], 0.0, w.rpdf, ctx=ctx) for si in sorted({si for si, _ in chunk})}
         if form == "batched":
@@ -456,8 +456,8 @@ def main():
     ap.add_argument("--full-part", default="0/1",
                     help="with --full-config: run the i-th of n contiguous shares of its calls (i/n)")
     ap.add_argument("--full-config", choices=("batched", "plain"),
-                    help="only all 148 jobs of the batched configuration through one form (about half "
-                         "an hour batched, longer plain)")
+                    help="only all 148 jobs of the batched configuration through one form (about an "
+                         "hour batched, longer plain)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 256 MiB device copy kernel 3x (reference for traffic counters)")
     ap.add_argument("--valu-calibration", action="store_true",

@@ -60,8 +60,10 @@ uint64_t ppf_ctx_device_bytes(ppf_ctx ctx);
  * rows_per_launch reference rows (0: all rows in one launch) and calls
  * fn(user, stage, done, total) after each launch ("vote": rows voted of the
  * scene's reference rows) and after each later stage ("select", "poses",
- * "cluster": done = total = kept votes).  Results do not depend on it.  fn
- * runs on the calling thread; NULL turns reports off. */
+ * "cluster": done = total = kept votes).  ppf_batch_lookup likewise votes
+ * its jobs' rows in launches of about rows_per_launch rows and reports
+ * ("vote": rows voted of all jobs' reference rows) after each.  Results do
+ * not depend on it.  fn runs on the calling thread; NULL turns reports off. */
 typedef void (*ppf_progress_fn)(void *user, const char *stage, int64_t done, int64_t total);
 int ppf_ctx_set_progress(ppf_ctx ctx, ppf_progress_fn fn, void *user, int64_t rows_per_launch);
 
@@ -269,7 +271,7 @@ typedef struct {
   int64_t n_jobs;
   int64_t lookups;       /* sum over jobs */
   int64_t votes;         /* sum over jobs */
-  int32_t vote_launches; /* 1, or more when a job's candidate buffer overflowed */
+  int32_t vote_launches; /* 1, or more with progress reports or when a job's candidate buffer overflowed */
   int32_t lanes;         /* lanes that finished the jobs (0: finished together) */
   double vote_ms;        /* device time of the vote launches */
   double finish_ms;      /* wall time of threshold/order, poses and clustering */

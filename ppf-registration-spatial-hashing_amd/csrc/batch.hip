@@ -50,11 +50,17 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
   std::vector<uint32_t> mh(nj * 4, 0u);
   std::vector<size_t> todo(nj);
   std::iota(todo.begin(), todo.end(), (size_t)0);
+  // with progress reports (ppf_ctx_set_progress), the jobs' reference rows in
+  // slices of progress_rows, voted in launches of about that many rows, with
+  // a report after each launch; a job's slices share its candidates and
+  // running maximum, as the plain lookup's row launches do
+  const ppf_ctx_s *root = ctx->root;
+  const int64_t step = root->progress && root->progress_rows > 0 ? root->progress_rows : 0;
   for (int attempt = 0; !todo.empty(); ++attempt) {
-    std::vector<ppfk::VoteJob> vj;
-    uint64_t items = 0;
+    std::vector<ppfk::VoteJob> vj;  // one per (job, row slice), in launch order
+    std::vector<int64_t> rows;      // rows of each entry
     int max_tm = 0;
-    int64_t max_ns = 0;
+    int64_t max_ns = 0, total_rows = 0;
     for (size_t j : todo) {
       BatchJob &b = jobs[j];
       if (b.cand.bytes < (size_t)b.cap * 12) PPF_TRY(b.cand.alloc(ctx, (size_t)b.cap * 12));
@@ -62,22 +68,40 @@ int batch_votes(ppf_ctx ctx, std::vector<BatchJob> &jobs, bool exact, float *vot
       auto *cn = reinterpret_cast<uint32_t *>(cc + b.cap);
       ppfk::VoteJob J = vote_job(b.m, b.s, b.thr, cc, cn, md + 4 * j, b.cap);
       J.per_ref = b.per_ref;
-      J.item_begin = items;
-      items += (uint64_t)b.s->R;
+      const int64_t R = b.s->R, sl = step > 0 ? step : std::max<int64_t>(R, 1);
+      for (int64_t r0 = 0; r0 < R; r0 += sl) {
+        J.r_begin = (int32_t)r0;
+        vj.push_back(J);
+        rows.push_back(std::min(sl, R - r0));
+      }
+      total_rows += R;
       max_tm = std::max(max_tm, (int)b.m->TM);
       max_ns = std::max(max_ns, b.s->cloud.n);
-      vj.push_back(J);
       // preset: no candidates yet, running maximum from the previous pass
       mh[4 * j] = 0u;
       mh[4 * j + 2] = mh[4 * j + 3] = 0u;
     }
     PPF_HIP(hipMemcpyAsync(md, mh.data(), nj * 16, hipMemcpyHostToDevice, ctx->stream));
-    float ms = 0.0f;
-    bool g = want_graph;
-    PPF_TRY(ppfk::run_votes(ctx, vj.data(), (int)vj.size(), items, max_tm, max_ns, exact, &ms, 0, &g));
-    if (graphs) *graphs = *graphs && g;
-    *vote_ms += ms;
-    ++*launches;
+    int64_t done = 0;
+    for (size_t a = 0; a < vj.size();) {
+      // one launch: the whole list, or consecutive slices of about `step` rows
+      size_t e = a;
+      uint64_t items = 0;
+      while (e < vj.size() && (step == 0 || e == a || (int64_t)items + rows[e] <= step)) {
+        vj[e].item_begin = items;
+        items += (uint64_t)rows[e];
+        ++e;
+      }
+      float ms = 0.0f;
+      bool g = want_graph && a == 0 && e == vj.size();
+      PPF_TRY(ppfk::run_votes(ctx, vj.data() + a, (int)(e - a), items, max_tm, max_ns, exact, &ms, 0, &g));
+      if (graphs) *graphs = *graphs && g;
+      *vote_ms += ms;
+      ++*launches;
+      done += (int64_t)items;
+      if (step > 0) root->progress(root->progress_user, "vote", done, total_rows);
+      a = e;
+    }
     PPF_HIP(hipMemcpy(mh.data(), md, nj * 16, hipMemcpyDeviceToHost));
     std::vector<size_t> again;
     for (size_t j : todo) {

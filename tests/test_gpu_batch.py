@@ -243,6 +243,37 @@ def test_vote_graph_replays_equal_direct_launches():
         c.close()
 
 
+def test_batch_vote_progress_slices_equal_one_launch():
+    """With progress reports, ppf_batch_lookup votes its jobs' rows in
+    launches of about rows_per_launch rows (a job's slices share its
+    candidates and running maximum): the same results as one launch, and a
+    report after each launch up to all rows."""
+    c = ppf_amd.Context(0)
+    try:
+        w = _small_batch(seed=27)
+        models = [ppf_amd.Model(m, d, w.vote_count_threshold, ctx=c) for m, d in zip(w.models, w.d_dists)]
+        scenes = [ppf_amd.Scene(s, 0.0, w.rpdf, ctx=c) for s in w.scenes]
+        jm, js = _jobs(w, models, scenes)
+        one, oi = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
+        reports = []
+        c.set_progress(lambda st, done, tot: reports.append((st, done, tot)), 37)
+        try:
+            sliced, si = ppf_amd.batch_lookup(jm, js, w.vote_count_threshold, ctx=c)
+        finally:
+            c.set_progress(None)
+        votes = [r for r in reports if r[0] == "vote"]
+        assert si["vote_launches"] > 1 and oi["vote_launches"] == 1
+        assert len(votes) == si["vote_launches"]
+        assert all(a[1] < b[1] for a, b in zip(votes, votes[1:]))
+        assert votes[-1][1] == votes[-1][2]
+        for a, b in zip(sliced, one):
+            _same(a, b, True)
+        for o in one + sliced + models + scenes:
+            o.close()
+    finally:
+        c.close()
+
+
 def test_fused_finishing_with_empty_jobs(ctx):
     """Jobs without votes (a scene whose pairs match no model key) among jobs
     with votes, and a batch where no job keeps a vote: same status and
