@@ -1010,7 +1010,10 @@ int lookup_core(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *
                 (fixed_cap ? ", more than the fixed capacity of " + std::to_string(fixed_cap) : "") + ")");
       return PPF_ERR_NOMEM;
     }
-    if (attempt >= 1 && !fixed_cap) cap = round_cap(meta_h[0]);
+    // the rerun starts from the final maximum, so it emits only the kept
+    // votes, of which there are at most meta_h[0]: sized so, it needs no
+    // third pass (a fixed capacity reruns once at that capacity)
+    if (!fixed_cap) cap = round_cap(meta_h[0]);
   }
   const uint32_t n_cand = meta_h[0];
   r->info.lookups = (int64_t)s->R * (s->cloud.n - 1);

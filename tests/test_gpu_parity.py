@@ -485,15 +485,15 @@ def test_device_copy_kernel(ctx):
 
 
 def test_candidate_overflow_reruns_give_same_result(ctx):
-    """A candidate buffer far too small for the kept votes forces two reruns
-    of the vote pass (the second sized to an odd kept-vote count rounds up);
+    """A candidate buffer far too small for the kept votes forces a rerun of
+    the vote pass, sized to the first pass's candidate count (rounded up);
     results equal the default-capacity lookup, in the plain, batched and
     served forms."""
     w = synth.plain_small(seed=13, n_model=500, n_clutter=800)
     m = ppf_amd.Model(w.models[0], w.d_dists[0], w.vote_count_threshold, ctx=ctx)
     s = ppf_amd.Scene(w.scenes[0], 0.0, w.rpdf, ctx=ctx)
     ref = m.ppf_lookup(s)
-    assert ref.info["n_kept"] > 600  # > 512: the 257-entry buffer (rounded to 512) overflows twice
+    assert ref.info["n_kept"] > 600  # > 512: the 257-entry buffer (rounded to 512) overflows
     c0, n0 = ref.votes()
     small = m.ppf_lookup(s, candidate_capacity=257)
     c1, n1 = small.votes()
