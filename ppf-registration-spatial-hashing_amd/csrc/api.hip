@@ -960,7 +960,7 @@ int lookup_core(ppf_ctx ctx, ppf_model m, ppf_scene s, const ppf_lookup_params *
   // the final maximum preset, which emits exactly the kept votes.
   ppfi::DevBuf cand;
   // candidates per reference point are bounded by the pruning; start at
-  // 2048 per reference point (at least 8M), grow on overflow
+  // CAND_PER_REF per reference point (at least 8M), grow on overflow
   uint32_t cap = fixed_cap ? fixed_cap
                  : p->candidate_capacity
                      ? round_cap(p->candidate_capacity)

@@ -320,9 +320,12 @@ int debug_fma_stream(ppf_ctx ctx, float *out, int64_t blocks, int iters);
 }  // namespace ppfk
 
 namespace ppfi {
-// Candidate-buffer sizing shared by the plain and batched paths: (code, count)
-// pairs per reference point before growth on overflow.
-constexpr uint32_t CAND_PER_REF = 2048;
+// Candidate-buffer sizing shared by the plain, batched and served paths:
+// (code, count) pairs per reference point before growth on overflow.  An
+// overflow costs a second vote pass; a 75,366-point scene against an
+// 11,048-point model emits 2,209 per reference point in its first pass (0.2
+// per model point), so 4,096 leaves room for models up to about 20k points.
+constexpr uint32_t CAND_PER_REF = 4096;
 constexpr uint32_t CAND_CAP_MIN = 1u << 23;  // plain path: at least 8M candidates
 constexpr uint32_t CAND_CAP_MAX = 1u << 28;
 constexpr size_t CAND_HEADER = 256;  // counters ahead of the (code, count) arrays

@@ -78,7 +78,7 @@ def test_budget_cancel_and_errors(ctx):
     # budget: one single-target small request fits, the big one never does
     n_small = len(small[0])
     R_small = (n_small + 4) // 5
-    need_small = n_small * 28 + R_small * 64 + 512 + max(1 << 23, R_small * 2048) * 12 + 512
+    need_small = n_small * 28 + R_small * 64 + 512 + max(1 << 23, R_small * 4096) * 12 + 512
     srv = ppf_amd.Server(models, hbm_budget_bytes=need_small + 1024, round_ms=2.0, ctx=ctx)
     try:
         tb = srv.submit(big, [0], 5, w.vote_count_threshold)
