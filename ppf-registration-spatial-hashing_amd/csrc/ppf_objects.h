@@ -322,9 +322,10 @@ int debug_fma_stream(ppf_ctx ctx, float *out, int64_t blocks, int iters);
 namespace ppfi {
 // Candidate-buffer sizing shared by the plain, batched and served paths:
 // (code, count) pairs per reference point before growth on overflow.  An
-// overflow costs a second vote pass; a 75,366-point scene against an
-// 11,048-point model emits 2,209 per reference point in its first pass (0.2
-// per model point), so 4,096 leaves room for models up to about 20k points.
+// overflow costs a second vote pass.  First-pass candidates per reference
+// point depend on the model's bucket sizes: 1,205 for a 109,414-point scene
+// against a 19,073-point model, 2,209 for 75,366 against 11,048 points (both
+// one pass at 4,096), 9,394 for 45,069 against 16,863 points (two passes).
 constexpr uint32_t CAND_PER_REF = 4096;
 constexpr uint32_t CAND_CAP_MIN = 1u << 23;  // plain path: at least 8M candidates
 constexpr uint32_t CAND_CAP_MAX = 1u << 28;
