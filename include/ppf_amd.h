@@ -133,8 +133,9 @@ typedef struct {
   int32_t use_l1_norm;         /* 1: skip the translation test (kernel.cu:740-746) */
   int32_t use_averaged_clusters;
   int32_t exact_votes;         /* 1: evaluate every alpha bin with the full float path */
-  uint32_t candidate_capacity; /* initial (code, count) buffer capacity; 0 = automatic.  A
-                                  smaller buffer costs reruns of the vote pass, never results */
+  uint32_t candidate_capacity; /* initial (code, count) buffer capacity; 0 = automatic (4,096
+                                  per reference point, at least 8M).  A smaller buffer than
+                                  the vote pass emits costs one rerun of it, never results */
   int32_t per_reference_argmax; /* 1: keep only each reference point's best (model point,
                                    alpha) bin -- larger count, then smaller code -- before the
                                    threshold: the score pass that include/kernel.h:78-82
